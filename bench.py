@@ -1,0 +1,181 @@
+"""Benchmark of the game value-function solver on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload (the configuration the headline metric is quoted on): the 5D
+free-space reversible-car game, RelativeGrid(65, 36, 15, 15, 8.0) = 34,222,500
+nodes, A3 limits (configs/reversible_free_a3.json) with capture radius 0.5 m,
+ENO2 + local Lax-Friedrichs + RK2, FP32 local arithmetic, CFL 0.8.  A step is
+one full RK2 step of the whole grid; the default K = 810 steps is the whole
+3.0 s horizon.  The field is resident in HBM before the timed region; the
+timed region is exactly K steps bracketed by device synchronisation.
+
+Prints ONE JSON line: node-updates/s (whole job), the roofline of the
+dominant kernel (the RK2 blend stage) from per-launch hipEvents recorded on
+the library's own stream, and the oracle CPU baseline timed on a bounded
+sample of the same workload on this host's cores.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+from pathlib import Path
+import sys
+import time
+
+ROOT = Path(__file__).resolve().parent
+for p in (ROOT / "pursuit-evasion-core_amd", ROOT / "oracle"):
+    if str(p) not in sys.path:
+        sys.path.insert(0, str(p))
+
+METRIC = ("node-updates/s + time-to-converge at max|ΔV|≤1e-3: "
+          "MLP warm-start vs plain DP")
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per node of each RK2 stage launch:
+#   stage 1: read a, write b                      = 8 B
+#   stage 2: read b, read a (previous), write a   = 12 B
+STAGE_BYTES = (8, 12)
+
+
+def c1_config():
+    from pe_amd.reversible.free_dp import RelativeGrid, native_config
+    from pe_amd.reversible.model import ReversibleConfig, VehicleLimits
+
+    config = ReversibleConfig(
+        name="reversible_free_a3",
+        pursuer=VehicleLimits(-1.0, 2.0, 3.0, 1.0),
+        evader=VehicleLimits(-0.5, 1.0, 3.0, 2.0),
+        capture_radius=0.5,
+    )
+    grid = RelativeGrid(65, 36, 15, 15, 8.0)
+    return config, grid, native_config(config, grid, order=2, cfl=0.8)
+
+
+def load_traffic():
+    """HBM bytes per launch of the dominant kernel, from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled), if present."""
+    path = ROOT / "profiles" / "hji5d_blend_stage_c1_traffic.json"
+    if not path.exists():
+        return None
+    try:
+        return float(json.loads(path.read_text())["hbm_bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def cpu_baseline(cfg_native, sample_steps: int):
+    """The oracle (C/OpenMP restatement, FP64 local arithmetic) on this host."""
+    import oracle
+
+    threads = min(os.cpu_count() or 1, 16)
+    ocfg = oracle.HJI5DCfg(
+        cfg_native.positions, cfg_native.headings, cfg_native.p_speeds,
+        cfg_native.e_speeds, cfg_native.order, threads, 0, cfg_native.extent,
+        cfg_native.cfl, cfg_native.radius, cfg_native.p_min, cfg_native.p_max,
+        cfg_native.e_min, cfg_native.e_max, cfg_native.p_acceleration,
+        cfg_native.e_acceleration, cfg_native.p_curvature, cfg_native.e_curvature,
+    )
+    field, arrival = oracle.hji5d_init(ocfg)
+    dt = oracle.hji5d_cfl_limit(ocfg)
+    seconds = oracle.hji5d_time_steps(ocfg, field, arrival, dt, sample_steps)
+    n = field.size
+    return {
+        "value": n * sample_steps / seconds,
+        "unit": "node-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{sample_steps} RK2 steps of the same 34,222,500-node grid from phi "
+                  f"({seconds:.2f} s, FP64 local arithmetic, float storage, OpenMP)",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=810)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--profile-steps", type=int, default=100)
+    ap.add_argument("--cpu-sample-steps", type=int, default=16)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the 5D grid would shard by vP slabs with a +-2-plane halo; see "
+              "DESIGN.md).  Run with --gpus 1.", file=sys.stderr)
+        return 2
+    from pe_amd import _native
+
+    info = _native.device_info()
+    config, grid, cfg = c1_config()
+    n = grid.states
+    print(f"[bench] {info['name']}: C1 5D HJI {n} nodes, warmup {args.warmup}, "
+          f"steps {args.steps}", file=sys.stderr, flush=True)
+    with _native.HJI5D(cfg) as h:
+        dt = h.stats()["cfl_dt_limit"]
+        if args.warmup:
+            h.step(dt, args.warmup)
+        before = h.stats()["update_seconds"]
+        t0 = time.perf_counter()
+        h.step(dt, args.steps)  # returns after the device has finished
+        wall = time.perf_counter() - t0
+        device_seconds = h.stats()["update_seconds"] - before
+        print(f"[bench] timed {args.steps} steps: {wall:.4f} s wall, "
+              f"{device_seconds:.4f} s device", file=sys.stderr, flush=True)
+        # Roofline pass: per-launch hipEvents on the library's stream.
+        h.set_profiling(True)
+        h.step(dt, max(1, min(args.profile_steps, args.steps)))
+        st = h.stats()
+        h.set_profiling(False)
+        residual = st["last_residual"]
+    avg_ms = [st["stage_ms"][k] / max(1, st["stage_launches"][k]) for k in range(2)]
+    achieved = [STAGE_BYTES[k] * n / (avg_ms[k] * 1e-3) / 1e9 for k in range(2)]
+    traffic = load_traffic()
+    roofline = {
+        "bound": "hbm",
+        "kernel": "hji5d_stage<float,false,true> (RK2 blend stage)",
+        "achieved": round(achieved[1], 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved[1] / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+        "algorithmic_bytes_per_launch": STAGE_BYTES[1] * n,
+        "avg_launch_ms": round(avg_ms[1], 5),
+        "stage1": {"achieved": round(achieved[0], 1), "avg_launch_ms": round(avg_ms[0], 5),
+                   "algorithmic_bytes_per_launch": STAGE_BYTES[0] * n},
+    }
+    value = n * args.steps / wall
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "node-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "5D free-space reversible-car HJI (ENO2/LF/RK2), "
+                        "RelativeGrid(65,36,15,15,8.0), A3 limits, capture radius 0.5 m, "
+                        "CFL 0.8; one step = one RK2 step of all nodes",
+            "nodes": n,
+            "dt": dt,
+            "horizon_s": dt * args.steps,
+            "device_seconds": device_seconds,
+            "last_step_residual": residual,
+        },
+        "roofline": roofline,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_sample_steps)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,21 @@
+/* ORACLE — test infrastructure only (see the header of each .c file). */
+#ifndef PE_ORACLE_H
+#define PE_ORACLE_H
+#include <stddef.h>
+#include <stdint.h>
+
+/* 5D relative HJI options; same meaning as the relative5d argv flags of
+ * reversible/native_relative.cpp:236-259. */
+typedef struct {
+  int nx, nh, nvp, nve, order, threads, store_f64;
+  double extent, cfl, radius;
+  double vpmin, vpmax, vemin, vemax, ap, ae, kp, ke;
+} pe_oracle_hji5d_cfg;
+
+/* Vehicle model parameters [low, high, a, k, L, W, dt]
+ * (local_game/native_geometry.cpp:5-7,89-91). */
+typedef struct {
+  double low, high, a, k, L, W, dt;
+} pe_oracle_model;
+
+#endif

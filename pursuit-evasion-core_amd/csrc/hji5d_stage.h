@@ -1,0 +1,247 @@
+// One RK stage of the 5D relative ENO/Lax-Friedrichs HJI update, as a HIP
+// kernel template.  Included by two translation units:
+//   pe_hji5d.hip      R = float,  EXACT = false  (FP32 local arithmetic; the
+//                     performance path; FMA contraction allowed)
+//   pe_hji5d_f64.hip  R = double, EXACT = true   (FP64 local arithmetic in the
+//                     reference's operation order, built with
+//                     -ffp-contract=off, so fields match the reference CPU
+//                     solver bit for bit)
+//
+// Semantics follow reversible/native_relative.cpp:38-67 (slopes) and
+// :124-172 (Hamiltonian, dissipation, tube projection, RK blend, arrival),
+// with the local-exit clamp of reversible/gpu_relative.py:31,126-131.
+//
+// Work mapping.  Storage is [vP, vE, psi, y, x] with x fastest.  One thread
+// owns one (vE, psi, y, x) column and walks the vP axis (the slowest, stride
+// `sv`) keeping a five-value register window V(vP-2 .. vP+2), so vP
+// neighbours are loaded once per node instead of five times.  Blocks are
+// numbered with vE fastest and then remapped so that each of the 8 XCDs
+// processes one contiguous part of the grid: blocks resident together on an
+// XCD then share their vE / psi / y halos through that XCD's L2.
+#pragma once
+
+#include "pe_common.h"
+
+namespace pe {
+
+struct Hji5dGrid {
+  int nx, nh, nvp, nve;
+  int nx2;          // nx * nx
+  int plane;        // nh * nx * nx: one (vP, vE) slice
+  int tiles;        // ceil(plane / 256)
+  int order;        // 1 or 2
+  int exit_on;      // gpu_relative LOCAL_EXIT clamp
+  long long sp;     // psi stride = nx * nx
+  long long se;     // vE stride = plane
+  long long sv;     // vP stride = plane * nve
+  double extent, dx, dphi, dvp, dve;
+  double vpmin, vemin, ap, ae, kp, ke;
+  double exit_r2, exit_value;
+  int blocks;       // tiles * nve
+  int per_xcd;      // ceil(blocks / 8)
+};
+
+// Arguments of one stage launch.
+template <typename R>
+struct StageArgs {
+  const float *src;
+  float *dst;
+  float *arrival;
+  const R *sn;  // sin(psi_k), psi_k = -pi + k * dphi
+  const R *cs;  // cos(psi_k)
+  R dt;
+  double t;
+  unsigned long long *slot;  // [increase, |change|, nonfinite] of this step
+};
+
+// ENO1 / ENO2 one-sided differences along one axis
+// (native_relative.cpp:38-67).  m2..p2 are the values at offsets -2..+2
+// (unused ones may hold anything).
+template <typename R, bool EXACT>
+__device__ __forceinline__ void eno(R v, R m2, R m1, R p1, R p2, int pos,
+                                    int n, bool periodic, int order, R h,
+                                    R inv_h, R inv_hh, R &dm, R &dp) {
+  if (!periodic && pos == 0) {
+    dm = dp = EXACT ? (p1 - v) / h : (p1 - v) * inv_h;
+    return;
+  }
+  if (!periodic && pos == n - 1) {
+    dm = dp = EXACT ? (v - m1) / h : (v - m1) * inv_h;
+    return;
+  }
+  dm = EXACT ? (v - m1) / h : (v - m1) * inv_h;
+  dp = EXACT ? (p1 - v) / h : (p1 - v) * inv_h;
+  if (order == 1) return;
+  R sc, sl, sr;
+  if (EXACT) {
+    const R hh = h * h;
+    sc = (p1 - R(2) * v + m1) / hh;
+    sl = (periodic || pos >= 2) ? (v - R(2) * m1 + m2) / hh : sc;
+    sr = (periodic || pos + 2 < n) ? (p2 - R(2) * p1 + v) / hh : sc;
+  } else {
+    sc = (p1 - R(2) * v + m1) * inv_hh;
+    sl = (periodic || pos >= 2) ? (v - R(2) * m1 + m2) * inv_hh : sc;
+    sr = (periodic || pos + 2 < n) ? (p2 - R(2) * p1 + v) * inv_hh : sc;
+  }
+  // Strict comparisons (native_relative.cpp:65-66).
+  const R half_h = R(.5) * h;
+  dm += half_h * (fabs(sl) < fabs(sc) ? sl : sc);
+  dp -= half_h * (fabs(sc) < fabs(sr) ? sc : sr);
+}
+
+// Loads the four neighbours at offsets -2, -1, +1, +2 along a non-periodic
+// axis of length n (out-of-range ones are not read).
+__device__ __forceinline__ void axis4(const float *__restrict__ s, long long j,
+                                      long long stride, int pos, int n,
+                                      float &m2, float &m1, float &p1,
+                                      float &p2) {
+  m2 = pos >= 2 ? s[j - 2 * stride] : 0.f;
+  m1 = pos >= 1 ? s[j - stride] : 0.f;
+  p1 = pos + 1 < n ? s[j + stride] : 0.f;
+  p2 = pos + 2 < n ? s[j + 2 * stride] : 0.f;
+}
+
+template <typename R, bool EXACT, bool BLEND>
+__global__ __launch_bounds__(256) void hji5d_stage(StageArgs<R> a,
+                                                   Hji5dGrid g,
+                                                   int final_stage) {
+  // XCD-contiguous block numbering: hardware block b runs on XCD b % 8.
+  const int hw = blockIdx.x;
+  const int block = (hw & 7) * g.per_xcd + (hw >> 3);
+  R inc_max = R(0), chg_max = R(0);
+  bool nonfinite = false;
+  if (block < g.blocks) {
+    const int ie = block % g.nve;
+    const int tile = block / g.nve;
+    const int jp = tile * 256 + (int)threadIdx.x;
+    if (jp < g.plane) {
+      const int ih = jp / g.nx2;
+      const int r = jp - ih * g.nx2;
+      const int iy = r / g.nx;
+      const int ix = r - iy * g.nx;
+      const R dx = R(g.dx), dphi = R(g.dphi), dvp = R(g.dvp), dve = R(g.dve);
+      const R inv_dx = R(1) / dx, inv_dphi = R(1) / dphi,
+              inv_dvp = R(1) / dvp, inv_dve = R(1) / dve;
+      const R inv_dx2 = inv_dx * inv_dx, inv_dphi2 = inv_dphi * inv_dphi,
+              inv_dvp2 = inv_dvp * inv_dvp, inv_dve2 = inv_dve * inv_dve;
+      const R x = R(-g.extent) + R(ix) * dx;
+      const R y = R(-g.extent) + R(iy) * dx;
+      const R ve = R(g.vemin) + R(ie) * dve;
+      const R ap = R(g.ap), ae = R(g.ae), kp = R(g.kp);
+      const R we = fabs(ve) * R(g.ke);
+      const R cs = a.cs[ih], sn = a.sn[ih];
+      const R elow = ie == 0 ? R(0) : -ae;
+      const R ehigh = ie == g.nve - 1 ? R(0) : ae;
+      const bool outside =
+          g.exit_on && double(x * x + y * y) > g.exit_r2;
+      // psi neighbours (periodic).
+      const int hm1 = ih == 0 ? g.nh - 1 : ih - 1;
+      const int hm2 = ih >= 2 ? ih - 2 : ih - 2 + g.nh;
+      const int hp1 = ih == g.nh - 1 ? 0 : ih + 1;
+      const int hp2 = ih + 2 < g.nh ? ih + 2 : ih + 2 - g.nh;
+      const long long ohm1 = (long long)(hm1 - ih) * g.sp,
+                      ohm2 = (long long)(hm2 - ih) * g.sp,
+                      ohp1 = (long long)(hp1 - ih) * g.sp,
+                      ohp2 = (long long)(hp2 - ih) * g.sp;
+      const float *__restrict__ src = a.src;
+      long long j = (long long)ie * g.se + jp;  // node at vP index 0
+      // Register window along vP: w[0..4] = V(iv-2 .. iv+2).
+      float w0 = 0.f, w1 = 0.f, w2 = src[j];
+      float w3 = g.nvp > 1 ? src[j + g.sv] : 0.f;
+      float w4 = g.nvp > 2 ? src[j + 2 * g.sv] : 0.f;
+      for (int iv = 0; iv < g.nvp; iv++, j += g.sv) {
+        if (outside) {
+          a.dst[j] = float(g.exit_value);
+        } else {
+          const R vp = R(g.vpmin) + R(iv) * dvp;
+          const R wp = fabs(vp) * kp;
+          const R bx = ve * cs - vp, by = ve * sn;
+          float xm2, xm1, xp1, xp2, ym2, ym1, yp1, yp2, em2, em1, ep1, ep2;
+          axis4(src, j, 1, ix, g.nx, xm2, xm1, xp1, xp2);
+          axis4(src, j, g.nx, iy, g.nx, ym2, ym1, yp1, yp2);
+          axis4(src, j, g.se, ie, g.nve, em2, em1, ep1, ep2);
+          const float hvm2 = src[j + ohm2], hvm1 = src[j + ohm1],
+                      hvp1 = src[j + ohp1], hvp2 = src[j + ohp2];
+          const R v = R(w2);
+          R xm, xp, ym, yp, pm, pp, vm, vpl, em, ep;
+          eno<R, EXACT>(v, xm2, xm1, xp1, xp2, ix, g.nx, false, g.order, dx,
+                        inv_dx, inv_dx2, xm, xp);
+          eno<R, EXACT>(v, ym2, ym1, yp1, yp2, iy, g.nx, false, g.order, dx,
+                        inv_dx, inv_dx2, ym, yp);
+          eno<R, EXACT>(v, hvm2, hvm1, hvp1, hvp2, ih, g.nh, true, g.order,
+                        dphi, inv_dphi, inv_dphi2, pm, pp);
+          eno<R, EXACT>(v, w0, w1, w3, w4, iv, g.nvp, false, g.order, dvp,
+                        inv_dvp, inv_dvp2, vm, vpl);
+          eno<R, EXACT>(v, em2, em1, ep1, ep2, ie, g.nve, false, g.order, dve,
+                        inv_dve, inv_dve2, em, ep);
+          const R gx = R(.5) * (xm + xp), gy = R(.5) * (ym + yp),
+                  gp = R(.5) * (pm + pp), gv = R(.5) * (vm + vpl),
+                  ge = R(.5) * (em + ep);
+          const R plow = iv == 0 ? R(0) : -ap;
+          const R phigh = iv == g.nvp - 1 ? R(0) : ap;
+          // std::min(a, b) == (b < a ? b : a); std::max(a, b) == (a < b ? b : a)
+          const R pa = plow * gv, pb = phigh * gv;
+          const R ea = elow * ge, eb = ehigh * ge;
+          R H = bx * gx + by * gy - wp * fabs(y * gx - x * gy - gp) +
+                we * fabs(gp) + (pb < pa ? pb : pa) + (ea < eb ? eb : ea);
+          H += R(.5) * ((fabs(bx) + wp * fabs(y)) * (xp - xm) +
+                        (fabs(by) + wp * fabs(x)) * (yp - ym) +
+                        (wp + we) * (pp - pm) + ap * (vpl - vm) +
+                        ae * (ep - em));
+          const R old = v;
+          const R previous = BLEND ? R(a.dst[j]) : old;
+          R value = old + a.dt * (H < R(0) ? H : R(0));
+          if (BLEND) value = R(.5) * (previous + value);
+          nonfinite |= !isfinite(value);
+          const float stored = float(value);
+          a.dst[j] = stored;
+          if (final_stage) {
+            const R inc = R(stored) - previous;
+            inc_max = inc > inc_max ? inc : inc_max;
+            chg_max = fabs(inc) > chg_max ? fabs(inc) : chg_max;
+            // Arrival time by linear zero crossing (native_relative.cpp:166-168).
+            if (previous > R(0) && stored <= 0.f) {
+              const float old_arrival = a.arrival[j];
+              if (isnan(old_arrival))
+                a.arrival[j] = float(a.t + double(a.dt) * double(previous) /
+                                              (double(previous) - double(stored)));
+            }
+          }
+        }
+        // Slide the vP window.
+        w0 = w1;
+        w1 = w2;
+        w2 = w3;
+        w3 = w4;
+        w4 = iv + 3 < g.nvp ? src[j + 3 * g.sv] : 0.f;
+      }
+    }
+  }
+  if (!final_stage) return;  // uniform across the launch
+  if (nonfinite) atomicMax(a.slot + 2, 1ull);
+  __shared__ R red[2][4];
+  const R i_w = wave_max(inc_max), c_w = wave_max(chg_max);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[0][wave] = i_w;
+    red[1][wave] = c_w;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    R i_b = red[0][0], c_b = red[1][0];
+    for (int k = 1; k < 4; k++) {
+      i_b = red[0][k] > i_b ? red[0][k] : i_b;
+      c_b = red[1][k] > c_b ? red[1][k] : c_b;
+    }
+    raise_max(a.slot + 0, double(i_b));
+    raise_max(a.slot + 1, double(c_b));
+  }
+}
+
+// Host launchers, one per translation unit.
+void hji5d_launch_f32(const Hji5dGrid &g, const StageArgs<float> &a,
+                      bool blend, bool final_stage, hipStream_t stream);
+void hji5d_launch_f64(const Hji5dGrid &g, const StageArgs<double> &a,
+                      bool blend, bool final_stage, hipStream_t stream);
+
+}  // namespace pe

@@ -1,0 +1,238 @@
+"""GPU parity of the 5D relative HJI path (through the C ABI).
+
+* float64 local arithmetic: bit-identical to the reference binary's golden
+  vectors (fields, final, arrival) on C0 and the reference unit grid.
+* float32 local arithmetic (the performance path): against the same golden
+  vectors and the FP64-storage oracle.  Tolerance: max|dV| <= 1e-3 and
+  mean|dV| <= 1e-5.  The scheme's ENO2 selection (strict "<") turns rounding
+  into O(h * second difference) jumps at exact ties, 4e-4 at most on C0 even
+  between float and double storage of the CPU reference itself.  Capture-set
+  membership (V <= 0) must agree except at nodes whose reference value lies
+  within that tolerance of zero; those nodes are printed.
+* C1 at full size (34,222,500 nodes, 810 RK2 steps): size-independent
+  properties (monotone in time, V <= phi, arrival set == {V <= 0}, mirror
+  symmetry) plus a bounded-sample comparison with the oracle.
+"""
+
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from pe_amd import _native
+from pe_amd.reversible.free_dp import RelativeGrid, native_config, solve
+from pe_amd.reversible.model import ReversibleConfig, VehicleLimits
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+TOL_MAX, TOL_MEAN = 1e-3, 1e-5
+
+
+def config_from_case(case, radius=None):
+    ph = case["physics"]
+    return ReversibleConfig(
+        pursuer=VehicleLimits(ph["p_min"], ph["p_max"], ph["p_acceleration"], ph["p_curvature"]),
+        evader=VehicleLimits(ph["e_min"], ph["e_max"], ph["e_acceleration"], ph["e_curvature"]),
+        capture_radius=ph["radius"] if radius is None else radius,
+    )
+
+
+def load_case(name):
+    z = np.load(GOLDEN / f"hji5d_{name}.npz")
+    case = json.loads(str(z["case"]))
+    grid = RelativeGrid(case["positions"], case["headings"], case["p_speeds"],
+                        case["e_speeds"], case["extent"])
+    return z, case, grid
+
+
+def run(tmp_path, name, precision):
+    z, case, grid = load_case(name)
+    print(f"[gpu] 5D solve {name} {precision}", flush=True)
+    meta = solve(config_from_case(case), grid, tmp_path / name, horizon=case["horizon"],
+                 output_dt=case["output_dt"], order=case["order"], cfl=case["cfl"],
+                 local_precision=precision)
+    out = tmp_path / name
+    n = grid.states
+    fields = np.fromfile(out / "fields.f32", np.float32).reshape(-1, n)
+    final = np.fromfile(out / "final.f32", np.float32)
+    arrival = np.fromfile(out / "arrival.f32", np.float32)
+    return z, case, meta, fields, final, arrival
+
+
+def u32(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+@pytest.mark.parametrize("name", ["c0_a3_17x12x7x7", "unit_9x8x7x7_order1", "unit_9x8x7x7_order2"])
+def test_float64_local_is_bit_exact_with_reference_binary(gpu, tmp_path, name):
+    z, case, meta, fields, final, arrival = run(tmp_path, name, "float64")
+    assert meta["status"] == "complete"
+    assert meta["internal_steps"] == int(z["internal_steps"])
+    assert meta["frames"] == int(z["frames"])
+    assert meta["maximum_increase"] == float(z["maximum_increase"])
+    np.testing.assert_array_equal(u32(final), u32(z["final"]))
+    stored = fields[1:-1] if name.startswith("c0") else fields
+    np.testing.assert_array_equal(u32(stored), u32(z["fields"]))
+    np.testing.assert_array_equal(np.isnan(arrival), np.isnan(z["arrival"]))
+    np.testing.assert_array_equal(u32(np.nan_to_num(arrival)), u32(np.nan_to_num(z["arrival"])))
+
+
+def capture_disagreement(v, ref, tol):
+    """Nodes whose capture membership (V <= 0) differs; all must lie within
+    `tol` of the boundary in the reference."""
+    diff = np.flatnonzero((v <= 0) != (ref <= 0))
+    near = np.abs(ref[diff]) <= tol
+    return diff, near
+
+
+@pytest.mark.parametrize("name", ["c0_a3_17x12x7x7", "unit_9x8x7x7_order2"])
+def test_float32_local_within_tolerance_of_reference_binary(gpu, tmp_path, name):
+    z, case, meta, fields, final, arrival = run(tmp_path, name, "float32")
+    ref = z["final"].astype(np.float64)
+    d = np.abs(final - ref)
+    print(f"[{name}] float32 vs reference: max|dV|={d.max():.3e} mean|dV|={d.mean():.3e}")
+    assert d.max() <= TOL_MAX and d.mean() <= TOL_MEAN
+    diff, near = capture_disagreement(final, ref, TOL_MAX)
+    print(f"[{name}] capture-set disagreements: {len(diff)} (listed): {diff.tolist()[:50]}")
+    assert near.all()
+    assert meta["maximum_increase"] <= 1e-6
+    assert np.diff(fields, axis=0).max() <= 1e-6
+
+
+def a3(radius=0.3):
+    return ReversibleConfig(
+        name="reversible_free_a3",
+        pursuer=VehicleLimits(-1.0, 2.0, 3.0, 1.0),
+        evader=VehicleLimits(-0.5, 1.0, 3.0, 2.0),
+        capture_radius=radius,
+    )
+
+
+def test_float32_against_fp64_storage_oracle_mid_grid(gpu, tmp_path, oracle_lib):
+    """A grid larger than the golden ones: GPU FP32 vs the FP64-storage
+    oracle (double everywhere), states the FP32 error."""
+    grid = RelativeGrid(33, 24, 9, 9, 4.0)
+    cfg = oracle_lib.HJI5DCfg(33, 24, 9, 9, 2, 16, 1, 4.0, 0.8, 0.3,
+                              -1.0, 2.0, -0.5, 1.0, 3.0, 3.0, 1.0, 2.0)
+    ref = oracle_lib.hji5d_solve(cfg, 0.5, 0.25, history=False)
+    print("[gpu] 5D mid grid float32", flush=True)
+    meta = solve(a3(), grid, tmp_path / "mid", horizon=0.5, output_dt=0.25, history=False)
+    final = np.fromfile(tmp_path / "mid" / "final.f32", np.float32)
+    d = np.abs(final - ref["final"])
+    print(f"[33x24x9x9] float32 vs fp64 oracle: max|dV|={d.max():.3e} mean|dV|={d.mean():.3e}")
+    assert meta["internal_steps"] == ref["internal_steps"]
+    assert d.max() <= TOL_MAX and d.mean() <= TOL_MEAN
+    diff, near = capture_disagreement(final, ref["final"], TOL_MAX)
+    print(f"capture-set disagreements: {len(diff)}: {diff.tolist()[:50]}")
+    assert near.all()
+
+
+C1 = RelativeGrid(65, 36, 15, 15, 8.0)
+
+
+def test_c1_full_size_properties(gpu, oracle_lib):
+    """34.2M nodes, horizon 3.0 s (810 RK2 steps), FP32 local."""
+    cfg = native_config(a3(0.5), C1, order=2, cfl=0.8)
+    print("[gpu] C1 34.2M nodes, 810 steps", flush=True)
+    with _native.HJI5D(cfg) as h:
+        phi, arr0 = h.read()
+        for k in range(1, 31):
+            h.advance_to(min(3.0, k * 0.1))
+        st = h.stats()
+        final, arrival = h.read()
+        trace = h.trace()
+    assert st["internal_steps"] == 810
+    assert st["time"] == pytest.approx(3.0, abs=1e-12)
+    assert st["maximum_increase"] <= 1e-6
+    assert (final <= phi + 1e-6).all()
+    assert (final[phi <= 0] <= 0).all()
+    np.testing.assert_array_equal(np.isfinite(arrival), final <= 0)
+    f = final.reshape(C1.shape)
+    nh = C1.headings
+    mirrored = f[:, :, (-np.arange(nh)) % nh][:, :, :, ::-1, :]
+    asym = np.abs(f - mirrored)
+    print(f"C1 mirror asymmetry max={asym.max():.3e} mean={asym.mean():.3e}; "
+          f"captured fraction {np.mean(final <= 0):.4f}; last residual {trace[-1]:.3e}")
+    assert asym.max() < 5e-3 and asym.mean() < 1e-5
+    assert len(trace) == 810 and np.all(trace >= 0)
+
+
+def test_c1_bounded_sample_against_oracle(gpu, oracle_lib):
+    """Three full RK2 steps of C1 from phi: GPU FP32 vs the oracle (FP64
+    local arithmetic, float storage) on all 34.2M nodes."""
+    ocfg = oracle_lib.HJI5DCfg(65, 36, 15, 15, 2, 16, 0, 8.0, 0.8, 0.5,
+                               -1.0, 2.0, -0.5, 1.0, 3.0, 3.0, 1.0, 2.0)
+    field, arrival = oracle_lib.hji5d_init(ocfg)
+    dt = oracle_lib.hji5d_cfl_limit(ocfg)
+    oracle_lib.hji5d_time_steps(ocfg, field, arrival, dt, 3)
+    cfg = native_config(a3(0.5), C1, order=2, cfl=0.8)
+    print("[gpu] C1 3 steps vs oracle", flush=True)
+    with _native.HJI5D(cfg) as h:
+        assert h.stats()["cfl_dt_limit"] == pytest.approx(dt, rel=1e-15)
+        h.step(dt, 3)
+        gpu_field, gpu_arrival = h.read()
+    d = np.abs(gpu_field.astype(np.float64) - field)
+    print(f"C1 3 steps: max|dV|={d.max():.3e} mean|dV|={d.mean():.3e}")
+    assert d.max() <= TOL_MAX and d.mean() <= 1e-6
+    np.testing.assert_array_equal(np.isfinite(gpu_arrival), np.isfinite(arrival))
+
+
+def test_float64_local_matches_oracle_bitwise_on_c1_sample(gpu, oracle_lib):
+    ocfg = oracle_lib.HJI5DCfg(65, 36, 15, 15, 2, 16, 0, 8.0, 0.8, 0.5,
+                               -1.0, 2.0, -0.5, 1.0, 3.0, 3.0, 1.0, 2.0)
+    field, arrival = oracle_lib.hji5d_init(ocfg)
+    dt = oracle_lib.hji5d_cfl_limit(ocfg)
+    oracle_lib.hji5d_time_steps(ocfg, field, arrival, dt, 2)
+    cfg = native_config(a3(0.5), C1, order=2, cfl=0.8, local_precision="float64")
+    print("[gpu] C1 2 steps float64 vs oracle", flush=True)
+    with _native.HJI5D(cfg) as h:
+        h.step(dt, 2)
+        gpu_field, _ = h.read()
+    np.testing.assert_array_equal(u32(gpu_field), u32(field))
+
+
+def test_residual_iteration_past_horizon(gpu):
+    cfg = native_config(a3(), RelativeGrid(17, 12, 7, 7, 4.0), order=2, cfl=0.8)
+    print("[gpu] residual iteration", flush=True)
+    with _native.HJI5D(cfg) as h:
+        h.advance_to(0.2)
+        converged = h.run_to_residual(1e-4, 20000, 32)
+        st = h.stats()
+        trace = h.trace()
+    assert converged and st["last_residual"] <= 1e-4
+    assert len(trace) == st["internal_steps"]
+
+
+def test_exit_radius_and_rectangle_capture_initialisation(gpu):
+    from pe_amd.local_game.geometry import Footprint, rectangle_separation
+
+    grid = RelativeGrid(17, 12, 7, 7, 4.0)
+    rect = ReversibleConfig(point_vehicles=False, capture_rule="rectangle_contact",
+                            boundary_rule="collision_constrained")
+    feet = (Footprint(0.5, 0.3), Footprint(0.5, 0.3))
+    cfg = native_config(rect, grid, exit_radius=3.0, capture_footprints=feet)
+    print("[gpu] rectangle capture + exit radius", flush=True)
+    with _native.HJI5D(cfg) as h:
+        phi, arrival = h.read()
+        h.advance_to(0.1)
+        after, _ = h.read()
+    _, _, psi, y, x = np.meshgrid(*grid.axes(rect), indexing="ij")
+    e = np.stack([x, y, psi], -1)
+    expect = rectangle_separation(np.zeros(3), e, *feet)
+    expect = np.where(x * x + y * y > 9.0, 3.0, expect).astype(np.float32).ravel()
+    np.testing.assert_allclose(phi, expect, atol=1e-6)
+    outside = (x * x + y * y > 9.0).ravel()
+    assert (after[outside] == 3.0).all()
+    assert (after <= phi + 1e-6).all()
+
+
+def test_profiling_counts_launches(gpu):
+    cfg = native_config(a3(), RelativeGrid(17, 12, 7, 7, 4.0))
+    with _native.HJI5D(cfg) as h:
+        h.set_profiling(True)
+        h.step(0.01, 5)
+        st = h.stats()
+    assert st["stage_launches"] == [5, 5]
+    assert all(ms > 0 for ms in st["stage_ms"])
