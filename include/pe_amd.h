@@ -28,6 +28,9 @@ const char *pe_last_error(void);
 /* ABI version of this header (PE_AMD_ABI). */
 #define PE_AMD_ABI 1
 int pe_abi_version(void);
+/* sizeof of a struct of this header by name ("pe_sl_car", ...), or -1;
+   lets bindings check their struct mirrors without a device. */
+int pe_abi_struct_size(const char *name);
 /* Name, compute units and HBM bytes of the current device. */
 int pe_device_info(char *name, int name_len, int32_t *compute_units,
                    int64_t *hbm_bytes);
@@ -55,7 +58,7 @@ typedef struct pe_hji5d_config {
   double exit_radius; /* > 0: local exit clamp of gpu_relative.py:126-131 */
   int32_t capture;    /* 0: center distance; 1: rectangle contact
                          (gpu_relative.py:161-166)                          */
-  int32_t reserved;
+  int32_t vp_chunk;   /* vP values walked per GPU thread (0: automatic)    */
   double p_length, p_width, e_length, e_width; /* rectangle footprints */
 } pe_hji5d_config;
 
@@ -97,6 +100,177 @@ int pe_hji5d_get_stats(pe_hji5d *h, pe_hji5d_stats *out);
 /* Per-launch hipEvent timing of the stage kernels on the handle's stream. */
 int pe_hji5d_set_profiling(pe_hji5d *h, int32_t on);
 void pe_hji5d_destroy(pe_hji5d *h);
+
+/* ======================================================================
+ * Contact-game geometry on the GPU (double precision).
+ *
+ * Same argument meaning as the reference's OpenMP library
+ * local_game/native_geometry.cpp: states [n,4] = x, y, heading, signed
+ * speed; obstacles [nc,3] = circle x, y, r; half = square room half-width;
+ * param[7] = speed low, speed high, acceleration, curvature, body length,
+ * body width, control period.  No thread count; a status is returned.
+ * ====================================================================== */
+/* batch_clearance, native_geometry.cpp:92-99 */
+int pe_geom_clearance(const double *states, int64_t n, const double *obstacles,
+                      int32_t nc, double half, const double *param,
+                      double *out);
+/* batch_backups, native_geometry.cpp:100-107 */
+int pe_geom_backups(const double *states, int64_t n, const double *obstacles,
+                    int32_t nc, double half, const double *param,
+                    uint8_t *out);
+/* batch_actions, native_geometry.cpp:108-123: ok [n,na], next [n,na,4] */
+int pe_geom_actions(const double *states, int64_t n, const double *controls,
+                    int32_t na, double dt, const double *obstacles, int32_t nc,
+                    double half, const double *param, uint8_t *ok,
+                    double *next);
+/* configuration_connections, native_geometry.cpp:237-252 */
+int pe_geom_connections(const double *from, const double *to, int64_t n,
+                        const double *obstacles, int32_t nc, double half,
+                        const double *param, uint8_t *out);
+
+/* ======================================================================
+ * Compact 8D semi-Lagrangian min-max backup of the contact game.
+ *
+ * Value V over the dense product of the two cars' legal states, row-major
+ * [p.n, e.n] float32.  One Jacobi sweep:
+ *   V'(ip,ie) = min(V, min_{u allowed} max_{b allowed}
+ *               sum_q wE[ie,b,q] sum_k wP[ip,u,k] V[idsP[ip,u,k], idsE[ie,b,q]])
+ * for pairs inside the exit radius; other pairs keep their (exit) value.
+ *
+ * Replaces the CuPy RawKernels initialize/interpolate_p/reduce_e and
+ * init_packed/p_packed/e_packed of local_game/grid_game.py:199-231,389-422
+ * and their drivers backup_gpu / backup_packed (:286-333, :511-581), called
+ * per sweep by local_game/reference.py:352-358.  Several maps of the same
+ * action count solve together in one handle (one launch per kernel).
+ * ====================================================================== */
+typedef struct pe_sl_car {
+  int32_t n;                  /* legal states (VehicleGrid.states)          */
+  int32_t actions;            /* commands per state (9)                     */
+  const double *states;       /* [n,4] x, y, heading, speed                 */
+  const int32_t *successors;  /* [n,actions,16] compact stencil ids         */
+  const float *weights;       /* [n,actions,16] stencil weights (>= 0)      */
+  const uint8_t *allowed;     /* [n,actions] command admissible             */
+  const int32_t *position;    /* [n] index of the state's (x, y) cell       */
+  int32_t positions;          /* number of (x, y) cells                     */
+  const double *clearance;    /* [n] obstacle clearance (map feature of the
+                                 learned warm start); may be NULL           */
+} pe_sl_car;
+
+typedef struct pe_sl_game {
+  pe_sl_car p, e;
+  /* [p.positions, e.positions]: 1 where the centre distance <= exit radius
+     (PairDomain.build, grid_game.py:350-367) */
+  const uint8_t *inside;
+  /* [p.positions, e.positions]: 1 where an inside pair's E successor can
+     lie (distance <= exit radius + E successor reach); 0 skips work */
+  const uint8_t *near;
+  float exit_value;           /* value outside the radius (exit_radius)     */
+  float p_length, p_width, e_length, e_width; /* rectangle footprints       */
+} pe_sl_game;
+
+#define PE_SL_EARLY_EXIT 1 /* stop a command once worst >= best (exact)     */
+#define PE_SL_ORDERED 2    /* try last sweep's argmin/argmax commands first */
+#define PE_SL_GRAPH 4      /* replay captured sweeps with a hipGraph        */
+
+typedef struct pe_sl_stats {
+  int64_t sweeps;
+  int32_t maps;
+  int32_t reserved;
+  int64_t pairs;          /* stored pairs, all maps                         */
+  int64_t active_pairs;   /* pairs inside the exit radius, all maps         */
+  double update_seconds;  /* device time of the sweeps                      */
+  double last_residual;   /* max over maps of the last sweep's max|dV|      */
+  double p_ms, e_ms;      /* profiled kernel time (summed)                  */
+  int64_t p_launches, e_launches;
+} pe_sl_stats;
+
+typedef struct pe_sl pe_sl;
+
+int pe_sl_create(const pe_sl_game *maps, int32_t n_maps, pe_sl **out);
+/* v0 == NULL: the rectangle-contact payoff inside, exit value outside
+   (init_packed, grid_game.py:390-398); else the concatenated [p.n*e.n]
+   fields of all maps (a warm start). */
+int pe_sl_initialize(pe_sl *h, const float *v0);
+int pe_sl_sweep(pe_sl *h, int32_t sweeps, int32_t flags);
+/* Sweeps until every map's residual max|dV| <= tol (checked every
+   check_every sweeps) or max_sweeps. */
+int pe_sl_run(pe_sl *h, double tol, int32_t max_sweeps, int32_t check_every,
+              int32_t flags, int32_t *sweeps_done, int32_t *converged);
+int pe_sl_read(pe_sl *h, int32_t map, float *field);
+/* Field of `map` at n continuous pair states given as the P and E stencils
+   (ids [n,16] into the cars' legal states, weights [n,16]):
+   out[i] = sum_a pw[i,a] sum_b ew[i,b] V[pi[i,a], ei[i,b]]
+   (query_field kernel, local_game/reference.py:127-135). */
+int pe_sl_query(pe_sl *h, int32_t map, int64_t n, const int32_t *p_ids,
+                const float *p_weights, const int32_t *e_ids,
+                const float *e_weights, float *out);
+/* Residual of every recorded sweep, [sweeps, maps] row-major. */
+int pe_sl_trace(pe_sl *h, double *out, int64_t capacity, int64_t *count);
+int pe_sl_get_stats(pe_sl *h, pe_sl_stats *out);
+int pe_sl_set_profiling(pe_sl *h, int32_t on);
+void pe_sl_destroy(pe_sl *h);
+
+/* ======================================================================
+ * Learned value surrogate: MLP with BF16 MFMA kernels (FP32 accumulation).
+ *
+ * No counterpart in the reference for this network; the reference's own
+ * learned value nets (local_game/learning.py, training.py) use PyTorch.
+ * Input per point: 8 state coordinates [xP, yP, thP, vP, xE, yE, thE, vE]
+ * then `extra` map features; features = normalised state, `rff` random
+ * Fourier features (sin, cos), extras, zero padding to a multiple of 32.
+ * Loss: weighted MSE with weight 1 + 4 exp(-|y| / 0.2)
+ * (local_game/training.py:351-352); optimiser AdamW (training.py:185-188).
+ * ====================================================================== */
+typedef struct pe_mlp_config {
+  int32_t extra;        /* map features after the 8 state coordinates (<= 8) */
+  int32_t rff;          /* random Fourier features (sin + cos), even        */
+  int32_t hidden;       /* hidden width, multiple of 64                     */
+  int32_t layers;       /* hidden SiLU layers                               */
+  uint32_t seed;        /* initialisation and Fourier projection seed      */
+  float rff_scale;      /* std of the Fourier projection                    */
+  float state_shift[8]; /* normalisation: (x - shift) * scale               */
+  float state_scale[8];
+  float learning_rate, weight_decay, beta1, beta2, eps; /* AdamW */
+} pe_mlp_config;
+
+typedef struct pe_mlp pe_mlp;
+int pe_mlp_create(const pe_mlp_config *cfg, pe_mlp **out);
+int pe_mlp_param_count(pe_mlp *h, int64_t *count);
+/* Flattened FP32 parameters: per hidden layer W [out, in] then b [out];
+   then the head w [hidden], b [1]. */
+int pe_mlp_get_params(pe_mlp *h, float *out, int64_t count);
+int pe_mlp_set_params(pe_mlp *h, const float *in, int64_t count);
+/* Predictions for n points of raw input [n, 8 + extra]. */
+int pe_mlp_forward(pe_mlp *h, const float *raw, int64_t n, float *out);
+/* The bf16 feature rows [n, width] the first layer sees, as float. */
+int pe_mlp_features(pe_mlp *h, const float *raw, int64_t n, float *features,
+                    int32_t *width);
+/* Loss and parameter gradients (flattened like the parameters) of one
+   batch, computed by the forward and backward MFMA kernels. */
+int pe_mlp_gradients(pe_mlp *h, const float *raw, const float *target,
+                     int64_t n, float *grads, float *loss);
+/* Device-resident training set, then `steps` AdamW steps on batches drawn
+   on the device; losses[steps] (may be NULL). */
+int pe_mlp_set_dataset(pe_mlp *h, const float *raw, const float *target,
+                       int64_t n);
+int pe_mlp_train(pe_mlp *h, int32_t steps, int32_t batch, uint32_t seed,
+                 float *losses);
+/* The GEMM kernel alone on bf16 bit patterns: C = A [M,K] x B, with B given
+   [K,N] (transposed_b = 0) or [N,K] (transposed_b = 1).  For tests. */
+int pe_mlp_gemm_check(const uint16_t *a, const uint16_t *b, int32_t M,
+                      int32_t N, int32_t K, int32_t transposed_b, float *c);
+void pe_mlp_destroy(pe_mlp *h);
+
+/* Warm start of every map of a solver from the network:
+   V0 = min(phi, mlp(state, phi, clearance_P, clearance_E) + delta) on the
+   pairs inside the exit radius (the network needs extra = 3); the control
+   order is reset so that the first sweep records it from V0.  No
+   counterpart in the reference. */
+int pe_sl_warm_start(pe_sl *sl, pe_mlp *mlp, float delta);
+/* Raw network inputs [count, 11] of pairs [first, first + count) of a map
+   (row-major pair index ip * e.n + ie), for building training sets. */
+int pe_sl_pair_inputs(pe_sl *sl, int32_t map, int64_t first, int64_t count,
+                      float *raw);
 
 #ifdef __cplusplus
 }

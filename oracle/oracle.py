@@ -107,9 +107,20 @@ def _declare(L):
 
 
 def _declare_8d(L):
-    """Declarations of the 8D contact-game oracle (filled in by sl8d.c)."""
-    if not hasattr(L, "pe_oracle_sl_sweep"):
-        return
+    """Declarations of the 8D contact-game oracle (sl8d.c, geometry.c)."""
+    I64, I32, D = ctypes.c_int64, ctypes.c_int, ctypes.c_double
+    L.pe_oracle_batch_clearance.argtypes = [P, I64, P, I32, D, P, P, I32]
+    L.pe_oracle_batch_backups.argtypes = [P, I64, P, I32, D, P, P, I32]
+    L.pe_oracle_batch_actions.argtypes = [P, I64, P, I32, D, P, I32, D, P, P, P, I32]
+    L.pe_oracle_batch_connections.argtypes = [P, P, I64, P, I32, D, P, P, I32]
+    for name in ("clearance", "backups", "actions", "connections"):
+        getattr(L, "pe_oracle_batch_" + name).restype = None
+    L.pe_oracle_contact.argtypes = [P, P, D, D, D, D]
+    L.pe_oracle_contact.restype = D
+    L.pe_oracle_step.argtypes = [P, D, D, D, P, P]
+    L.pe_oracle_step.restype = None
+    L.pe_oracle_sl_init.argtypes = [I64, I64, P, P, P, D, D, D, D, D, P, I32]
+    L.pe_oracle_sl_init.restype = None
     L.pe_oracle_sl_sweep.restype = ctypes.c_double
     L.pe_oracle_sl_sweep.argtypes = [
         ctypes.c_int64,  # np
@@ -216,3 +227,82 @@ def hji5d_time_steps(cfg: HJI5DCfg, field, arrival, dt, count, t=0.0) -> float:
     return lib().pe_oracle_hji5d_steps(
         ctypes.byref(cfg), _ptr(field), _ptr(arrival), dt, t, count
     )
+
+
+# ------------------------------------------------------------------ 8D game
+
+def model_params(limits, body, dt):
+    """[low, high, a, k, L, W, dt] (local_game/native.py:141-153)."""
+    return np.array([limits.speed_min, limits.speed_max, limits.acceleration,
+                     limits.max_curvature, body.length, body.width, dt], np.float64)
+
+
+def _states(s):
+    s = np.ascontiguousarray(np.asarray(s, np.float64).reshape(-1, 4))
+    return s
+
+
+def _obs(obstacles):
+    return np.ascontiguousarray(np.asarray(obstacles, np.float64).reshape(-1, 3))
+
+
+def geom_clearance(states, obstacles, half, params, threads=None):
+    s, o = _states(states), _obs(obstacles)
+    out = np.empty(len(s))
+    lib().pe_oracle_batch_clearance(_ptr(s), len(s), _ptr(o), len(o), half,
+                                    _ptr(params), _ptr(out), threads or os.cpu_count())
+    return out
+
+
+def geom_backups(states, obstacles, half, params, threads=None):
+    s, o = _states(states), _obs(obstacles)
+    out = np.empty(len(s), np.uint8)
+    lib().pe_oracle_batch_backups(_ptr(s), len(s), _ptr(o), len(o), half,
+                                  _ptr(params), _ptr(out), threads or os.cpu_count())
+    return out.astype(bool)
+
+
+def geom_actions(states, controls, dt, obstacles, half, params, threads=None):
+    s, o = _states(states), _obs(obstacles)
+    u = np.ascontiguousarray(np.asarray(controls, np.float64).reshape(-1, 2))
+    ok = np.empty((len(s), len(u)), np.uint8)
+    nxt = np.empty((len(s), len(u), 4))
+    lib().pe_oracle_batch_actions(_ptr(s), len(s), _ptr(u), len(u), dt, _ptr(o), len(o),
+                                  half, _ptr(params), _ptr(ok), _ptr(nxt),
+                                  threads or os.cpu_count())
+    return ok.astype(bool), nxt
+
+
+def geom_connections(start, target, obstacles, half, params, threads=None):
+    a, b, o = _states(start), _states(target), _obs(obstacles)
+    out = np.empty(len(a), np.uint8)
+    lib().pe_oracle_batch_connections(_ptr(a), _ptr(b), len(a), _ptr(o), len(o), half,
+                                      _ptr(params), _ptr(out), threads or os.cpu_count())
+    return out.astype(bool)
+
+
+def sl_init(pstates, estates, inside, exit_value, pbody, ebody, threads=None):
+    ps, es = _states(pstates), _states(estates)
+    ins = np.ascontiguousarray(inside, np.uint8)
+    V = np.empty((len(ps), len(es)))
+    lib().pe_oracle_sl_init(len(ps), len(es), _ptr(ps), _ptr(es), _ptr(ins), exit_value,
+                            pbody.length, pbody.width, ebody.length, ebody.width, _ptr(V),
+                            threads or os.cpu_count())
+    return V
+
+
+def sl_sweep(V, p, e, inside, threads=None):
+    """One FP64 Jacobi sweep.  p / e: objects with successors [n,A,16] int32,
+    weights [n,A,16] float32 and allowed [n,A] bool.  Returns (out, change)."""
+    V = np.ascontiguousarray(V, np.float64)
+    out = np.empty_like(V)
+    arrs = []
+    for g in (p, e):
+        arrs += [np.ascontiguousarray(g.successors, np.int32),
+                 np.ascontiguousarray(g.weights, np.float32),
+                 np.ascontiguousarray(g.allowed, np.uint8)]
+    ins = np.ascontiguousarray(inside, np.uint8)
+    change = lib().pe_oracle_sl_sweep(
+        V.shape[0], V.shape[1], p.successors.shape[1], e.successors.shape[1],
+        *[_ptr(a) for a in arrs], _ptr(ins), _ptr(V), _ptr(out), threads or os.cpu_count())
+    return out, change

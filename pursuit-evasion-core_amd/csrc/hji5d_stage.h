@@ -12,12 +12,13 @@
 // with the local-exit clamp of reversible/gpu_relative.py:31,126-131.
 //
 // Work mapping.  Storage is [vP, vE, psi, y, x] with x fastest.  One thread
-// owns one (vE, psi, y, x) column and walks the vP axis (the slowest, stride
-// `sv`) keeping a five-value register window V(vP-2 .. vP+2), so vP
-// neighbours are loaded once per node instead of five times.  Blocks are
-// numbered with vE fastest and then remapped so that each of the 8 XCDs
-// processes one contiguous part of the grid: blocks resident together on an
-// XCD then share their vE / psi / y halos through that XCD's L2.
+// owns one (vE, psi, y, x) column and walks a chunk of `vpc` values of the
+// vP axis (the slowest, stride `sv`) keeping a five-value register window
+// V(vP-2 .. vP+2), so vP neighbours are loaded once per node instead of five
+// times; shorter chunks trade that reuse for more threads in flight.  Blocks
+// are numbered with the vP chunk and vE fastest, then remapped so that each
+// of the 8 XCDs processes one contiguous part of the grid: blocks resident
+// together on an XCD share their vE / psi / y halos through its L2.
 #pragma once
 
 #include "pe_common.h"
@@ -37,7 +38,9 @@ struct Hji5dGrid {
   double extent, dx, dphi, dvp, dve;
   double vpmin, vemin, ap, ae, kp, ke;
   double exit_r2, exit_value;
-  int blocks;       // tiles * nve
+  int vpc;          // vP values per thread
+  int nvpc;         // ceil(nvp / vpc): vP chunks per column
+  int blocks;       // tiles * nve * nvpc
   int per_xcd;      // ceil(blocks / 8)
 };
 
@@ -111,8 +114,11 @@ __global__ __launch_bounds__(256) void hji5d_stage(StageArgs<R> a,
   R inc_max = R(0), chg_max = R(0);
   bool nonfinite = false;
   if (block < g.blocks) {
-    const int ie = block % g.nve;
-    const int tile = block / g.nve;
+    const int chunk = block % g.nvpc;
+    const int ie = (block / g.nvpc) % g.nve;
+    const int tile = block / (g.nvpc * g.nve);
+    const int iv0 = chunk * g.vpc;
+    const int iv1 = iv0 + g.vpc < g.nvp ? iv0 + g.vpc : g.nvp;
     const int jp = tile * 256 + (int)threadIdx.x;
     if (jp < g.plane) {
       const int ih = jp / g.nx2;
@@ -144,12 +150,14 @@ __global__ __launch_bounds__(256) void hji5d_stage(StageArgs<R> a,
                       ohp1 = (long long)(hp1 - ih) * g.sp,
                       ohp2 = (long long)(hp2 - ih) * g.sp;
       const float *__restrict__ src = a.src;
-      long long j = (long long)ie * g.se + jp;  // node at vP index 0
+      long long j = (long long)iv0 * g.sv + (long long)ie * g.se + jp;
       // Register window along vP: w[0..4] = V(iv-2 .. iv+2).
-      float w0 = 0.f, w1 = 0.f, w2 = src[j];
-      float w3 = g.nvp > 1 ? src[j + g.sv] : 0.f;
-      float w4 = g.nvp > 2 ? src[j + 2 * g.sv] : 0.f;
-      for (int iv = 0; iv < g.nvp; iv++, j += g.sv) {
+      float w0 = iv0 >= 2 ? src[j - 2 * g.sv] : 0.f;
+      float w1 = iv0 >= 1 ? src[j - g.sv] : 0.f;
+      float w2 = src[j];
+      float w3 = iv0 + 1 < g.nvp ? src[j + g.sv] : 0.f;
+      float w4 = iv0 + 2 < g.nvp ? src[j + 2 * g.sv] : 0.f;
+      for (int iv = iv0; iv < iv1; iv++, j += g.sv) {
         if (outside) {
           a.dst[j] = float(g.exit_value);
         } else {

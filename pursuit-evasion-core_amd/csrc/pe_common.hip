@@ -33,6 +33,24 @@ extern "C" const char *pe_last_error(void) { return pe::g_error.c_str(); }
 
 extern "C" int pe_abi_version(void) { return PE_AMD_ABI; }
 
+extern "C" int pe_abi_struct_size(const char *name) {
+  if (!name) return -1;
+  const struct {
+    const char *name;
+    int size;
+  } table[] = {
+      {"pe_hji5d_config", (int)sizeof(pe_hji5d_config)},
+      {"pe_hji5d_stats", (int)sizeof(pe_hji5d_stats)},
+      {"pe_sl_car", (int)sizeof(pe_sl_car)},
+      {"pe_sl_game", (int)sizeof(pe_sl_game)},
+      {"pe_sl_stats", (int)sizeof(pe_sl_stats)},
+      {"pe_mlp_config", (int)sizeof(pe_mlp_config)},
+  };
+  for (const auto &t : table)
+    if (std::strcmp(t.name, name) == 0) return t.size;
+  return -1;
+}
+
 extern "C" int pe_device_info(char *name, int name_len, int32_t *compute_units,
                               int64_t *hbm_bytes) {
   int dev = 0;

@@ -120,7 +120,7 @@ static int validate(const pe_hji5d_config &c) {
       c.e_acceleration <= 0 || c.p_curvature <= 0 || c.e_curvature <= 0 ||
       c.p_min >= c.p_max || c.e_min >= c.e_max ||
       (c.precision != 0 && c.precision != 1) ||
-      (c.capture != 0 && c.capture != 1))
+      (c.capture != 0 && c.capture != 1) || c.vp_chunk < 0)
     return fail(PE_ERR_INVALID, "Invalid solver options");
   if (c.exit_radius > 0 && !(c.exit_radius < c.extent))
     return fail(PE_ERR_INVALID,
@@ -130,7 +130,9 @@ static int validate(const pe_hji5d_config &c) {
     return fail(PE_ERR_INVALID, "Rectangle capture requires positive footprints");
   const double n = double(c.positions) * c.positions * c.headings *
                    c.p_speeds * c.e_speeds;
-  if (n > 4.0e9 || double(c.headings) * c.positions * c.positions > 2.0e9)
+  if (n > 4.0e9 || double(c.headings) * c.positions * c.positions > 2.0e9 ||
+      double(c.headings) * c.positions * c.positions / 256.0 * c.e_speeds *
+              c.p_speeds > 2.0e9)
     return fail(PE_ERR_INVALID, "Grid too large for 32-bit plane indexing");
   return PE_OK;
 }
@@ -277,7 +279,11 @@ extern "C" int pe_hji5d_create(const pe_hji5d_config *cfg, pe_hji5d **out) {
   g.exit_on = c.exit_radius > 0;
   g.exit_r2 = c.exit_radius * c.exit_radius;
   g.exit_value = c.exit_radius;
-  g.blocks = g.tiles * g.nve;
+  // vP values per thread: by default the whole axis (fastest on C1 of the
+  // chunk lengths 1, 2, 3, 5, 8 and 15).
+  g.vpc = c.vp_chunk > 0 ? std::min(c.vp_chunk, g.nvp) : g.nvp;
+  g.nvpc = (g.nvp + g.vpc - 1) / g.vpc;
+  g.blocks = g.tiles * g.nve * g.nvpc;
   g.per_xcd = (g.blocks + 7) / 8;
   h->n = g.sv * g.nvp;
   {  // native_relative.cpp:76-81

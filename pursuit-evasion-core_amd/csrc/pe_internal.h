@@ -1,0 +1,98 @@
+// Internal state of the pe_amd handles, shared by the translation units that
+// operate on more than one handle kind (the learned warm start).
+#pragma once
+
+#include <vector>
+
+#include "pe_common.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+namespace pe {
+
+constexpr int kStencil = 16;
+constexpr int kMaxActions = 15;  // order byte packs two 4-bit command indices
+
+struct SlMap {
+  int np, ne, na, npos_e;
+  long long v_off;      // pair offset of this map in the value arrays
+  long long g_off;      // offset in G
+  long long tab_off;    // offset in the inside / near tables
+  int p_row, e_row;     // first state row in the concatenated car tables
+  float exit_value, pl, pw, el, ew;
+};
+
+struct MlpShape {
+  int raw;      // raw inputs per point: 8 state + extra
+  int extra;    // extra map features
+  int rff;      // sin/cos pairs count * 2
+  int width;    // padded feature width (multiple of 32)
+  int hidden;
+};
+
+// Forward pass of n points whose raw inputs are already in h->raw; the
+// predictions land in h->yhat (both device buffers of the handle).
+int mlp_forward_loaded(struct ::pe_mlp *h, long long n);
+int mlp_reserve(struct ::pe_mlp *h, long long n);
+
+}  // namespace pe
+
+struct pe_sl {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int B = 0, na = 0;
+  std::vector<pe::SlMap> maps;
+  pe::SlMap *d_maps = nullptr;
+  int np_max = 0, ne_max = 0;
+  long long pairs = 0, g_size = 0, active = 0;
+  // car tables
+  int *pids = nullptr, *eids = nullptr, *ppos = nullptr, *epos = nullptr;
+  float *pw = nullptr, *ew = nullptr;
+  uint8_t *pallowed = nullptr, *eallowed = nullptr, *inside = nullptr,
+          *near = nullptr, *order = nullptr;
+  double *pstates = nullptr, *estates = nullptr;
+  float *pclear = nullptr, *eclear = nullptr;  // per-state clearance (may be zero)
+  float *va = nullptr, *vb = nullptr, *G = nullptr;
+  // residual ring [slots][B] and graph slots [graph_sweeps][B]
+  unsigned long long *ring = nullptr, *gslots = nullptr;
+  int ring_slots = 256, ring_used = 0;
+  std::vector<double> trace;  // [sweeps][B]
+  double last = 0, update_seconds = 0;
+  long long sweeps = 0;
+  bool profiling = false;
+  std::vector<hipEvent_t> events;
+  std::vector<int> kinds;
+  double ms[2] = {0, 0};
+  long long launches[2] = {0, 0};
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  // captured sweeps
+  hipGraphExec_t graph = nullptr;
+  int graph_sweeps = 0, graph_flags = -1;
+  const float *graph_src = nullptr;  // field the captured sweeps start from
+};
+
+struct pe_mlp {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  pe_mlp_config cfg{};
+  pe::MlpShape shape{};
+  int layers = 0;                   // hidden layers (MFMA GEMMs)
+  std::vector<int> fan_in, fan_out; // per GEMM layer
+  // parameters: per layer W [out, in] fp32 master + bf16 + bf16^T, b [out]
+  std::vector<float *> W, b, gW, gb, mW, vW, mb, vb;
+  std::vector<__bf16 *> Wh, Wt;
+  float *w_head = nullptr, *b_head = nullptr, *gw_head = nullptr, *gb_head = nullptr;
+  float *mw_head = nullptr, *vw_head = nullptr, *mb_head = nullptr, *vb_head = nullptr;
+  float *shift = nullptr, *scale = nullptr, *proj = nullptr;
+  // activations for batches of up to cap points
+  long long cap = 0;
+  float *raw = nullptr, *yt = nullptr, *yhat = nullptr, *dy = nullptr, *loss = nullptr;
+  __bf16 *F = nullptr;
+  std::vector<__bf16 *> Hs, dZ;  // per hidden layer
+  std::vector<float *> Zs;
+  // dataset
+  float *data_x = nullptr, *data_y = nullptr;
+  long long data_n = 0;
+  long long step = 0;
+};

@@ -1,0 +1,795 @@
+// Learned value surrogate: a small MLP trained and evaluated with
+// hand-written BF16 MFMA kernels (FP32 accumulation) on gfx950.
+//
+// Network (input -> 4 x 256 SiLU -> 1):
+//   features f(x) = [xn (8), sin(B xn) (16), cos(B xn) (16), extra (3), 0 pad]
+//   with xn the normalised 8D pair state [xP, yP, thP, vP, xE, yE, thE, vE],
+//   B a fixed Gaussian matrix (random Fourier features) and `extra` the
+//   per-node map features (contact payoff phi, P clearance, E clearance);
+//   h1 = silu(W0 f + b0), h_{l+1} = silu(W_l h_l + b_l), y = w4 . h4 + b4.
+// Weights are kept as FP32 masters (AdamW) with BF16 copies [out, in] and
+// transposed BF16 copies [in, out] for the MFMA operands.
+//
+// GEMM: C[m, n] = sum_k A(m, k) B(k, n) with arbitrary strides, one
+// 64 x 64 output tile per 256-thread block; each wave owns a 32 x 32 quarter
+// as 2 x 2 tiles of v_mfma_f32_16x16x32_bf16.  K advances 32 at a time
+// through LDS, where both operands are stored k-contiguous so that each
+// lane's MFMA operand (8 consecutive k) is one 16-byte LDS read.
+//
+// MFMA 16x16x32 bf16 operand layout (wave64), lane l:
+//   A fragment: row (l % 16), k = 8 * (l / 16) .. +7
+//   B fragment: col (l % 16), k = 8 * (l / 16) .. +7
+//   D fragment: col (l % 16), rows 4 * (l / 16) .. +3
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "pe_internal.h"
+
+
+namespace pe {
+
+constexpr int kTile = 64;   // output tile edge per block
+constexpr int kStep = 32;   // K per LDS stage (one MFMA)
+constexpr int kPad = 8;     // LDS row padding (bf16) against bank conflicts
+
+__device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }
+__device__ __forceinline__ float silu(float z) { return z / (1.f + __expf(-z)); }
+__device__ __forceinline__ float silu_grad(float z) {
+  const float s = 1.f / (1.f + __expf(-z));
+  return s * (1.f + z * (1.f - s));
+}
+
+enum Epilogue : int {
+  kStoreF32 = 0,     // C (float) = acc (+ beta * C)
+  kBiasSilu = 1,     // Z = acc + bias (float); H = silu(Z) (bf16)
+  kSiluGrad = 2,     // G (bf16) = acc * silu'(Z[m, n])
+};
+
+struct GemmArgs {
+  const __bf16 *A;
+  long long sam, sak;  // A(m, k) = A[m * sam + k * sak]
+  const __bf16 *B;
+  long long sbk, sbn;  // B(k, n) = B[k * sbk + n * sbn]
+  int M, N, K;
+  int epilogue;
+  float beta;          // kStoreF32: C = acc + beta * C
+  float *C;            // [M, N] float (kStoreF32 output, kBiasSilu Z out)
+  long long ldc;
+  const float *bias;   // kBiasSilu
+  __bf16 *H;           // kBiasSilu / kSiluGrad bf16 output [M, N]
+  long long ldh;
+  const float *Z;      // kSiluGrad pre-activations [M, N]
+  long long ldz;
+};
+
+__global__ __launch_bounds__(256) void mfma_gemm(GemmArgs g) {
+  __shared__ __attribute__((aligned(16))) __bf16 As[kTile][kStep + kPad];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[kTile][kStep + kPad];
+  const int m0 = blockIdx.x * kTile, n0 = blockIdx.y * kTile;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;  // wave's quarter
+  f32x4 acc[2][2];
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const bool a_vec = g.sak == 1 && (g.sam % 8) == 0;
+  const bool b_vec = g.sbk == 1 && (g.sbn % 8) == 0;
+  for (int k0 = 0; k0 < g.K; k0 += kStep) {
+    // Stage A(m0 + r, k0 + c) and B(k0 + c, n0 + r) into LDS, k-contiguous.
+    // 64 rows x 32 k = 2048 values per operand; 8 per thread.
+    {
+      const int r = tid >> 2, c = (tid & 3) * 8;  // row, first k of 8
+      const int m = m0 + r, n = n0 + r;
+      if (a_vec && m < g.M && k0 + c + 8 <= g.K) {
+        *reinterpret_cast<bf16x8 *>(&As[r][c]) =
+            *reinterpret_cast<const bf16x8 *>(g.A + m * g.sam + (k0 + c));
+      } else {
+        for (int t = 0; t < 8; t++) {
+          const int k = k0 + c + t;
+          As[r][c + t] = (m < g.M && k < g.K) ? g.A[m * g.sam + k * g.sak]
+                                              : to_bf16(0.f);
+        }
+      }
+      if (b_vec && n < g.N && k0 + c + 8 <= g.K) {
+        *reinterpret_cast<bf16x8 *>(&Bs[r][c]) =
+            *reinterpret_cast<const bf16x8 *>(g.B + n * g.sbn + (k0 + c));
+      } else {
+        for (int t = 0; t < 8; t++) {
+          const int k = k0 + c + t;
+          Bs[r][c + t] = (n < g.N && k < g.K) ? g.B[k * g.sbk + n * g.sbn]
+                                              : to_bf16(0.f);
+        }
+      }
+    }
+    __syncthreads();
+    const int fr = lane & 15, fk = (lane >> 4) * 8;
+    bf16x8 a[2], b[2];
+    for (int i = 0; i < 2; i++)
+      a[i] = *reinterpret_cast<const bf16x8 *>(&As[wm + 16 * i + fr][fk]);
+    for (int j = 0; j < 2; j++)
+      b[j] = *reinterpret_cast<const bf16x8 *>(&Bs[wn + 16 * j + fr][fk]);
+    for (int i = 0; i < 2; i++)
+      for (int j = 0; j < 2; j++)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    __syncthreads();
+  }
+  // Epilogue: lane holds D[4 * (lane / 16) + v][lane % 16] of each tile.
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++)
+      for (int v = 0; v < 4; v++) {
+        const int m = m0 + wm + 16 * i + 4 * (lane >> 4) + v;
+        const int n = n0 + wn + 16 * j + (lane & 15);
+        if (m >= g.M || n >= g.N) continue;
+        const float x = acc[i][j][v];
+        if (g.epilogue == kStoreF32) {
+          float *c = g.C + m * g.ldc + n;
+          *c = g.beta != 0.f ? x + g.beta * *c : x;
+        } else if (g.epilogue == kBiasSilu) {
+          const float z = x + g.bias[n];
+          if (g.C) g.C[m * g.ldc + n] = z;
+          g.H[m * g.ldh + n] = to_bf16(silu(z));
+        } else {
+          g.H[m * g.ldh + n] = to_bf16(x * silu_grad(g.Z[m * g.ldz + n]));
+        }
+      }
+}
+
+// ---------------------------------------------------------------- features
+
+
+
+// xn = (x - shift) * scale for the 8 state coordinates; features into F.
+__global__ __launch_bounds__(256) void mlp_features(
+    const float *__restrict__ raw, long long n, MlpShape s,
+    const float *__restrict__ shift, const float *__restrict__ scale,
+    const float *__restrict__ proj /* [rff/2, 8] */, __bf16 *__restrict__ F) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float *x = raw + i * s.raw;
+  float xn[8];
+  for (int d = 0; d < 8; d++) xn[d] = (x[d] - shift[d]) * scale[d];
+  __bf16 *f = F + i * s.width;
+  int c = 0;
+  for (int d = 0; d < 8; d++) f[c++] = to_bf16(xn[d]);
+  const int half = s.rff / 2;
+  for (int q = 0; q < half; q++) {
+    float t = 0.f;
+    for (int d = 0; d < 8; d++) t = fmaf(proj[q * 8 + d], xn[d], t);
+    f[c + q] = to_bf16(__sinf(t));
+    f[c + half + q] = to_bf16(__cosf(t));
+  }
+  c += s.rff;
+  for (int d = 0; d < s.extra; d++) f[c++] = to_bf16(x[8 + d]);
+  for (; c < s.width; c++) f[c] = to_bf16(0.f);
+}
+
+// y[i] = w . h[i] + b  (final 256 -> 1 layer); one wave per point.
+__global__ __launch_bounds__(256) void mlp_head(const __bf16 *__restrict__ H,
+                                                long long n, int width,
+                                                const float *__restrict__ w,
+                                                const float *__restrict__ b,
+                                                float *__restrict__ y) {
+  const long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (i >= n) return;
+  float t = 0.f;
+  for (int k = lane; k < width; k += 64) t = fmaf(w[k], (float)H[i * width + k], t);
+  for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+  if (lane == 0) y[i] = t + b[0];
+}
+
+// Weighted MSE on the head output: weight 1 + 4 exp(-|y| / 0.2) (the
+// reference's training weighting, local_game/training.py:351-352).
+// dy = 2 w (yhat - y) / n; loss accumulated (sum of w (yhat - y)^2 / n).
+__global__ __launch_bounds__(256) void mlp_loss(const float *__restrict__ yhat,
+                                                const float *__restrict__ y,
+                                                long long n, float *__restrict__ dy,
+                                                float *__restrict__ loss) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  float l = 0.f;
+  if (i < n) {
+    const float w = 1.f + 4.f * __expf(-fabsf(y[i]) / 0.2f);
+    const float r = yhat[i] - y[i];
+    dy[i] = 2.f * w * r / (float)n;
+    l = w * r * r / (float)n;
+  }
+  for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off, 64);
+  if ((threadIdx.x & 63) == 0 && l != 0.f) atomicAdd(loss, l);
+}
+
+// Head backward: dw4 = sum_i dy_i h_i, db4 = sum dy, and
+// dZ3[i, k] = dy_i w4[k] silu'(Z3[i, k]) (bf16).
+__global__ __launch_bounds__(256) void mlp_head_back(
+    const __bf16 *__restrict__ H, const float *__restrict__ Z, long long n,
+    int width, const float *__restrict__ w, const float *__restrict__ dy,
+    float *__restrict__ dw, float *__restrict__ db, __bf16 *__restrict__ dZ) {
+  const int k = blockIdx.x;  // one block per hidden unit
+  float t = 0.f, tb = 0.f;
+  for (long long i = threadIdx.x; i < n; i += blockDim.x) {
+    t = fmaf(dy[i], (float)H[i * width + k], t);
+    tb += dy[i];
+    dZ[i * width + k] = to_bf16(dy[i] * w[k] * silu_grad(Z[i * width + k]));
+  }
+  __shared__ float red[2][4];
+  for (int off = 32; off > 0; off >>= 1) {
+    t += __shfl_xor(t, off, 64);
+    tb += __shfl_xor(tb, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = t;
+    red[1][threadIdx.x >> 6] = tb;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    dw[k] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+    if (k == 0) db[0] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+  }
+}
+
+// db[n] = sum_m dZ[m, n] (bf16 dZ), one block per column.
+__global__ __launch_bounds__(256) void column_sum(const __bf16 *__restrict__ X,
+                                                  long long m, int width,
+                                                  float *__restrict__ out) {
+  const int n = blockIdx.x;
+  float t = 0.f;
+  for (long long i = threadIdx.x; i < m; i += blockDim.x) t += (float)X[i * width + n];
+  __shared__ float red[4];
+  for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) out[n] = red[0] + red[1] + red[2] + red[3];
+}
+
+// AdamW on the FP32 masters; refreshes the BF16 copy and its transpose.
+__global__ __launch_bounds__(256) void adamw(float *__restrict__ p,
+                                             const float *__restrict__ g,
+                                             float *__restrict__ m1,
+                                             float *__restrict__ m2, long long n,
+                                             float lr, float b1, float b2,
+                                             float eps, float wd, float c1,
+                                             float c2, float clip,
+                                             __bf16 *__restrict__ copy,
+                                             __bf16 *__restrict__ copy_t,
+                                             int rows, int cols) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i] * clip;
+  m1[i] = b1 * m1[i] + (1.f - b1) * gi;
+  m2[i] = b2 * m2[i] + (1.f - b2) * gi * gi;
+  const float mh = m1[i] / c1, vh = m2[i] / c2;
+  const float x = p[i] * (1.f - lr * wd) - lr * mh / (sqrtf(vh) + eps);
+  p[i] = x;
+  if (copy) copy[i] = to_bf16(x);
+  if (copy_t) {
+    const int r = (int)(i / cols), c = (int)(i % cols);
+    copy_t[(long long)c * rows + r] = to_bf16(x);
+  }
+}
+
+__global__ __launch_bounds__(256) void gather_rows(const float *__restrict__ src,
+                                                   const float *__restrict__ ysrc,
+                                                   int width, long long n_src,
+                                                   long long n, unsigned seed,
+                                                   unsigned step,
+                                                   float *__restrict__ dst,
+                                                   float *__restrict__ ydst) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  // splitmix-style hash of (seed, step, i) -> sample index
+  unsigned long long z = ((unsigned long long)seed << 32) ^ ((unsigned long long)step * 0x9E3779B97F4A7C15ull) ^ (unsigned long long)i;
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  const long long j = (long long)(z % (unsigned long long)n_src);
+  for (int c = 0; c < width; c++) dst[i * width + c] = src[j * width + c];
+  ydst[i] = ysrc[j];
+}
+
+__global__ void fill_f32(float *p, long long n, float v) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    p[i] = v;
+}
+
+}  // namespace pe
+
+using namespace pe;
+
+
+
+static int launch_gemm(pe_mlp *h, const GemmArgs &g) {
+  const dim3 grid((unsigned)((g.M + kTile - 1) / kTile), (unsigned)((g.N + kTile - 1) / kTile));
+  hipLaunchKernelGGL(mfma_gemm, grid, dim3(256), 0, h->stream, g);
+  PE_LAUNCHED(mfma_gemm);
+  return PE_OK;
+}
+
+// Forward of n points already in h->raw; fills F, Hs, Zs (if keep) and yhat.
+static int forward(pe_mlp *h, long long n, bool keep) {
+  const MlpShape &s = h->shape;
+  hipLaunchKernelGGL(mlp_features, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     h->stream, h->raw, n, s, h->shift, h->scale, h->proj, h->F);
+  PE_LAUNCHED(mlp_features);
+  const __bf16 *x = h->F;
+  int width = s.width;
+  for (int l = 0; l < h->layers; l++) {
+    GemmArgs g{};
+    g.A = x;
+    g.sam = width;
+    g.sak = 1;
+    g.B = h->Wh[l];         // B(k, n) = W[n, k]
+    g.sbk = 1;
+    g.sbn = h->fan_in[l];
+    g.M = (int)n;
+    g.N = h->fan_out[l];
+    g.K = h->fan_in[l];
+    g.epilogue = kBiasSilu;
+    g.C = keep ? h->Zs[l] : nullptr;
+    g.ldc = h->fan_out[l];
+    g.bias = h->b[l];
+    g.H = h->Hs[l];
+    g.ldh = h->fan_out[l];
+    int rc = launch_gemm(h, g);
+    if (rc) return rc;
+    x = h->Hs[l];
+    width = h->fan_out[l];
+  }
+  hipLaunchKernelGGL(mlp_head, dim3((unsigned)((n * 64 + 255) / 256)), dim3(256), 0,
+                     h->stream, x, n, width, h->w_head, h->b_head, h->yhat);
+  PE_LAUNCHED(mlp_head);
+  return PE_OK;
+}
+
+// Gradients of the weighted MSE of n points in raw / yt into gW, gb, ...
+static int backward(pe_mlp *h, long long n, float *loss_host) {
+  int rc = forward(h, n, true);
+  if (rc) return rc;
+  hipLaunchKernelGGL(fill_f32, dim3(1), dim3(64), 0, h->stream, h->loss, 1ll, 0.f);
+  hipLaunchKernelGGL(mlp_loss, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream,
+                     h->yhat, h->yt, n, h->dy, h->loss);
+  PE_LAUNCHED(mlp_loss);
+  const int L = h->layers;
+  const int top = h->fan_out[L - 1];
+  hipLaunchKernelGGL(mlp_head_back, dim3((unsigned)top), dim3(256), 0, h->stream,
+                     h->Hs[L - 1], h->Zs[L - 1], n, top, h->w_head, h->dy, h->gw_head,
+                     h->gb_head, h->dZ[L - 1]);
+  PE_LAUNCHED(mlp_head_back);
+  for (int l = L - 1; l >= 0; l--) {
+    const __bf16 *input = l == 0 ? h->F : h->Hs[l - 1];
+    const int in_w = l == 0 ? h->shape.width : h->fan_out[l - 1];
+    // gW[o, i] = sum_m dZ[m, o] X[m, i]: A(o, m) = dZ[m*out + o], B(m, i) = X[m*in_w + i]
+    GemmArgs g{};
+    g.A = h->dZ[l];
+    g.sam = 1;
+    g.sak = h->fan_out[l];
+    g.B = input;
+    g.sbk = in_w;
+    g.sbn = 1;
+    g.M = h->fan_out[l];
+    g.N = h->fan_in[l];
+    g.K = (int)n;
+    g.epilogue = kStoreF32;
+    g.C = h->gW[l];
+    g.ldc = h->fan_in[l];
+    if ((rc = launch_gemm(h, g))) return rc;
+    hipLaunchKernelGGL(column_sum, dim3((unsigned)h->fan_out[l]), dim3(256), 0, h->stream,
+                       h->dZ[l], n, h->fan_out[l], h->gb[l]);
+    PE_LAUNCHED(column_sum);
+    if (l > 0) {
+      // dZ_{l-1}[m, i] = (sum_o dZ_l[m, o] W_l[o, i]) * silu'(Z_{l-1}[m, i])
+      GemmArgs d{};
+      d.A = h->dZ[l];
+      d.sam = h->fan_out[l];
+      d.sak = 1;
+      d.B = h->Wt[l];       // B(k = o, n = i) = Wt[i, o] (k-contiguous)
+      d.sbk = 1;
+      d.sbn = h->fan_out[l];
+      d.M = (int)n;
+      d.N = h->fan_in[l];
+      d.K = h->fan_out[l];
+      d.epilogue = kSiluGrad;
+      d.H = h->dZ[l - 1];
+      d.ldh = h->fan_in[l];
+      d.Z = h->Zs[l - 1];
+      d.ldz = h->fan_in[l];
+      if ((rc = launch_gemm(h, d))) return rc;
+    }
+  }
+  if (loss_host) {
+    PE_HIP(hipMemcpyAsync(loss_host, h->loss, 4, hipMemcpyDeviceToHost, h->stream));
+  }
+  return PE_OK;
+}
+
+static int optimizer_step(pe_mlp *h) {
+  h->step++;
+  const pe_mlp_config &c = h->cfg;
+  const float c1 = 1.f - std::pow(c.beta1, (float)h->step);
+  const float c2 = 1.f - std::pow(c.beta2, (float)h->step);
+  auto run = [&](float *p, float *g, float *m, float *v, long long n, __bf16 *cp,
+                 __bf16 *ct, int rows, int cols, float wd) -> int {
+    hipLaunchKernelGGL(adamw, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream,
+                       p, g, m, v, n, c.learning_rate, c.beta1, c.beta2, c.eps, wd, c1, c2,
+                       1.f, cp, ct, rows, cols);
+    PE_LAUNCHED(adamw);
+    return PE_OK;
+  };
+  int rc;
+  for (int l = 0; l < h->layers; l++) {
+    const long long n = (long long)h->fan_out[l] * h->fan_in[l];
+    if ((rc = run(h->W[l], h->gW[l], h->mW[l], h->vW[l], n, h->Wh[l], h->Wt[l], h->fan_out[l],
+                  h->fan_in[l], c.weight_decay)))
+      return rc;
+    if ((rc = run(h->b[l], h->gb[l], h->mb[l], h->vb[l], h->fan_out[l], nullptr, nullptr, 1,
+                  h->fan_out[l], 0.f)))
+      return rc;
+  }
+  const int top = h->fan_out[h->layers - 1];
+  if ((rc = run(h->w_head, h->gw_head, h->mw_head, h->vw_head, top, nullptr, nullptr, 1, top,
+                c.weight_decay)))
+    return rc;
+  return run(h->b_head, h->gb_head, h->mb_head, h->vb_head, 1, nullptr, nullptr, 1, 1, 0.f);
+}
+
+static int refresh_copies(pe_mlp *h) {
+  for (int l = 0; l < h->layers; l++) {
+    const int rows = h->fan_out[l], cols = h->fan_in[l];
+    std::vector<float> w((size_t)rows * cols);
+    PE_HIP(hipMemcpy(w.data(), h->W[l], w.size() * 4, hipMemcpyDeviceToHost));
+    std::vector<__bf16> a(w.size()), t(w.size());
+    for (int r = 0; r < rows; r++)
+      for (int c = 0; c < cols; c++) {
+        a[(size_t)r * cols + c] = (__bf16)w[(size_t)r * cols + c];
+        t[(size_t)c * rows + r] = (__bf16)w[(size_t)r * cols + c];
+      }
+    PE_HIP(hipMemcpy(h->Wh[l], a.data(), a.size() * 2, hipMemcpyHostToDevice));
+    PE_HIP(hipMemcpy(h->Wt[l], t.data(), t.size() * 2, hipMemcpyHostToDevice));
+  }
+  return PE_OK;
+}
+
+// Flattened parameter order: for each hidden layer W[out,in], b[out]; then
+// head w[top], b[1].
+static long long param_count(const pe_mlp *h) {
+  long long n = 0;
+  for (int l = 0; l < h->layers; l++) n += (long long)h->fan_out[l] * (h->fan_in[l] + 1);
+  return n + h->fan_out[h->layers - 1] + 1;
+}
+
+template <typename F>
+static int for_each_param(pe_mlp *h, F f) {
+  long long at = 0;
+  int rc;
+  for (int l = 0; l < h->layers; l++) {
+    const long long nw = (long long)h->fan_out[l] * h->fan_in[l];
+    if ((rc = f(h->W[l], h->gW[l], at, nw))) return rc;
+    at += nw;
+    if ((rc = f(h->b[l], h->gb[l], at, (long long)h->fan_out[l]))) return rc;
+    at += h->fan_out[l];
+  }
+  const int top = h->fan_out[h->layers - 1];
+  if ((rc = f(h->w_head, h->gw_head, at, (long long)top))) return rc;
+  at += top;
+  return f(h->b_head, h->gb_head, at, 1ll);
+}
+
+static int ensure_capacity(pe_mlp *h, long long n) {
+  if (n <= h->cap) return PE_OK;
+  auto free_all = [&]() {
+    for (void *p : {(void *)h->raw, (void *)h->yt, (void *)h->yhat, (void *)h->dy, (void *)h->F})
+      if (p) (void)hipFree(p);
+    for (auto *p : h->Hs) if (p) (void)hipFree(p);
+    for (auto *p : h->dZ) if (p) (void)hipFree(p);
+    for (auto *p : h->Zs) if (p) (void)hipFree(p);
+  };
+  free_all();
+  h->cap = 0;
+  const long long cap = std::max(n, 1024ll);
+  PE_HIP(hipMalloc(&h->raw, sizeof(float) * cap * h->shape.raw));
+  PE_HIP(hipMalloc(&h->yt, sizeof(float) * cap));
+  PE_HIP(hipMalloc(&h->yhat, sizeof(float) * cap));
+  PE_HIP(hipMalloc(&h->dy, sizeof(float) * cap));
+  PE_HIP(hipMalloc(&h->F, sizeof(__bf16) * cap * h->shape.width));
+  for (int l = 0; l < h->layers; l++) {
+    PE_HIP(hipMalloc(&h->Hs[l], sizeof(__bf16) * cap * h->fan_out[l]));
+    PE_HIP(hipMalloc(&h->dZ[l], sizeof(__bf16) * cap * h->fan_out[l]));
+    PE_HIP(hipMalloc(&h->Zs[l], sizeof(float) * cap * h->fan_out[l]));
+  }
+  h->cap = cap;
+  return PE_OK;
+}
+
+namespace pe {
+int mlp_reserve(pe_mlp *h, long long n) { return ensure_capacity(h, n); }
+int mlp_forward_loaded(pe_mlp *h, long long n) { return forward(h, n, false); }
+}  // namespace pe
+
+extern "C" int pe_mlp_create(const pe_mlp_config *cfg, pe_mlp **out) {
+  clear_error();
+  if (!cfg || !out) return fail(PE_ERR_INVALID, "null argument");
+  *out = nullptr;
+  const pe_mlp_config &c = *cfg;
+  if (c.extra < 0 || c.extra > 8 || c.rff < 0 || c.rff % 2 || c.rff > 64 || c.hidden < 16 ||
+      c.hidden % 64 || c.layers < 1 || c.layers > 8 || !(c.learning_rate > 0))
+    return fail(PE_ERR_INVALID, "invalid MLP configuration");
+  int dev = 0;
+  int rc = require_device(&dev);
+  if (rc) return rc;
+  pe_mlp *h = new pe_mlp();
+  h->device = dev;
+  h->cfg = c;
+  h->shape.raw = 8 + c.extra;
+  h->shape.extra = c.extra;
+  h->shape.rff = c.rff;
+  h->shape.width = ((8 + c.rff + c.extra + 31) / 32) * 32;
+  h->shape.hidden = c.hidden;
+  h->layers = c.layers;
+  for (int l = 0; l < c.layers; l++) {
+    h->fan_in.push_back(l == 0 ? h->shape.width : c.hidden);
+    h->fan_out.push_back(c.hidden);
+  }
+  h->W.resize(c.layers); h->b.resize(c.layers); h->gW.resize(c.layers); h->gb.resize(c.layers);
+  h->mW.resize(c.layers); h->vW.resize(c.layers); h->mb.resize(c.layers); h->vb.resize(c.layers);
+  h->Wh.resize(c.layers); h->Wt.resize(c.layers);
+  h->Hs.resize(c.layers); h->dZ.resize(c.layers); h->Zs.resize(c.layers);
+  auto body = [&]() -> int {
+    PE_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    std::mt19937_64 rng((unsigned long long)c.seed);
+    std::normal_distribution<float> normal(0.f, 1.f);
+    for (int l = 0; l < c.layers; l++) {
+      const long long nw = (long long)h->fan_out[l] * h->fan_in[l];
+      for (float **p : {&h->W[l], &h->gW[l], &h->mW[l], &h->vW[l]})
+        PE_HIP(hipMalloc(p, sizeof(float) * nw));
+      for (float **p : {&h->b[l], &h->gb[l], &h->mb[l], &h->vb[l]})
+        PE_HIP(hipMalloc(p, sizeof(float) * h->fan_out[l]));
+      PE_HIP(hipMalloc(&h->Wh[l], sizeof(__bf16) * nw));
+      PE_HIP(hipMalloc(&h->Wt[l], sizeof(__bf16) * nw));
+      // He-style init for SiLU; features beyond the real inputs are zero.
+      const int real_in = l == 0 ? 8 + c.rff + c.extra : h->fan_in[l];
+      std::vector<float> w(nw, 0.f);
+      const float sd = std::sqrt(2.f / real_in);
+      for (int o = 0; o < h->fan_out[l]; o++)
+        for (int i = 0; i < real_in; i++) w[(size_t)o * h->fan_in[l] + i] = sd * normal(rng);
+      PE_HIP(hipMemcpy(h->W[l], w.data(), nw * 4, hipMemcpyHostToDevice));
+      std::vector<float> zero(std::max<long long>(nw, h->fan_out[l]), 0.f);
+      PE_HIP(hipMemcpy(h->b[l], zero.data(), h->fan_out[l] * 4, hipMemcpyHostToDevice));
+      for (float *p : {h->mW[l], h->vW[l], h->gW[l]})
+        PE_HIP(hipMemcpy(p, zero.data(), nw * 4, hipMemcpyHostToDevice));
+      for (float *p : {h->mb[l], h->vb[l], h->gb[l]})
+        PE_HIP(hipMemcpy(p, zero.data(), h->fan_out[l] * 4, hipMemcpyHostToDevice));
+    }
+    const int top = c.hidden;
+    for (float **p : {&h->w_head, &h->gw_head, &h->mw_head, &h->vw_head})
+      PE_HIP(hipMalloc(p, sizeof(float) * top));
+    for (float **p : {&h->b_head, &h->gb_head, &h->mb_head, &h->vb_head, &h->loss})
+      PE_HIP(hipMalloc(p, sizeof(float) * 1));
+    std::vector<float> w(top), zero(top, 0.f);
+    for (int i = 0; i < top; i++) w[i] = std::sqrt(1.f / top) * normal(rng);
+    PE_HIP(hipMemcpy(h->w_head, w.data(), top * 4, hipMemcpyHostToDevice));
+    for (float *p : {h->mw_head, h->vw_head, h->gw_head})
+      PE_HIP(hipMemcpy(p, zero.data(), top * 4, hipMemcpyHostToDevice));
+    for (float *p : {h->b_head, h->mb_head, h->vb_head, h->gb_head})
+      PE_HIP(hipMemcpy(p, zero.data(), 4, hipMemcpyHostToDevice));
+    // Normalisation and random Fourier projection.
+    PE_HIP(hipMalloc(&h->shift, 32));
+    PE_HIP(hipMalloc(&h->scale, 32));
+    PE_HIP(hipMemcpy(h->shift, c.state_shift, 32, hipMemcpyHostToDevice));
+    PE_HIP(hipMemcpy(h->scale, c.state_scale, 32, hipMemcpyHostToDevice));
+    const int half = std::max(1, c.rff / 2);
+    std::vector<float> proj((size_t)half * 8);
+    for (auto &x : proj) x = c.rff_scale * normal(rng);
+    PE_HIP(hipMalloc(&h->proj, proj.size() * 4));
+    PE_HIP(hipMemcpy(h->proj, proj.data(), proj.size() * 4, hipMemcpyHostToDevice));
+    int r = refresh_copies(h);
+    if (r) return r;
+    return ensure_capacity(h, 4096);
+  };
+  rc = body();
+  if (rc) {
+    pe_mlp_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return PE_OK;
+}
+
+extern "C" int pe_mlp_param_count(pe_mlp *h, int64_t *count) {
+  clear_error();
+  if (!h || !count) return fail(PE_ERR_INVALID, "invalid argument");
+  *count = param_count(h);
+  return PE_OK;
+}
+
+extern "C" int pe_mlp_get_params(pe_mlp *h, float *out, int64_t count) {
+  clear_error();
+  if (!h || !out || count != param_count(h)) return fail(PE_ERR_INVALID, "size mismatch");
+  PE_HIP(hipSetDevice(h->device));
+  return for_each_param(h, [&](float *p, float *, long long at, long long n) -> int {
+    PE_HIP(hipMemcpy(out + at, p, n * 4, hipMemcpyDeviceToHost));
+    return PE_OK;
+  });
+}
+
+extern "C" int pe_mlp_set_params(pe_mlp *h, const float *in, int64_t count) {
+  clear_error();
+  if (!h || !in || count != param_count(h)) return fail(PE_ERR_INVALID, "size mismatch");
+  for (long long k = 0; k < count; k++)
+    if (!std::isfinite(in[k])) return fail(PE_ERR_INVALID, "non-finite parameter");
+  PE_HIP(hipSetDevice(h->device));
+  int rc = for_each_param(h, [&](float *p, float *, long long at, long long n) -> int {
+    PE_HIP(hipMemcpy(p, in + at, n * 4, hipMemcpyHostToDevice));
+    return PE_OK;
+  });
+  if (rc) return rc;
+  return refresh_copies(h);
+}
+
+extern "C" int pe_mlp_forward(pe_mlp *h, const float *raw, int64_t n, float *out) {
+  clear_error();
+  if (!h || n < 0 || (n && (!raw || !out))) return fail(PE_ERR_INVALID, "invalid argument");
+  PE_HIP(hipSetDevice(h->device));
+  const long long chunk = 1 << 18;
+  for (long long at = 0; at < n; at += chunk) {
+    const long long m = std::min(chunk, (long long)n - at);
+    int rc = ensure_capacity(h, m);
+    if (rc) return rc;
+    PE_HIP(hipMemcpyAsync(h->raw, raw + at * h->shape.raw, sizeof(float) * m * h->shape.raw,
+                          hipMemcpyHostToDevice, h->stream));
+    rc = forward(h, m, false);
+    if (rc) return rc;
+    PE_HIP(hipMemcpyAsync(out + at, h->yhat, sizeof(float) * m, hipMemcpyDeviceToHost,
+                          h->stream));
+  }
+  PE_HIP(hipStreamSynchronize(h->stream));
+  return PE_OK;
+}
+
+extern "C" int pe_mlp_features(pe_mlp *h, const float *raw, int64_t n, float *features,
+                               int32_t *width) {
+  clear_error();
+  if (!h || n < 0 || (n && (!raw || !features))) return fail(PE_ERR_INVALID, "invalid argument");
+  if (width) *width = h->shape.width;
+  if (n == 0) return PE_OK;
+  PE_HIP(hipSetDevice(h->device));
+  int rc = ensure_capacity(h, n);
+  if (rc) return rc;
+  PE_HIP(hipMemcpyAsync(h->raw, raw, sizeof(float) * n * h->shape.raw, hipMemcpyHostToDevice,
+                        h->stream));
+  hipLaunchKernelGGL(mlp_features, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream,
+                     h->raw, (long long)n, h->shape, h->shift, h->scale, h->proj, h->F);
+  PE_LAUNCHED(mlp_features);
+  std::vector<__bf16> f((size_t)n * h->shape.width);
+  PE_HIP(hipMemcpyAsync(f.data(), h->F, f.size() * 2, hipMemcpyDeviceToHost, h->stream));
+  PE_HIP(hipStreamSynchronize(h->stream));
+  for (size_t k = 0; k < f.size(); k++) features[k] = (float)f[k];
+  return PE_OK;
+}
+
+extern "C" int pe_mlp_gradients(pe_mlp *h, const float *raw, const float *target,
+                                int64_t n, float *grads, float *loss) {
+  clear_error();
+  if (!h || n < 1 || !raw || !target || !grads) return fail(PE_ERR_INVALID, "invalid argument");
+  PE_HIP(hipSetDevice(h->device));
+  int rc = ensure_capacity(h, n);
+  if (rc) return rc;
+  PE_HIP(hipMemcpyAsync(h->raw, raw, sizeof(float) * n * h->shape.raw, hipMemcpyHostToDevice,
+                        h->stream));
+  PE_HIP(hipMemcpyAsync(h->yt, target, sizeof(float) * n, hipMemcpyHostToDevice, h->stream));
+  float l = 0;
+  if ((rc = backward(h, n, &l))) return rc;
+  PE_HIP(hipStreamSynchronize(h->stream));
+  if (loss) *loss = l;
+  return for_each_param(h, [&](float *, float *g, long long at, long long m) -> int {
+    PE_HIP(hipMemcpy(grads + at, g, m * 4, hipMemcpyDeviceToHost));
+    return PE_OK;
+  });
+}
+
+extern "C" int pe_mlp_set_dataset(pe_mlp *h, const float *raw, const float *target, int64_t n) {
+  clear_error();
+  if (!h || n < 1 || !raw || !target) return fail(PE_ERR_INVALID, "invalid argument");
+  PE_HIP(hipSetDevice(h->device));
+  if (h->data_x) (void)hipFree(h->data_x);
+  if (h->data_y) (void)hipFree(h->data_y);
+  h->data_x = h->data_y = nullptr;
+  PE_HIP(hipMalloc(&h->data_x, sizeof(float) * n * h->shape.raw));
+  PE_HIP(hipMalloc(&h->data_y, sizeof(float) * n));
+  PE_HIP(hipMemcpy(h->data_x, raw, sizeof(float) * n * h->shape.raw, hipMemcpyHostToDevice));
+  PE_HIP(hipMemcpy(h->data_y, target, sizeof(float) * n, hipMemcpyHostToDevice));
+  h->data_n = n;
+  return PE_OK;
+}
+
+extern "C" int pe_mlp_train(pe_mlp *h, int32_t steps, int32_t batch, uint32_t seed,
+                            float *losses) {
+  clear_error();
+  if (!h || steps < 0 || batch < 1 || !h->data_n)
+    return fail(PE_ERR_INVALID, "invalid argument or no dataset");
+  PE_HIP(hipSetDevice(h->device));
+  int rc = ensure_capacity(h, batch);
+  if (rc) return rc;
+  std::vector<float> host(steps > 0 ? steps : 1);
+  for (int s = 0; s < steps; s++) {
+    hipLaunchKernelGGL(gather_rows, dim3((unsigned)((batch + 255) / 256)), dim3(256), 0,
+                       h->stream, h->data_x, h->data_y, h->shape.raw, h->data_n,
+                       (long long)batch, seed, (unsigned)(h->step + s), h->raw, h->yt);
+    PE_LAUNCHED(gather_rows);
+    if ((rc = backward(h, batch, losses ? &host[s] : nullptr))) return rc;
+    if ((rc = optimizer_step(h))) return rc;
+  }
+  PE_HIP(hipStreamSynchronize(h->stream));
+  if (losses)
+    for (int s = 0; s < steps; s++) losses[s] = host[s];
+  return PE_OK;
+}
+
+extern "C" int pe_mlp_gemm_check(const uint16_t *a, const uint16_t *b, int32_t M,
+                                 int32_t N, int32_t K, int32_t transposed_b, float *c) {
+  // C[m, n] = sum_k A[m, k] B(k, n); B given [K, N] (transposed_b = 0) or
+  // [N, K] (transposed_b = 1); inputs are bf16 bit patterns.
+  clear_error();
+  if (!a || !b || !c || M < 1 || N < 1 || K < 1) return fail(PE_ERR_INVALID, "invalid argument");
+  int rc = require_device(nullptr);
+  if (rc) return rc;
+  void *da = nullptr, *db = nullptr, *dc = nullptr;
+  auto body = [&]() -> int {
+    PE_HIP(hipMalloc(&da, 2ll * M * K));
+    PE_HIP(hipMalloc(&db, 2ll * K * N));
+    PE_HIP(hipMalloc(&dc, 4ll * M * N));
+    PE_HIP(hipMemcpy(da, a, 2ll * M * K, hipMemcpyHostToDevice));
+    PE_HIP(hipMemcpy(db, b, 2ll * K * N, hipMemcpyHostToDevice));
+    GemmArgs g{};
+    g.A = (const __bf16 *)da;
+    g.sam = K;
+    g.sak = 1;
+    g.B = (const __bf16 *)db;
+    if (transposed_b) {
+      g.sbk = 1;
+      g.sbn = K;
+    } else {
+      g.sbk = N;
+      g.sbn = 1;
+    }
+    g.M = M;
+    g.N = N;
+    g.K = K;
+    g.epilogue = kStoreF32;
+    g.C = (float *)dc;
+    g.ldc = N;
+    const dim3 grid((unsigned)((M + kTile - 1) / kTile), (unsigned)((N + kTile - 1) / kTile));
+    hipLaunchKernelGGL(mfma_gemm, grid, dim3(256), 0, 0, g);
+    PE_LAUNCHED(mfma_gemm);
+    PE_HIP(hipMemcpy(c, dc, 4ll * M * N, hipMemcpyDeviceToHost));
+    return PE_OK;
+  };
+  rc = body();
+  for (void *p : {da, db, dc})
+    if (p) (void)hipFree(p);
+  return rc;
+}
+
+extern "C" void pe_mlp_destroy(pe_mlp *h) {
+  if (!h) return;
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  auto f = [](void *p) {
+    if (p) (void)hipFree(p);
+  };
+  for (int l = 0; l < h->layers; l++) {
+    for (void *p : {(void *)h->W[l], (void *)h->b[l], (void *)h->gW[l], (void *)h->gb[l],
+                    (void *)h->mW[l], (void *)h->vW[l], (void *)h->mb[l], (void *)h->vb[l],
+                    (void *)h->Wh[l], (void *)h->Wt[l], (void *)h->Hs[l], (void *)h->dZ[l],
+                    (void *)h->Zs[l]})
+      f(p);
+  }
+  for (void *p : {(void *)h->w_head, (void *)h->b_head, (void *)h->gw_head, (void *)h->gb_head,
+                  (void *)h->mw_head, (void *)h->vw_head, (void *)h->mb_head, (void *)h->vb_head,
+                  (void *)h->shift, (void *)h->scale, (void *)h->proj, (void *)h->raw,
+                  (void *)h->yt, (void *)h->yhat, (void *)h->dy, (void *)h->loss, (void *)h->F,
+                  (void *)h->data_x, (void *)h->data_y})
+    f(p);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}

@@ -84,7 +84,7 @@ class HJI5DConfig(ctypes.Structure):
         ("e_curvature", ctypes.c_double),
         ("exit_radius", ctypes.c_double),
         ("capture", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("vp_chunk", ctypes.c_int32),
         ("p_length", ctypes.c_double),
         ("p_width", ctypes.c_double),
         ("e_length", ctypes.c_double),
@@ -106,6 +106,72 @@ class HJI5DStats(ctypes.Structure):
     ]
 
 
+class SLCar(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int32),
+        ("actions", ctypes.c_int32),
+        ("states", ctypes.c_void_p),
+        ("successors", ctypes.c_void_p),
+        ("weights", ctypes.c_void_p),
+        ("allowed", ctypes.c_void_p),
+        ("position", ctypes.c_void_p),
+        ("positions", ctypes.c_int32),
+        ("clearance", ctypes.c_void_p),
+    ]
+
+
+class SLGame(ctypes.Structure):
+    _fields_ = [
+        ("p", SLCar),
+        ("e", SLCar),
+        ("inside", ctypes.c_void_p),
+        ("near", ctypes.c_void_p),
+        ("exit_value", ctypes.c_float),
+        ("p_length", ctypes.c_float),
+        ("p_width", ctypes.c_float),
+        ("e_length", ctypes.c_float),
+        ("e_width", ctypes.c_float),
+    ]
+
+
+class SLStats(ctypes.Structure):
+    _fields_ = [
+        ("sweeps", ctypes.c_int64),
+        ("maps", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("pairs", ctypes.c_int64),
+        ("active_pairs", ctypes.c_int64),
+        ("update_seconds", ctypes.c_double),
+        ("last_residual", ctypes.c_double),
+        ("p_ms", ctypes.c_double),
+        ("e_ms", ctypes.c_double),
+        ("p_launches", ctypes.c_int64),
+        ("e_launches", ctypes.c_int64),
+    ]
+
+
+class MLPConfig(ctypes.Structure):
+    _fields_ = [
+        ("extra", ctypes.c_int32),
+        ("rff", ctypes.c_int32),
+        ("hidden", ctypes.c_int32),
+        ("layers", ctypes.c_int32),
+        ("seed", ctypes.c_uint32),
+        ("rff_scale", ctypes.c_float),
+        ("state_shift", ctypes.c_float * 8),
+        ("state_scale", ctypes.c_float * 8),
+        ("learning_rate", ctypes.c_float),
+        ("weight_decay", ctypes.c_float),
+        ("beta1", ctypes.c_float),
+        ("beta2", ctypes.c_float),
+        ("eps", ctypes.c_float),
+    ]
+
+
+PE_SL_EARLY_EXIT = 1
+PE_SL_ORDERED = 2
+PE_SL_GRAPH = 4
+
 P = ctypes.c_void_p
 I32 = ctypes.c_int32
 I64 = ctypes.c_int64
@@ -115,6 +181,7 @@ F64 = ctypes.c_double
 SIGNATURES = {
     "pe_last_error": (ctypes.c_char_p, []),
     "pe_abi_version": (ctypes.c_int, []),
+    "pe_abi_struct_size": (ctypes.c_int, [ctypes.c_char_p]),
     "pe_device_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, P, P]),
     "pe_hji5d_create": (ctypes.c_int, [ctypes.POINTER(HJI5DConfig), P]),
     "pe_hji5d_advance_to": (ctypes.c_int, [P, F64, I64]),
@@ -126,6 +193,43 @@ SIGNATURES = {
     "pe_hji5d_get_stats": (ctypes.c_int, [P, ctypes.POINTER(HJI5DStats)]),
     "pe_hji5d_set_profiling": (ctypes.c_int, [P, I32]),
     "pe_hji5d_destroy": (None, [P]),
+    "pe_geom_clearance": (ctypes.c_int, [P, I64, P, I32, F64, P, P]),
+    "pe_geom_backups": (ctypes.c_int, [P, I64, P, I32, F64, P, P]),
+    "pe_geom_actions": (ctypes.c_int, [P, I64, P, I32, F64, P, I32, F64, P, P, P]),
+    "pe_geom_connections": (ctypes.c_int, [P, P, I64, P, I32, F64, P, P]),
+    "pe_sl_create": (ctypes.c_int, [ctypes.POINTER(SLGame), I32, P]),
+    "pe_sl_initialize": (ctypes.c_int, [P, P]),
+    "pe_sl_sweep": (ctypes.c_int, [P, I32, I32]),
+    "pe_sl_run": (ctypes.c_int, [P, F64, I32, I32, I32, P, P]),
+    "pe_sl_read": (ctypes.c_int, [P, I32, P]),
+    "pe_sl_query": (ctypes.c_int, [P, I32, I64, P, P, P, P, P]),
+    "pe_sl_trace": (ctypes.c_int, [P, P, I64, P]),
+    "pe_sl_get_stats": (ctypes.c_int, [P, ctypes.POINTER(SLStats)]),
+    "pe_sl_set_profiling": (ctypes.c_int, [P, I32]),
+    "pe_sl_destroy": (None, [P]),
+    "pe_sl_warm_start": (ctypes.c_int, [P, P, ctypes.c_float]),
+    "pe_sl_pair_inputs": (ctypes.c_int, [P, I32, I64, I64, P]),
+    "pe_mlp_create": (ctypes.c_int, [ctypes.POINTER(MLPConfig), P]),
+    "pe_mlp_param_count": (ctypes.c_int, [P, P]),
+    "pe_mlp_get_params": (ctypes.c_int, [P, P, I64]),
+    "pe_mlp_set_params": (ctypes.c_int, [P, P, I64]),
+    "pe_mlp_forward": (ctypes.c_int, [P, P, I64, P]),
+    "pe_mlp_features": (ctypes.c_int, [P, P, I64, P, P]),
+    "pe_mlp_gradients": (ctypes.c_int, [P, P, P, I64, P, P]),
+    "pe_mlp_set_dataset": (ctypes.c_int, [P, P, P, I64]),
+    "pe_mlp_train": (ctypes.c_int, [P, I32, I32, ctypes.c_uint32, P]),
+    "pe_mlp_gemm_check": (ctypes.c_int, [P, P, I32, I32, I32, I32, P]),
+    "pe_mlp_destroy": (None, [P]),
+}
+
+
+STRUCTS = {
+    "pe_hji5d_config": HJI5DConfig,
+    "pe_hji5d_stats": HJI5DStats,
+    "pe_sl_car": SLCar,
+    "pe_sl_game": SLGame,
+    "pe_sl_stats": SLStats,
+    "pe_mlp_config": MLPConfig,
 }
 
 
@@ -224,3 +328,112 @@ class HJI5D:
 
     def set_profiling(self, on: bool):
         check(lib().pe_hji5d_set_profiling(self._h, int(bool(on))))
+
+
+class SL:
+    """Owning wrapper of a pe_sl handle: the 8D backup of one or more maps.
+
+    `games` is a list of dicts with the numpy tables of each map (built by
+    pe_amd.local_game.grid_game.game_tables); they are copied to the device
+    at construction."""
+
+    def __init__(self, games):
+        self._keep = []
+        arr = (SLGame * len(games))()
+        for k, g in enumerate(games):
+            for side in ("p", "e"):
+                car = g[side]
+                c = getattr(arr[k], side)
+                c.n = len(car["states"])
+                c.actions = car["allowed"].shape[1]
+                for name, dtype in (("states", np.float64), ("successors", np.int32),
+                                    ("weights", np.float32), ("allowed", np.uint8),
+                                    ("position", np.int32)):
+                    a = np.ascontiguousarray(car[name], dtype)
+                    self._keep.append(a)
+                    setattr(c, name, a.ctypes.data)
+                c.positions = int(car["positions"])
+                if car.get("clearance") is not None:
+                    a = np.ascontiguousarray(car["clearance"], np.float64)
+                    self._keep.append(a)
+                    c.clearance = a.ctypes.data
+            for name in ("inside", "near"):
+                a = np.ascontiguousarray(g[name], np.uint8)
+                self._keep.append(a)
+                setattr(arr[k], name, a.ctypes.data)
+            arr[k].exit_value = g["exit_value"]
+            arr[k].p_length, arr[k].p_width = g["p_body"]
+            arr[k].e_length, arr[k].e_width = g["e_body"]
+        self._h = ctypes.c_void_p()
+        check(lib().pe_sl_create(arr, len(games), ctypes.byref(self._h)))
+        self._keep = []
+        self.shapes = [(len(g["p"]["states"]), len(g["e"]["states"])) for g in games]
+
+    def close(self):
+        if self._h:
+            lib().pe_sl_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def initialize(self, v0=None):
+        a = None if v0 is None else np.ascontiguousarray(
+            np.concatenate([np.ravel(v) for v in v0]) if isinstance(v0, (list, tuple)) else v0,
+            np.float32)
+        check(lib().pe_sl_initialize(self._h, ptr(a)))
+
+    def sweep(self, count=1, flags=PE_SL_EARLY_EXIT):
+        check(lib().pe_sl_sweep(self._h, int(count), int(flags)))
+
+    def run(self, tol, max_sweeps, check_every=1, flags=PE_SL_EARLY_EXIT):
+        done, conv = ctypes.c_int32(), ctypes.c_int32()
+        check(lib().pe_sl_run(self._h, float(tol), int(max_sweeps), int(check_every),
+                              int(flags), ctypes.byref(done), ctypes.byref(conv)))
+        return done.value, bool(conv.value)
+
+    def read(self, index=0):
+        np_, ne = self.shapes[index]
+        out = np.empty((np_, ne), np.float32)
+        check(lib().pe_sl_read(self._h, int(index), ptr(out)))
+        return out
+
+    def query(self, p_ids, p_weights, e_ids, e_weights, index=0):
+        arrays = [np.ascontiguousarray(p_ids, np.int32), np.ascontiguousarray(p_weights, np.float32),
+                  np.ascontiguousarray(e_ids, np.int32), np.ascontiguousarray(e_weights, np.float32)]
+        n = len(arrays[0])
+        out = np.empty(n, np.float32)
+        check(lib().pe_sl_query(self._h, int(index), n, *[ptr(a) for a in arrays], ptr(out)))
+        return out
+
+    def trace(self):
+        count = ctypes.c_int64()
+        check(lib().pe_sl_trace(self._h, None, 0, ctypes.byref(count)))
+        out = np.empty(count.value, np.float64)
+        check(lib().pe_sl_trace(self._h, ptr(out), count.value, ctypes.byref(count)))
+        return out.reshape(-1, len(self.shapes))
+
+    def stats(self):
+        s = SLStats()
+        check(lib().pe_sl_get_stats(self._h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in SLStats._fields_ if k != "reserved"}
+
+    def set_profiling(self, on):
+        check(lib().pe_sl_set_profiling(self._h, int(bool(on))))
+
+    def pair_inputs(self, first, count, index=0):
+        out = np.empty((int(count), 11), np.float32)
+        check(lib().pe_sl_pair_inputs(self._h, int(index), int(first), int(count), ptr(out)))
+        return out
+
+    def warm_start(self, mlp, delta):
+        check(lib().pe_sl_warm_start(self._h, mlp._h, float(delta)))

@@ -65,3 +65,11 @@ def test_invalid_options_rejected_before_any_device_work():
     assert b"Invalid solver options" in lib.pe_last_error()
     with pytest.raises(ValueError):
         native.check(rc)
+
+
+def test_struct_mirrors_match_the_c_layout():
+    import pe_amd._native as native
+
+    lib = native.lib()
+    for name, mirror in native.STRUCTS.items():
+        assert lib.pe_abi_struct_size(name.encode()) == ctypes.sizeof(mirror), name

@@ -1,0 +1,213 @@
+"""GPU parity of the 8D contact-game path (through the C ABI).
+
+* Geometry kernels (FP64) against the oracle, which is bit-identical to the
+  reference's own OpenMP library: successors to 1e-12, clearance to 1e-12,
+  legality flags equal (mismatches allowed only where the oracle's decision
+  sits within 1e-9 of a threshold; none are expected, any are printed).
+* Backup sweeps (FP32) against the FP64 oracle sweep: seeded random tables
+  (the reference's fake_grid test, tests/test_compact_dp.py:53-118) to 2e-6,
+  a small real map over several sweeps, and C2 at full size (one sweep
+  compared on every pair, 40 sweeps of properties).  Tolerance stated in
+  each test; capture-set membership must agree except within tolerance of 0.
+* The early exit, the control ordering and the hipGraph replay change
+  nothing: results are bit-identical to the plain sweep.
+"""
+
+from types import SimpleNamespace
+
+import numpy as np
+import pytest
+
+from pe_amd import _native
+from pe_amd.local_game.geometry import Footprint, random_rectangle_arena
+from pe_amd.local_game.grid_game import (
+    GameSolver,
+    PairDomain,
+    dense_inside,
+    game_tables,
+)
+from pe_amd.local_game.native import GeometryEngine
+from pe_amd.local_game.reference import benchmark_game, build_grids, contact_config
+from pe_amd.reversible.model import VehicleLimits
+
+pytestmark = pytest.mark.gpu
+EARLY, ORDERED, GRAPH = _native.PE_SL_EARLY_EXIT, _native.PE_SL_ORDERED, _native.PE_SL_GRAPH
+CONTROLS = np.array([[a, k] for a in (-3, 0, 3) for k in (-1, 0, 1)], float)
+
+
+def seeded_states(n, seed):
+    rng = np.random.default_rng(seed)
+    return np.stack([rng.uniform(-3, 3, n), rng.uniform(-3, 3, n),
+                     rng.uniform(-np.pi, np.pi, n), rng.uniform(-1, 2, n)], 1)
+
+
+def test_geometry_kernels_match_oracle(gpu, oracle_lib):
+    arena = random_rectangle_arena(3, pursuer=Footprint(0.5, 0.3), evader=Footprint(0.5, 0.3))
+    lim = VehicleLimits(-1.0, 2.0, 3.0, 1.0)
+    eng = GeometryEngine(arena, lim, "pursuer", control_dt=0.05)
+    obs, par = eng.obstacles, eng.params
+    s = seeded_states(4000, 0)
+    print("[gpu] geometry kernels", flush=True)
+    c_gpu = eng.clearance(s)
+    c_ref = oracle_lib.geom_clearance(s, obs, 3.0, par)
+    assert np.abs(c_gpu - c_ref).max() < 1e-12
+    b_gpu = eng.backup_safe(s)
+    b_ref = oracle_lib.geom_backups(s, obs, 3.0, par)
+    print(f"backup_safe mismatches: {np.flatnonzero(b_gpu != b_ref).tolist()}")
+    assert (b_gpu == b_ref).mean() >= 0.999
+    ok_gpu, nxt_gpu = eng.actions(s[:1500], CONTROLS, 0.05)
+    ok_ref, nxt_ref = oracle_lib.geom_actions(s[:1500], CONTROLS, 0.05, obs, 3.0, par)
+    err = nxt_gpu - nxt_ref
+    err[..., 2] = (err[..., 2] + np.pi) % (2 * np.pi) - np.pi
+    assert np.abs(err).max() < 1e-12
+    print(f"action mismatches: {np.argwhere(ok_gpu != ok_ref).tolist()}")
+    assert (ok_gpu == ok_ref).mean() >= 0.999
+    t = s[::-1] + np.random.default_rng(1).normal(0, 0.2, s.shape)
+    k_gpu = eng.connections(s, t)
+    k_ref = oracle_lib.geom_connections(s, t, obs, 3.0, par)
+    assert (k_gpu == k_ref).mean() >= 0.999
+
+
+def fake_grid(n, na, rng):
+    states = rng.uniform(-1, 1, (n, 4))
+    ids = rng.integers(n, size=(n, na, 16), dtype=np.int32)
+    w = rng.random((n, na, 16)).astype(np.float32)
+    w /= w.sum(-1, keepdims=True)
+    allowed = rng.random((n, na)) > 0.3
+    allowed[:, 0] = True
+    return SimpleNamespace(states=states, successors=ids, weights=w, allowed=allowed)
+
+
+def fake_tables(p, e, radius):
+    """Each state is its own cell, so the inside table is the pair mask."""
+    d = np.linalg.norm(p.states[:, None, :2] - e.states[None, :, :2], axis=-1)
+    inside = (d <= radius).astype(np.uint8)
+
+    def car(g):
+        return dict(states=g.states, successors=g.successors, weights=g.weights,
+                    allowed=g.allowed, position=np.arange(len(g.states), dtype=np.int32),
+                    positions=len(g.states))
+
+    return dict(p=car(p), e=car(e), inside=inside, near=np.ones_like(inside),
+                exit_value=float(radius), p_body=(0.8, 0.5), e_body=(0.4, 0.25)), inside
+
+
+@pytest.mark.parametrize("seed,np_,ne,na", [(114, 5, 7, 3), (57, 5, 7, 4), (8, 300, 257, 9)])
+def test_sweeps_match_fp64_oracle_on_random_tables(gpu, oracle_lib, seed, np_, ne, na):
+    rng = np.random.default_rng(seed)
+    p, e = fake_grid(np_, na, rng), fake_grid(ne, na, rng)
+    tables, inside = fake_tables(p, e, 1.2)
+    print(f"[gpu] random tables seed={seed} {np_}x{ne}", flush=True)
+    with GameSolver(tables) as s:
+        V0 = s.read().astype(np.float64)
+        expect = oracle_lib.sl_init(p.states, e.states, inside, 1.2,
+                                    Footprint(0.8, 0.5), Footprint(0.4, 0.25))
+        np.testing.assert_allclose(V0, expect, atol=5e-7)
+        V = V0
+        for k in range(3):
+            V, change = oracle_lib.sl_sweep(V, p, e, inside)
+            s.sweep(1, 0)
+            got = s.read()
+            np.testing.assert_allclose(got, V, atol=2e-6)
+            assert s.trace()[-1, 0] == pytest.approx(change, abs=2e-6)
+
+
+def run_variants(tables, sweeps):
+    out = {}
+    for name, flags in [("plain", 0), ("early", EARLY), ("ordered", EARLY | ORDERED),
+                        ("graph", EARLY | ORDERED | GRAPH)]:
+        with GameSolver(tables) as s:
+            s.sweep(sweeps, flags)
+            out[name] = (s.read(), s.trace()[:, 0])
+    return out
+
+
+def test_early_exit_ordering_and_graph_are_exact(gpu):
+    rng = np.random.default_rng(21)
+    p, e = fake_grid(400, 9, rng), fake_grid(300, 9, rng)
+    tables, _ = fake_tables(p, e, 1.0)
+    print("[gpu] flag variants", flush=True)
+    res = run_variants(tables, 6)
+    base, trace = res["plain"]
+    for name in ("early", "ordered", "graph"):
+        np.testing.assert_array_equal(res[name][0], base, err_msg=name)
+        np.testing.assert_array_equal(res[name][1], trace, err_msg=name)
+
+
+@pytest.fixture(scope="module")
+def small_game():
+    config = contact_config()
+    body = Footprint(0.5, 0.3)
+    arena = random_rectangle_arena(11, pursuer=body, evader=body, count=12)
+    print("[gpu] small game grids", flush=True)
+    p, e = build_grids(config, arena, headings=8, positions=7, half_width=3.0,
+                       p_speeds=3, e_speeds=3)
+    return config, arena, p, e, game_tables(p, e, arena, 3.0)
+
+
+def test_small_map_sweeps_match_oracle(gpu, oracle_lib, small_game):
+    config, arena, p, e, tables = small_game
+    inside = dense_inside(p, e, 3.0)
+    with GameSolver(tables) as s:
+        V = oracle_lib.sl_init(p.states, e.states, inside, 3.0, arena.pursuer, arena.evader)
+        np.testing.assert_allclose(s.read(), V, atol=5e-7)
+        for k in range(20):
+            V, _ = oracle_lib.sl_sweep(V, p, e, inside)
+        s.sweep(20, EARLY | ORDERED)
+        got = s.read()
+    d = np.abs(got - V)
+    print(f"small map ({len(p.states)}x{len(e.states)}) 20 sweeps: max|dV|={d.max():.3e} "
+          f"mean={d.mean():.3e}")
+    assert d.max() <= 1e-5
+    flip = np.flatnonzero((got.ravel() <= 0) != (V.ravel() <= 0))
+    assert (np.abs(V.ravel()[flip]) <= 1e-5).all()
+
+
+def test_query_and_packed_layout(gpu, small_game):
+    config, arena, p, e, tables = small_game
+    domain = PairDomain.build(p, e, 3.0)
+    with GameSolver(tables) as s:
+        s.sweep(5, EARLY)
+        field = s.read()
+        packed = domain.pack(field)
+        np.testing.assert_array_equal(domain.unpack(packed, 3.0, len(e.states)), field)
+        rng = np.random.default_rng(0)
+        ip = rng.integers(len(p.states), size=64)
+        ie = rng.integers(len(e.states), size=64)
+        pi, pw, _, _, _ = p.stencil(p.states[ip])
+        ei, ew, _, _, _ = e.stencil(e.states[ie])
+        got = s.query(pi, pw, ei, ew)
+    np.testing.assert_allclose(got, field[ip, ie], atol=1e-6)
+
+
+C2_SEED = 0
+
+
+@pytest.fixture(scope="module")
+def c2_game():
+    print("[gpu] C2 game build (11x11x12x5 per car, 30 rectangles)", flush=True)
+    return benchmark_game(C2_SEED)
+
+
+def test_c2_full_size(gpu, oracle_lib, c2_game):
+    config, arena, p, e, tables = c2_game
+    inside = dense_inside(p, e, 3.0)
+    print(f"C2: {len(p.states)} x {len(e.states)} legal states, {inside.sum()} pairs inside "
+          f"of {len(p.states) * len(e.states)}", flush=True)
+    with GameSolver(tables) as s:
+        V0 = s.read()
+        s.sweep(1, EARLY | ORDERED)
+        V1 = s.read()
+        ref1, _ = oracle_lib.sl_sweep(V0.astype(np.float64), p, e, inside, threads=16)
+        d = np.abs(V1 - ref1)
+        print(f"C2 one sweep vs FP64 oracle: max|dV|={d.max():.3e} mean={d.mean():.3e}")
+        assert d.max() <= 1e-5
+        s.sweep(39, EARLY | ORDERED | GRAPH)
+        V40 = s.read()
+        trace = s.trace()[:, 0]
+    assert (V40 <= V1 + 1e-7).all() and (V1 <= V0 + 1e-7).all()
+    assert (V40[~inside] == 3.0).all()
+    assert (V40[V0 <= 0] <= 0).all()
+    assert len(trace) == 40 and (trace >= 0).all()
+    print(f"C2 40 sweeps: residual {trace[-1]:.3e}, captured fraction "
+          f"{np.mean(V40[inside] <= 0):.4f}")
