@@ -91,14 +91,62 @@ def cpu_baseline(cfg_native, sample_steps: int):
     }
 
 
+def run_c2(args):
+    """8D contact game C2: sweeps of the compact min-max backup."""
+    from pe_amd import _native
+    from pe_amd.local_game.grid_game import GameSolver
+    from pe_amd.local_game.reference import benchmark_game
+
+    t0 = time.perf_counter()
+    config, arena, p, e, tables = benchmark_game(args.seed)
+    build_s = time.perf_counter() - t0
+    flags = _native.PE_SL_EARLY_EXIT | _native.PE_SL_ORDERED | _native.PE_SL_GRAPH
+    with GameSolver(tables) as s:
+        st0 = s.stats()
+        if args.warmup:
+            s.sweep(args.warmup, flags)
+        t1 = time.perf_counter()
+        s.sweep(args.steps, flags)
+        wall = time.perf_counter() - t1
+        s.set_profiling(True)
+        s.sweep(min(args.steps, 20), flags & ~_native.PE_SL_GRAPH)
+        prof = s.stats()
+        s.set_profiling(False)
+        out = {
+            "workload": "C2 8D contact game, 11x11 positions (+-3 m) x 12 headings x 5 speeds "
+                        "per car, 30 random rectangles (seed %d)" % args.seed,
+            "legal_pairs": st0["pairs"], "active_pairs": st0["active_pairs"],
+            "dense_pairs": (11 * 11 * 12 * 5) ** 2,
+            "table_build_s": build_s, "sweeps": args.steps, "ms_per_sweep": wall / args.steps * 1e3,
+            "legal_pair_updates_per_s": st0["pairs"] * args.steps / wall,
+            "active_pair_updates_per_s": st0["active_pairs"] * args.steps / wall,
+            "p_side_ms": prof["p_ms"] / max(1, prof["p_launches"]),
+            "e_side_ms": prof["e_ms"] / max(1, prof["e_launches"]),
+            "residual_after": s.stats()["last_residual"],
+        }
+        if args.to_residual:
+            s.initialize()
+            t2 = time.perf_counter()
+            done, ok = s.run(args.to_residual, args.max_sweeps, 16, flags)
+            out.update(to_residual=args.to_residual, sweeps_to_residual=done, converged=ok,
+                       seconds_to_residual=time.perf_counter() - t2)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=810)
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--profile-steps", type=int, default=100)
     ap.add_argument("--cpu-sample-steps", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["c1", "c2"], default="c1")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--to-residual", type=float, default=None)
+    ap.add_argument("--max-sweeps", type=int, default=20000)
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="after the run, copy this many bytes once (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -108,6 +156,14 @@ def main() -> int:
     from pe_amd import _native
 
     info = _native.device_info()
+    if args.workload == "c2":
+        args.steps = args.steps or 200
+        out = run_c2(args)
+        if args.copy_probe:
+            out["copy_probe_gbps"] = _native.copy_bandwidth(args.copy_probe, 1)
+        print(json.dumps(out), flush=True)
+        return 0
+    args.steps = args.steps or 810
     config, grid, cfg = c1_config()
     n = grid.states
     print(f"[bench] {info['name']}: C1 5D HJI {n} nodes, warmup {args.warmup}, "
@@ -173,6 +229,8 @@ def main() -> int:
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_sample_steps)
+    if args.copy_probe:
+        out["copy_probe_gbps"] = _native.copy_bandwidth(args.copy_probe, 1)
     print(json.dumps(out), flush=True)
     return 0
 

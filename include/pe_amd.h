@@ -31,6 +31,11 @@ int pe_abi_version(void);
 /* sizeof of a struct of this header by name ("pe_sl_car", ...), or -1;
    lets bindings check their struct mirrors without a device. */
 int pe_abi_struct_size(const char *name);
+/* Device-to-device copy of `bytes` bytes by a plain streaming kernel,
+   `repeats` times; *gbps receives (read + write bytes) / time.  The
+   calibration point for HBM traffic counters and the achievable copy
+   bandwidth.  No counterpart in the reference. */
+int pe_copy_bandwidth(int64_t bytes, int32_t repeats, double *gbps);
 /* Name, compute units and HBM bytes of the current device. */
 int pe_device_info(char *name, int name_len, int32_t *compute_units,
                    int64_t *hbm_bytes);
@@ -60,7 +65,16 @@ typedef struct pe_hji5d_config {
                          (gpu_relative.py:161-166)                          */
   int32_t vp_chunk;   /* vP values walked per GPU thread (0: automatic)    */
   double p_length, p_width, e_length, e_width; /* rectangle footprints */
+  int32_t flags;      /* PE_HJI5D_* launch options (0: defaults)           */
+  int32_t reserved;
 } pe_hji5d_config;
+
+/* pe_hji5d_config.flags: number blocks by psi-plane tiles (the slower,
+   earlier block order), for comparison runs. */
+#define PE_HJI5D_LAYOUT_PLANE 1
+/* pe_hji5d_config.flags: use the generic stage template for FP32 too
+   (comparison runs; the default FP32 kernel is instruction-lean). */
+#define PE_HJI5D_GENERIC 2
 
 typedef struct pe_hji5d_stats {
   int64_t states;           /* grid nodes                                  */

@@ -15,10 +15,14 @@
 // owns one (vE, psi, y, x) column and walks a chunk of `vpc` values of the
 // vP axis (the slowest, stride `sv`) keeping a five-value register window
 // V(vP-2 .. vP+2), so vP neighbours are loaded once per node instead of five
-// times; shorter chunks trade that reuse for more threads in flight.  Blocks
-// are numbered with the vP chunk and vE fastest, then remapped so that each
-// of the 8 XCDs processes one contiguous part of the grid: blocks resident
-// together on an XCD share their vE / psi / y halos through its L2.
+// times.  A block covers 256 consecutive (y, x) nodes of one psi plane.
+// Blocks are numbered (xy tile, psi, vE) with vE fastest (layout 1) and then
+// remapped so that each of the 8 XCDs processes one contiguous part of that
+// order: the blocks of every psi and vE of an xy tile are then resident
+// together on one XCD and share their psi and vE halos through its L2
+// instead of fetching them again from HBM.  (Layout 0, psi-plane tiles with
+// vE fastest, leaves psi neighbours 16-33 blocks apart: it fetched each
+// value about five times per stage on C1.)
 #pragma once
 
 #include "pe_common.h"
@@ -40,7 +44,10 @@ struct Hji5dGrid {
   double exit_r2, exit_value;
   int vpc;          // vP values per thread
   int nvpc;         // ceil(nvp / vpc): vP chunks per column
-  int blocks;       // tiles * nve * nvpc
+  int xytiles;      // ceil(nx * nx / 256): tiles of one (y, x) plane
+  int layout;       // block order: 0 psi-plane tiles; 1 (xy tile, psi, vE)
+  int fast;         // FP32 launches use hji5d_stage_fast (hji5d_fast.h)
+  int blocks;       // tiles * nve * nvpc (layout 0) or xytiles*nh*nve*nvpc
   int per_xcd;      // ceil(blocks / 8)
 };
 
@@ -116,13 +123,23 @@ __global__ __launch_bounds__(256) void hji5d_stage(StageArgs<R> a,
   if (block < g.blocks) {
     const int chunk = block % g.nvpc;
     const int ie = (block / g.nvpc) % g.nve;
-    const int tile = block / (g.nvpc * g.nve);
+    const int rest = block / (g.nvpc * g.nve);
+    int jp, ih, r;
+    bool inside_plane;
+    if (g.layout == 1) {
+      ih = rest % g.nh;
+      r = (rest / g.nh) * 256 + (int)threadIdx.x;  // (y, x) index
+      inside_plane = r < g.nx2;
+      jp = ih * g.nx2 + r;
+    } else {
+      jp = rest * 256 + (int)threadIdx.x;
+      inside_plane = jp < g.plane;
+      ih = jp / g.nx2;
+      r = jp - ih * g.nx2;
+    }
     const int iv0 = chunk * g.vpc;
     const int iv1 = iv0 + g.vpc < g.nvp ? iv0 + g.vpc : g.nvp;
-    const int jp = tile * 256 + (int)threadIdx.x;
-    if (jp < g.plane) {
-      const int ih = jp / g.nx2;
-      const int r = jp - ih * g.nx2;
+    if (inside_plane) {
       const int iy = r / g.nx;
       const int ix = r - iy * g.nx;
       const R dx = R(g.dx), dphi = R(g.dphi), dvp = R(g.dvp), dve = R(g.dve);

@@ -51,6 +51,52 @@ extern "C" int pe_abi_struct_size(const char *name) {
   return -1;
 }
 
+namespace pe {
+__global__ __launch_bounds__(256) void copy_probe(const float4 *__restrict__ src,
+                                                  float4 *__restrict__ dst,
+                                                  long long n) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = src[i];
+}
+}  // namespace pe
+
+extern "C" int pe_copy_bandwidth(int64_t bytes, int32_t repeats, double *gbps) {
+  pe::clear_error();
+  if (bytes < 16 || bytes % 16 || repeats < 1 || !gbps)
+    return pe::fail(PE_ERR_INVALID, "bytes must be a positive multiple of 16");
+  int rc = pe::require_device(nullptr);
+  if (rc) return rc;
+  void *a = nullptr, *b = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto body = [&]() -> int {
+    PE_HIP(hipMalloc(&a, bytes));
+    PE_HIP(hipMalloc(&b, bytes));
+    PE_HIP(hipMemset(a, 0, bytes));
+    PE_HIP(hipEventCreate(&e0));
+    PE_HIP(hipEventCreate(&e1));
+    const long long n = bytes / 16;
+    hipLaunchKernelGGL(pe::copy_probe, dim3(8192), dim3(256), 0, 0, (const float4 *)a,
+                       (float4 *)b, n);
+    PE_HIP(hipEventRecord(e0, 0));
+    for (int r = 0; r < repeats; r++)
+      hipLaunchKernelGGL(pe::copy_probe, dim3(8192), dim3(256), 0, 0, (const float4 *)a,
+                         (float4 *)b, n);
+    PE_HIP(hipEventRecord(e1, 0));
+    PE_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    PE_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *gbps = 2.0 * bytes * repeats / (ms * 1e-3) / 1e9;
+    return PE_OK;
+  };
+  rc = body();
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  return rc;
+}
+
 extern "C" int pe_device_info(char *name, int name_len, int32_t *compute_units,
                               int64_t *hbm_bytes) {
   int dev = 0;

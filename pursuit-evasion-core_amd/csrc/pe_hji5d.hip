@@ -13,13 +13,27 @@
 #include <limits>
 #include <vector>
 
-#include "hji5d_stage.h"
+#include "hji5d_fast.h"
 
 namespace pe {
 
 void hji5d_launch_f32(const Hji5dGrid &g, const StageArgs<float> &a,
                       bool blend, bool final_stage, hipStream_t stream) {
   const dim3 grid((unsigned)(g.per_xcd * 8)), block(256);
+  if (g.fast) {
+    if (g.order == 2) {
+      if (blend)
+        hipLaunchKernelGGL((hji5d_stage_fast<true, 2>), grid, block, 0, stream, a, g,
+                           (int)final_stage);
+      else
+        hipLaunchKernelGGL((hji5d_stage_fast<false, 2>), grid, block, 0, stream, a, g,
+                           (int)final_stage);
+    } else {
+      hipLaunchKernelGGL((hji5d_stage_fast<false, 1>), grid, block, 0, stream, a, g,
+                         (int)final_stage);
+    }
+    return;
+  }
   if (blend)
     hipLaunchKernelGGL((hji5d_stage<float, false, true>), grid, block, 0,
                        stream, a, g, (int)final_stage);
@@ -91,6 +105,8 @@ struct pe_hji5d {
   double limit = 0;
   long long n = 0;
   float *a = nullptr, *b = nullptr, *arrival = nullptr;
+  float *a_alloc = nullptr, *b_alloc = nullptr;  // a, b with guard margins
+  long long margin = 0;                          // floats before a and b
   void *sn = nullptr, *cs = nullptr;
   // Reduction ring: slot k holds [increase, |change|, nonfinite, pad].
   unsigned long long *ring = nullptr;
@@ -120,7 +136,8 @@ static int validate(const pe_hji5d_config &c) {
       c.e_acceleration <= 0 || c.p_curvature <= 0 || c.e_curvature <= 0 ||
       c.p_min >= c.p_max || c.e_min >= c.e_max ||
       (c.precision != 0 && c.precision != 1) ||
-      (c.capture != 0 && c.capture != 1) || c.vp_chunk < 0)
+      (c.capture != 0 && c.capture != 1) || c.vp_chunk < 0 ||
+      (c.flags & ~(PE_HJI5D_LAYOUT_PLANE | PE_HJI5D_GENERIC)) != 0)
     return fail(PE_ERR_INVALID, "Invalid solver options");
   if (c.exit_radius > 0 && !(c.exit_radius < c.extent))
     return fail(PE_ERR_INVALID,
@@ -283,9 +300,15 @@ extern "C" int pe_hji5d_create(const pe_hji5d_config *cfg, pe_hji5d **out) {
   // chunk lengths 1, 2, 3, 5, 8 and 15).
   g.vpc = c.vp_chunk > 0 ? std::min(c.vp_chunk, g.nvp) : g.nvp;
   g.nvpc = (g.nvp + g.vpc - 1) / g.vpc;
-  g.blocks = g.tiles * g.nve * g.nvpc;
+  g.xytiles = (g.nx2 + 255) / 256;
+  g.layout = (c.flags & PE_HJI5D_LAYOUT_PLANE) ? 0 : 1;
+  g.blocks = (g.layout == 1 ? g.xytiles * g.nh : g.tiles) * g.nve * g.nvpc;
   g.per_xcd = (g.blocks + 7) / 8;
   h->n = g.sv * g.nvp;
+  // The instruction-lean FP32 kernel (whole vP axis per thread, layout 1,
+  // 32-bit byte offsets).
+  g.fast = c.precision == 0 && g.layout == 1 && g.nvpc == 1 &&
+           !(c.flags & PE_HJI5D_GENERIC) && double(h->n) * 4.0 < 4.0e9;
   {  // native_relative.cpp:76-81
     const double VP = std::max(std::fabs(c.p_min), std::fabs(c.p_max));
     const double VE = std::max(std::fabs(c.e_min), std::fabs(c.e_max));
@@ -315,8 +338,22 @@ extern "C" int pe_hji5d_create(const pe_hji5d_config *cfg, pe_hji5d **out) {
     return PE_OK;
   };
   TRY(mk_stream());
-  TRY(alloc((void **)&h->a, h->n * sizeof(float)));
-  TRY(alloc((void **)&h->b, h->n * sizeof(float)));
+  // Fields with zeroed guard margins of 2 vE strides (+ alignment) on both
+  // sides: the FP32 kernel loads x, y and vE neighbours unconditionally.
+  h->margin = 2 * g.se + 64;
+  const size_t padded = (h->n + 2 * h->margin) * sizeof(float);
+  TRY(alloc((void **)&h->a_alloc, padded));
+  TRY(alloc((void **)&h->b_alloc, padded));
+  h->a = h->a_alloc + h->margin;
+  h->b = h->b_alloc + h->margin;
+  {
+    auto zero = [&]() -> int {
+      PE_HIP(hipMemset(h->a_alloc, 0, padded));
+      PE_HIP(hipMemset(h->b_alloc, 0, padded));
+      return PE_OK;
+    };
+    TRY(zero());
+  }
   TRY(alloc((void **)&h->arrival, h->n * sizeof(float)));
   TRY(alloc((void **)&h->ring, (size_t)h->ring_slots * 4 * 8));
   TRY(alloc((void **)&h->counter, 8));
@@ -516,7 +553,7 @@ extern "C" void pe_hji5d_destroy(pe_hji5d *h) {
   hipSetDevice(h->device);
   if (h->stream) hipStreamSynchronize(h->stream);
   for (hipEvent_t e : h->events) hipEventDestroy(e);
-  for (void *p : {(void *)h->a, (void *)h->b, (void *)h->arrival, h->sn, h->cs,
+  for (void *p : {(void *)h->a_alloc, (void *)h->b_alloc, (void *)h->arrival, h->sn, h->cs,
                   (void *)h->ring, (void *)h->counter})
     if (p) hipFree(p);
   if (h->t0) hipEventDestroy(h->t0);

@@ -89,7 +89,13 @@ class HJI5DConfig(ctypes.Structure):
         ("p_width", ctypes.c_double),
         ("e_length", ctypes.c_double),
         ("e_width", ctypes.c_double),
+        ("flags", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
+
+
+PE_HJI5D_LAYOUT_PLANE = 1
+PE_HJI5D_GENERIC = 2
 
 
 class HJI5DStats(ctypes.Structure):
@@ -183,6 +189,7 @@ SIGNATURES = {
     "pe_abi_version": (ctypes.c_int, []),
     "pe_abi_struct_size": (ctypes.c_int, [ctypes.c_char_p]),
     "pe_device_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, P, P]),
+    "pe_copy_bandwidth": (ctypes.c_int, [I64, I32, P]),
     "pe_hji5d_create": (ctypes.c_int, [ctypes.POINTER(HJI5DConfig), P]),
     "pe_hji5d_advance_to": (ctypes.c_int, [P, F64, I64]),
     "pe_hji5d_step": (ctypes.c_int, [P, F64, I64]),
@@ -246,6 +253,12 @@ def device_info():
     hbm = ctypes.c_int64()
     check(lib().pe_device_info(name, 256, ctypes.byref(cus), ctypes.byref(hbm)))
     return dict(name=name.value.decode(), compute_units=cus.value, hbm_bytes=hbm.value)
+
+
+def copy_bandwidth(nbytes, repeats=10):
+    gbps = ctypes.c_double()
+    check(lib().pe_copy_bandwidth(int(nbytes), int(repeats), ctypes.byref(gbps)))
+    return gbps.value
 
 
 class HJI5D:
