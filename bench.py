@@ -133,6 +133,55 @@ def run_c2(args):
     return out
 
 
+def run_c4(args):
+    """C4: many seeded maps (9x9 positions, 8 headings, 4 speeds per car)
+    solved together, one launch per kernel for all maps, sweeps replayed
+    from a captured hipGraph."""
+    from pe_amd import _native
+    from pe_amd.local_game.grid_game import GameSolver
+    from pe_amd.local_game.reference import benchmark_game
+
+    t0 = time.perf_counter()
+    tables = []
+    for k in range(args.maps):
+        tables.append(benchmark_game(1000 + k, positions=9, headings=8, speeds=4)[4])
+        if k % 16 == 15:
+            print(f"[bench] C4 tables {k + 1}/{args.maps} ({time.perf_counter() - t0:.1f} s)",
+                  file=sys.stderr, flush=True)
+    build_s = time.perf_counter() - t0
+    flags = _native.PE_SL_EARLY_EXIT | _native.PE_SL_ORDERED | _native.PE_SL_GRAPH
+    with GameSolver(tables) as s:
+        st0 = s.stats()
+        if args.warmup:
+            s.sweep(args.warmup, flags)
+        t1 = time.perf_counter()
+        s.sweep(args.steps, flags)
+        wall = time.perf_counter() - t1
+        res = s.trace()[-1]
+        out = {
+            "workload": f"C4 {args.maps} seeded 30-rectangle maps, 9x9 positions x 8 headings x "
+                        "4 speeds per car, batched, hipGraph-replayed sweeps",
+            "maps": args.maps, "legal_pairs": st0["pairs"], "active_pairs": st0["active_pairs"],
+            "dense_pairs_per_map": (9 * 9 * 8 * 4) ** 2, "table_build_s": build_s,
+            "sweeps": args.steps, "ms_per_sweep": wall / args.steps * 1e3,
+            "legal_pair_updates_per_s": st0["pairs"] * args.steps / wall,
+            "active_pair_updates_per_s": st0["active_pairs"] * args.steps / wall,
+            "max_residual_after": float(res.max()),
+        }
+        if args.to_residual:
+            s.initialize()
+            t2 = time.perf_counter()
+            done, ok = 0, False
+            while done < args.max_sweeps and not ok:
+                d, ok = s.run(args.to_residual, min(512, args.max_sweeps - done), 16, flags)
+                done += d
+                print(f"[bench] C4 {done} sweeps, max residual {s.trace()[-1].max():.3e}",
+                      file=sys.stderr, flush=True)
+            out.update(to_residual=args.to_residual, sweeps_to_residual=done, converged=ok,
+                       seconds_to_residual=time.perf_counter() - t2)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,7 +190,8 @@ def main() -> int:
     ap.add_argument("--profile-steps", type=int, default=100)
     ap.add_argument("--cpu-sample-steps", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["c1", "c2"], default="c1")
+    ap.add_argument("--workload", choices=["c1", "c2", "c4"], default="c1")
+    ap.add_argument("--maps", type=int, default=128)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--to-residual", type=float, default=None)
     ap.add_argument("--max-sweeps", type=int, default=20000)
@@ -156,9 +206,9 @@ def main() -> int:
     from pe_amd import _native
 
     info = _native.device_info()
-    if args.workload == "c2":
+    if args.workload in ("c2", "c4"):
         args.steps = args.steps or 200
-        out = run_c2(args)
+        out = run_c2(args) if args.workload == "c2" else run_c4(args)
         if args.copy_probe:
             out["copy_probe_gbps"] = _native.copy_bandwidth(args.copy_probe, 1)
         print(json.dumps(out), flush=True)

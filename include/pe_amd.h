@@ -231,7 +231,8 @@ void pe_sl_destroy(pe_sl *h);
  * learned value nets (local_game/learning.py, training.py) use PyTorch.
  * Input per point: 8 state coordinates [xP, yP, thP, vP, xE, yE, thE, vE]
  * then `extra` map features; features = normalised state, `rff` random
- * Fourier features (sin, cos), extras, zero padding to a multiple of 32.
+ * Fourier features (sin, cos), extras, E's offset in P's frame, cos and sin
+ * of the relative heading, the distance, zero padding to a multiple of 16.
  * Loss: weighted MSE with weight 1 + 4 exp(-|y| / 0.2)
  * (local_game/training.py:351-352); optimiser AdamW (training.py:185-188).
  * ====================================================================== */

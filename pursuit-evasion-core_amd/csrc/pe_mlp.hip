@@ -2,7 +2,9 @@
 // hand-written BF16 MFMA kernels (FP32 accumulation) on gfx950.
 //
 // Network (input -> 4 x 256 SiLU -> 1):
-//   features f(x) = [xn (8), sin(B xn) (16), cos(B xn) (16), extra (3), 0 pad]
+//   features f(x) = [xn (8), sin(B xn) (16), cos(B xn) (16), extra (3),
+//                    E offset in P's frame (2), cos/sin relative heading (2),
+//                    distance (1), 0 pad to a multiple of 16]
 //   with xn the normalised 8D pair state [xP, yP, thP, vP, xE, yE, thE, vE],
 //   B a fixed Gaussian matrix (random Fourier features) and `extra` the
 //   per-node map features (contact payoff phi, P clearance, E clearance);
@@ -33,6 +35,7 @@ namespace pe {
 constexpr int kTile = 64;   // output tile edge per block
 constexpr int kStep = 32;   // K per LDS stage (one MFMA)
 constexpr int kPad = 8;     // LDS row padding (bf16) against bank conflicts
+constexpr int kRelative = 5;  // relative-frame features appended by mlp_features
 
 __device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }
 __device__ __forceinline__ float silu(float z) { return z / (1.f + __expf(-z)); }
@@ -64,6 +67,44 @@ struct GemmArgs {
   long long ldz;
 };
 
+// Loads the [64 rows x 32 k] tile X(row0 + r, k0 + c) into T[r][c].
+// X(row, k) = X[row * srow + k * sk].
+__device__ __forceinline__ void stage_tile(__bf16 (*T)[kStep + kPad], const __bf16 *X,
+                                           long long srow, long long sk, int row0, int rows,
+                                           int k0, int K, int tid) {
+  if (sk == 1) {
+    const int r = tid >> 2, c = (tid & 3) * 8;
+    const int row = row0 + r;
+    if (row < rows && k0 + c + 8 <= K && (srow % 8) == 0) {
+      *reinterpret_cast<bf16x8 *>(&T[r][c]) =
+          *reinterpret_cast<const bf16x8 *>(X + row * srow + (k0 + c));
+    } else {
+      for (int t = 0; t < 8; t++) {
+        const int k = k0 + c + t;
+        T[r][c + t] = (row < rows && k < K) ? X[row * srow + k] : to_bf16(0.f);
+      }
+    }
+  } else if (srow == 1) {
+    const int c = tid >> 3, r = (tid & 7) * 8;  // k, first row of 8
+    const int k = k0 + c;
+    if (k < K && row0 + r + 8 <= rows && (sk % 8) == 0) {
+      const bf16x8 v = *reinterpret_cast<const bf16x8 *>(X + k * sk + (row0 + r));
+      for (int t = 0; t < 8; t++) T[r + t][c] = v[t];
+    } else {
+      for (int t = 0; t < 8; t++) {
+        const int row = row0 + r + t;
+        T[r + t][c] = (row < rows && k < K) ? X[row + k * sk] : to_bf16(0.f);
+      }
+    }
+  } else {
+    const int r = tid >> 2, c = (tid & 3) * 8;
+    for (int t = 0; t < 8; t++) {
+      const int row = row0 + r, k = k0 + c + t;
+      T[r][c + t] = (row < rows && k < K) ? X[row * srow + k * sk] : to_bf16(0.f);
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void mfma_gemm(GemmArgs g) {
   __shared__ __attribute__((aligned(16))) __bf16 As[kTile][kStep + kPad];
   __shared__ __attribute__((aligned(16))) __bf16 Bs[kTile][kStep + kPad];
@@ -73,35 +114,13 @@ __global__ __launch_bounds__(256) void mfma_gemm(GemmArgs g) {
   f32x4 acc[2][2];
   for (int i = 0; i < 2; i++)
     for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const bool a_vec = g.sak == 1 && (g.sam % 8) == 0;
-  const bool b_vec = g.sbk == 1 && (g.sbn % 8) == 0;
   for (int k0 = 0; k0 < g.K; k0 += kStep) {
     // Stage A(m0 + r, k0 + c) and B(k0 + c, n0 + r) into LDS, k-contiguous.
-    // 64 rows x 32 k = 2048 values per operand; 8 per thread.
-    {
-      const int r = tid >> 2, c = (tid & 3) * 8;  // row, first k of 8
-      const int m = m0 + r, n = n0 + r;
-      if (a_vec && m < g.M && k0 + c + 8 <= g.K) {
-        *reinterpret_cast<bf16x8 *>(&As[r][c]) =
-            *reinterpret_cast<const bf16x8 *>(g.A + m * g.sam + (k0 + c));
-      } else {
-        for (int t = 0; t < 8; t++) {
-          const int k = k0 + c + t;
-          As[r][c + t] = (m < g.M && k < g.K) ? g.A[m * g.sam + k * g.sak]
-                                              : to_bf16(0.f);
-        }
-      }
-      if (b_vec && n < g.N && k0 + c + 8 <= g.K) {
-        *reinterpret_cast<bf16x8 *>(&Bs[r][c]) =
-            *reinterpret_cast<const bf16x8 *>(g.B + n * g.sbn + (k0 + c));
-      } else {
-        for (int t = 0; t < 8; t++) {
-          const int k = k0 + c + t;
-          Bs[r][c + t] = (n < g.N && k < g.K) ? g.B[k * g.sbk + n * g.sbn]
-                                              : to_bf16(0.f);
-        }
-      }
-    }
+    // 64 rows x 32 k = 2048 values per operand; 8 per thread: either 8
+    // consecutive k of one row (k-contiguous operand), or 8 consecutive rows
+    // of one k (row-contiguous operand, transposed on the way into LDS).
+    stage_tile(As, g.A, g.sam, g.sak, m0, g.M, k0, g.K, tid);
+    stage_tile(Bs, g.B, g.sbn, g.sbk, n0, g.N, k0, g.K, tid);
     __syncthreads();
     const int fr = lane & 15, fk = (lane >> 4) * 8;
     bf16x8 a[2], b[2];
@@ -161,6 +180,14 @@ __global__ __launch_bounds__(256) void mlp_features(
   }
   c += s.rff;
   for (int d = 0; d < s.extra; d++) f[c++] = to_bf16(x[8 + d]);
+  // Relative frame: E's offset in P's frame, relative heading, distance.
+  const float cp = __cosf(x[2]), sp = __sinf(x[2]);
+  const float dx = x[4] - x[0], dy = x[5] - x[1];
+  f[c++] = to_bf16(cp * dx + sp * dy);
+  f[c++] = to_bf16(-sp * dx + cp * dy);
+  f[c++] = to_bf16(__cosf(x[6] - x[2]));
+  f[c++] = to_bf16(__sinf(x[6] - x[2]));
+  f[c++] = to_bf16(sqrtf(dx * dx + dy * dy));
   for (; c < s.width; c++) f[c] = to_bf16(0.f);
 }
 
@@ -523,7 +550,7 @@ extern "C" int pe_mlp_create(const pe_mlp_config *cfg, pe_mlp **out) {
   h->shape.raw = 8 + c.extra;
   h->shape.extra = c.extra;
   h->shape.rff = c.rff;
-  h->shape.width = ((8 + c.rff + c.extra + 31) / 32) * 32;
+  h->shape.width = ((8 + c.rff + c.extra + kRelative + 15) / 16) * 16;
   h->shape.hidden = c.hidden;
   h->layers = c.layers;
   for (int l = 0; l < c.layers; l++) {
@@ -547,7 +574,7 @@ extern "C" int pe_mlp_create(const pe_mlp_config *cfg, pe_mlp **out) {
       PE_HIP(hipMalloc(&h->Wh[l], sizeof(__bf16) * nw));
       PE_HIP(hipMalloc(&h->Wt[l], sizeof(__bf16) * nw));
       // He-style init for SiLU; features beyond the real inputs are zero.
-      const int real_in = l == 0 ? 8 + c.rff + c.extra : h->fan_in[l];
+      const int real_in = l == 0 ? 8 + c.rff + c.extra + kRelative : h->fan_in[l];
       std::vector<float> w(nw, 0.f);
       const float sd = std::sqrt(2.f / real_in);
       for (int o = 0; o < h->fan_out[l]; o++)

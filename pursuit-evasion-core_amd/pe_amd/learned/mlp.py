@@ -103,7 +103,7 @@ class ValueMLP:
     def unflatten(self, flat):
         """[(W [out, in], b [out]) per hidden layer] + [(w [hidden], b [1])]."""
         c = self.config
-        width = ((8 + c.rff + c.extra + 31) // 32) * 32
+        width = ((8 + c.rff + c.extra + 5 + 15) // 16) * 16
         shapes, at, out = [], 0, []
         fan_in = width
         for _ in range(c.layers):

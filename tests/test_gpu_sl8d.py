@@ -211,3 +211,22 @@ def test_c2_full_size(gpu, oracle_lib, c2_game):
     assert len(trace) == 40 and (trace >= 0).all()
     print(f"C2 40 sweeps: residual {trace[-1]:.3e}, captured fraction "
           f"{np.mean(V40[inside] <= 0):.4f}")
+
+
+def test_batched_maps_equal_individual_solves(gpu):
+    """C4 layout: several maps in one handle (one launch per kernel, graph
+    replay) give bit-identical fields to solving each map alone."""
+    from pe_amd.local_game.reference import benchmark_game
+
+    print("[gpu] batched maps", flush=True)
+    tables = [benchmark_game(1000 + k, positions=7, headings=8, speeds=3, rectangles=12)[4]
+              for k in range(4)]
+    with GameSolver(tables) as s:
+        s.sweep(10, EARLY | ORDERED | GRAPH)
+        batched = [s.read(k) for k in range(4)]
+        traces = s.trace()
+    for k, t in enumerate(tables):
+        with GameSolver(t) as s1:
+            s1.sweep(10, EARLY | ORDERED)
+            np.testing.assert_array_equal(s1.read(), batched[k])
+            np.testing.assert_array_equal(s1.trace()[:, 0], traces[:, k])
