@@ -190,6 +190,8 @@ def main() -> int:
     ap.add_argument("--profile-steps", type=int, default=100)
     ap.add_argument("--cpu-sample-steps", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="skip the 8D contact-game line after the 5D headline")
     ap.add_argument("--workload", choices=["c1", "c2", "c4"], default="c1")
     ap.add_argument("--maps", type=int, default=128)
     ap.add_argument("--seed", type=int, default=0)
@@ -199,9 +201,9 @@ def main() -> int:
                     help="after the run, copy this many bytes once (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the 5D grid would shard by vP slabs with a +-2-plane halo; see "
-              "DESIGN.md).  Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path (the 5D "
+              "grid would shard by vP slabs with a +-2-plane halo exchange per "
+              "stage).  Run with --gpus 1.", file=sys.stderr)
         return 2
     from pe_amd import _native
 
@@ -240,7 +242,7 @@ def main() -> int:
     traffic = load_traffic()
     roofline = {
         "bound": "hbm",
-        "kernel": "hji5d_stage<float,false,true> (RK2 blend stage)",
+        "kernel": "hji5d_stage_fast<true,2> (RK2 blend stage)",
         "achieved": round(achieved[1], 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
@@ -281,6 +283,23 @@ def main() -> int:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_sample_steps)
     if args.copy_probe:
         out["copy_probe_gbps"] = _native.copy_bandwidth(args.copy_probe, 1)
+    if not args.no_secondary:
+        # The 8D contact game beside the headline: sweep throughput and the
+        # plain DP's time to residual 1e-4 on the seed-0 map.
+        print("[bench] secondary: C1 from phi to per-step residual 1e-4", file=sys.stderr,
+              flush=True)
+        with _native.HJI5D(cfg) as h:
+            t0 = time.perf_counter()
+            ok = h.run_to_residual(1e-4, 30000, 16)
+            wall = time.perf_counter() - t0
+            st = h.stats()
+        res5 = {"tolerance": 1e-4, "converged": ok, "steps": st["internal_steps"],
+                "solve_time_s": st["time"], "seconds": wall,
+                "last_residual": st["last_residual"]}
+        print("[bench] secondary: 8D contact game (seed 0)", file=sys.stderr, flush=True)
+        c2 = argparse.Namespace(seed=0, warmup=10, steps=200, to_residual=1e-4,
+                                max_sweeps=20000)
+        out["secondary"] = {"hji5d_c1_to_residual": res5, "contact_game_c2": run_c2(c2)}
     print(json.dumps(out), flush=True)
     return 0
 

@@ -246,6 +246,8 @@ typedef struct pe_mlp_config {
   float state_shift[8]; /* normalisation: (x - shift) * scale               */
   float state_scale[8];
   float learning_rate, weight_decay, beta1, beta2, eps; /* AdamW */
+  float under_weight;   /* loss weight factor where yhat < y (>= 1; 1 = the
+                           symmetric weighted MSE)                          */
 } pe_mlp_config;
 
 typedef struct pe_mlp pe_mlp;

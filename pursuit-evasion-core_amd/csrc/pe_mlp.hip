@@ -208,16 +208,19 @@ __global__ __launch_bounds__(256) void mlp_head(const __bf16 *__restrict__ H,
 
 // Weighted MSE on the head output: weight 1 + 4 exp(-|y| / 0.2) (the
 // reference's training weighting, local_game/training.py:351-352).
+// Underestimates (yhat < y) are weighted `under` times more, which biases
+// the fit towards an upper envelope of the values.
 // dy = 2 w (yhat - y) / n; loss accumulated (sum of w (yhat - y)^2 / n).
 __global__ __launch_bounds__(256) void mlp_loss(const float *__restrict__ yhat,
                                                 const float *__restrict__ y,
-                                                long long n, float *__restrict__ dy,
+                                                long long n, float under,
+                                                float *__restrict__ dy,
                                                 float *__restrict__ loss) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   float l = 0.f;
   if (i < n) {
-    const float w = 1.f + 4.f * __expf(-fabsf(y[i]) / 0.2f);
     const float r = yhat[i] - y[i];
+    const float w = (1.f + 4.f * __expf(-fabsf(y[i]) / 0.2f)) * (r < 0.f ? under : 1.f);
     dy[i] = 2.f * w * r / (float)n;
     l = w * r * r / (float)n;
   }
@@ -375,7 +378,7 @@ static int backward(pe_mlp *h, long long n, float *loss_host) {
   if (rc) return rc;
   hipLaunchKernelGGL(fill_f32, dim3(1), dim3(64), 0, h->stream, h->loss, 1ll, 0.f);
   hipLaunchKernelGGL(mlp_loss, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream,
-                     h->yhat, h->yt, n, h->dy, h->loss);
+                     h->yhat, h->yt, n, h->cfg.under_weight, h->dy, h->loss);
   PE_LAUNCHED(mlp_loss);
   const int L = h->layers;
   const int top = h->fan_out[L - 1];
@@ -539,7 +542,8 @@ extern "C" int pe_mlp_create(const pe_mlp_config *cfg, pe_mlp **out) {
   *out = nullptr;
   const pe_mlp_config &c = *cfg;
   if (c.extra < 0 || c.extra > 8 || c.rff < 0 || c.rff % 2 || c.rff > 64 || c.hidden < 16 ||
-      c.hidden % 64 || c.layers < 1 || c.layers > 8 || !(c.learning_rate > 0))
+      c.hidden % 64 || c.layers < 1 || c.layers > 8 || !(c.learning_rate > 0) ||
+      !(c.under_weight >= 1.f && c.under_weight <= 1e4f))
     return fail(PE_ERR_INVALID, "invalid MLP configuration");
   int dev = 0;
   int rc = require_device(&dev);

@@ -171,6 +171,7 @@ class MLPConfig(ctypes.Structure):
         ("beta1", ctypes.c_float),
         ("beta2", ctypes.c_float),
         ("eps", ctypes.c_float),
+        ("under_weight", ctypes.c_float),
     ]
 
 

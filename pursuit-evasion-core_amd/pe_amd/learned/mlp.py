@@ -7,8 +7,11 @@
 
 Training follows the reference's supervised value-net conventions
 (local_game/training.py): weighted MSE with weight 1 + 4 exp(-|y| / 0.2)
-(:351-352) and AdamW (:185-188).  The network itself is new: the reference
-has no surrogate on this path.
+(:351-352) and AdamW (:185-188).  `under_weight` > 1 additionally weights
+underestimates (prediction below the label), so the fit leans towards an
+upper envelope: a warm start is only exact where it is not below the
+converged value.  The network itself is new: the reference has no surrogate
+on this path.
 """
 
 from __future__ import annotations
@@ -26,7 +29,7 @@ EXTRA = 3  # phi, clearance P, clearance E
 class ValueMLP:
     def __init__(self, *, shift, scale, extra=EXTRA, rff=32, hidden=256, layers=4, seed=0,
                  rff_scale=1.0, learning_rate=1e-3, weight_decay=1e-5, beta1=0.9,
-                 beta2=0.999, eps=1e-8):
+                 beta2=0.999, eps=1e-8, under_weight=1.0):
         c = _native.MLPConfig()
         c.extra, c.rff, c.hidden, c.layers, c.seed = extra, rff, hidden, layers, seed
         c.rff_scale = rff_scale
@@ -35,6 +38,7 @@ class ValueMLP:
             c.state_scale[k] = float(scale[k])
         c.learning_rate, c.weight_decay = learning_rate, weight_decay
         c.beta1, c.beta2, c.eps = beta1, beta2, eps
+        c.under_weight = under_weight
         self.config = c
         self.inputs = 8 + extra
         self._h = ctypes.c_void_p()

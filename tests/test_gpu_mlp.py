@@ -7,7 +7,8 @@
   (the checker), run on exactly the bf16 feature rows the kernels see and
   the bf16-rounded weights.  The kernels keep activations and gradients in
   bf16 between layers, so agreement is to bf16 precision: relative error of
-  each gradient tensor <= 3e-2, cosine >= 0.999.
+  each gradient tensor <= 3e-2, cosine >= 0.999.  Both the symmetric loss
+  and the one that weights underestimates 4x are checked.
 * Training reduces the loss on a smooth target.
 """
 
@@ -34,8 +35,9 @@ def test_mfma_gemm_matches_float64(gpu, M, N, K, tb):
     assert err < 1e-5
 
 
-def make_net(seed=0):
-    return ValueMLP(shift=np.zeros(8), scale=np.full(8, 0.5), seed=seed, rff_scale=1.0)
+def make_net(seed=0, under_weight=1.0):
+    return ValueMLP(shift=np.zeros(8), scale=np.full(8, 0.5), seed=seed, rff_scale=1.0,
+                    under_weight=under_weight)
 
 
 def random_inputs(n, seed):
@@ -45,7 +47,7 @@ def random_inputs(n, seed):
     return x, y
 
 
-def torch_reference(net, feats, y):
+def torch_reference(net, feats, y, under_weight=1.0):
     torch = pytest.importorskip("torch")
     params = net.unflatten(net.get_params())
     ts = []
@@ -62,19 +64,20 @@ def torch_reference(net, feats, y):
     ts += [w4, b4]
     out = h @ w4 + b4
     yt = torch.tensor(y, dtype=torch.float64)
-    w = 1 + 4 * torch.exp(-yt.abs() / 0.2)
+    w = (1 + 4 * torch.exp(-yt.abs() / 0.2)) * torch.where(out < yt, under_weight, 1.0)
     loss = (w * (out - yt) ** 2).mean()
     loss.backward()
     return out.detach().numpy(), loss.item(), [t.grad.numpy().ravel() for t in ts]
 
 
-def test_forward_and_gradients_match_float64_autograd(gpu):
-    net = make_net()
+@pytest.mark.parametrize("under_weight", [1.0, 4.0])
+def test_forward_and_gradients_match_float64_autograd(gpu, under_weight):
+    net = make_net(under_weight=under_weight)
     x, y = random_inputs(512, 1)
-    print("[gpu] mlp forward/backward", flush=True)
+    print(f"[gpu] mlp forward/backward under_weight={under_weight}", flush=True)
     feats = net.features(x)
     pred = net(x)
-    ref_out, ref_loss, ref_grads = torch_reference(net, feats, y)
+    ref_out, ref_loss, ref_grads = torch_reference(net, feats, y, under_weight)
     rel = np.abs(pred - ref_out).max() / np.abs(ref_out).max()
     print(f"forward rel err {rel:.2e}")
     assert rel < 2e-2
