@@ -111,6 +111,8 @@ def run_c2(args):
         s.set_profiling(True)
         s.sweep(min(args.steps, 20), flags & ~_native.PE_SL_GRAPH)
         prof = s.stats()
+        s.sweep(min(args.steps, 20), (flags & ~_native.PE_SL_GRAPH) | _native.PE_SL_GLOBAL_GATHER)
+        prof_g = s.stats()
         s.set_profiling(False)
         out = {
             "workload": "C2 8D contact game, 11x11 positions (+-3 m) x 12 headings x 5 speeds "
@@ -121,7 +123,10 @@ def run_c2(args):
             "legal_pair_updates_per_s": st0["pairs"] * args.steps / wall,
             "active_pair_updates_per_s": st0["active_pairs"] * args.steps / wall,
             "p_side_ms": prof["p_ms"] / max(1, prof["p_launches"]),
+            "e_side_lds": bool(st0["lds_e_side"]),
             "e_side_ms": prof["e_ms"] / max(1, prof["e_launches"]),
+            "e_side_global_gather_ms": (prof_g["e_ms"] - prof["e_ms"]) /
+                                       max(1, prof_g["e_launches"] - prof["e_launches"]),
             "residual_after": s.stats()["last_residual"],
         }
         if args.to_residual:

@@ -75,6 +75,9 @@ typedef struct pe_hji5d_config {
 /* pe_hji5d_config.flags: use the generic stage template for FP32 too
    (comparison runs; the default FP32 kernel is instruction-lean). */
 #define PE_HJI5D_GENERIC 2
+/* pe_hji5d_config.flags: the FP32 kernel loads each node's neighbours just
+   before computing it instead of one vP node ahead (comparison runs). */
+#define PE_HJI5D_NO_PREFETCH 4
 
 typedef struct pe_hji5d_stats {
   int64_t states;           /* grid nodes                                  */
@@ -185,11 +188,13 @@ typedef struct pe_sl_game {
 #define PE_SL_EARLY_EXIT 1 /* stop a command once worst >= best (exact)     */
 #define PE_SL_ORDERED 2    /* try last sweep's argmin/argmax commands first */
 #define PE_SL_GRAPH 4      /* replay captured sweeps with a hipGraph        */
+#define PE_SL_GLOBAL_GATHER 8 /* E side gathers G from global memory instead
+                                 of LDS-staged rows (comparison runs)      */
 
 typedef struct pe_sl_stats {
   int64_t sweeps;
   int32_t maps;
-  int32_t reserved;
+  int32_t lds_e_side;     /* 1: the E side stages G rows in LDS         */
   int64_t pairs;          /* stored pairs, all maps                         */
   int64_t active_pairs;   /* pairs inside the exit radius, all maps         */
   double update_seconds;  /* device time of the sweeps                      */
@@ -223,6 +228,24 @@ int pe_sl_trace(pe_sl *h, double *out, int64_t capacity, int64_t *count);
 int pe_sl_get_stats(pe_sl *h, pe_sl_stats *out);
 int pe_sl_set_profiling(pe_sl *h, int32_t on);
 void pe_sl_destroy(pe_sl *h);
+
+/* ======================================================================
+ * Queries of a saved field history (reader of a completed contact solve).
+ *
+ * Replaces the host gather of ContactReference.fields
+ * (local_game/reference.py:423-439): for query i and frame t,
+ *   out[t, i] = sum_k w[i,k] * (kept[i,k] ? float(hist[t, ids[i,k]]) : exit)
+ * over `terms` product-stencil terms (256 for two 16-vertex stencils);
+ * hist is the packed float16 history [frames, states] (fields.f16).
+ * ====================================================================== */
+typedef struct pe_history pe_history;
+int pe_history_create(const uint16_t *f16, int64_t frames, int64_t states,
+                      pe_history **out);
+/* ids, w, kept: [n, terms]; out: [frames, n]. */
+int pe_history_query(pe_history *h, int64_t n, int32_t terms, const int64_t *ids,
+                     const float *w, const uint8_t *kept, float exit_value,
+                     float *out);
+void pe_history_destroy(pe_history *h);
 
 /* ======================================================================
  * Learned value surrogate: MLP with BF16 MFMA kernels (FP32 accumulation).

@@ -1,24 +1,40 @@
 // FP32 RK stage of the 5D relative HJI update, written for instruction
-// economy on gfx950 (the stage is VALU-issue bound at about 200
-// instructions per node in the generic template of hji5d_stage.h).
-// Measured on C1: 151 us (stage 1) and 178 us (blend) per launch against
-// 195 / 232 us for the generic template.  Wave-uniform branches around the
-// y / vE / vP boundary cases instead of selects were slower (176 / 196 us).
+// economy on gfx950.  The stage is VALU-issue bound (packed FP32 is not
+// used), so the cost of a node is its VALU instruction count.
 //
-// Same arithmetic as hji5d_stage<float, false, *> node by node
-// (reversible/native_relative.cpp:38-67 slopes, :124-172 update), reorganised:
+// Measured on C1 (34.2M nodes), blend / stage-1 launch:
+//   generic template hji5d_stage<float, false, *>            232 / 195 us
+//   vP-walking threads + guard margins + selects              178 / 151 us
+//     (about 200 VALU per node; wave-uniform branches around the boundary
+//     cases instead of selects were slower, 196 / 176 us)
+//   this version: raw differences, folded scale factors,
+//     mask-free interior vE blocks and vP iterations
+//
+// Same scheme as hji5d_stage<float, false, *> node by node
+// (reversible/native_relative.cpp:38-67 slopes, :124-172 update), with the
+// arithmetic reorganised (FP32 results agree to rounding, not bitwise):
+//  * ENO differences are formed in value units and the 1/h factors are
+//    folded into per-thread coefficients of the Hamiltonian: with
+//    S = em + ep and D = ep - em in value units, the gradient is
+//    g = S / (2h) and the Lax-Friedrichs jump is ep - em = D / h.
+//  * Boundary ends of x, y, vE: setting dm = dp at the first node (dp = dm
+//    at the last) makes the centre second difference exactly zero, which the
+//    ENO selection then picks, so em = ep = that one-sided difference
+//    (native_relative.cpp:47-54) with two selects instead of four.
 //  * The fields carry zeroed guard margins of at least 2 * (vE stride)
-//    floats on both sides, so every x, y and vE neighbour load is issued
-//    unconditionally from a launch-uniform base pointer plus the node's
-//    32-bit index; values read past a boundary are discarded by selects.
-//    psi is periodic and uses per-thread wrapped offsets.
-//  * ENO boundary cases are per-thread masks computed once, applied with
-//    selects instead of branches.
-//  * Along vP the thread walks the axis with a five-value window, and the
-//    first and second differences slide with it: one new difference of
-//    each kind per node instead of two and three.
+//    floats on both sides, so every x, y, vE and psi neighbour load is
+//    issued unconditionally from a launch-uniform base pointer plus the
+//    node's 32-bit byte offset; values read past a boundary are discarded
+//    by the selects.  psi is periodic and uniform per block, so its wrapped
+//    neighbours are uniform base pointers too.
+//  * Blocks whose vE index is at least 2 from both ends run a copy of the
+//    column without vE masks; along vP the thread walks the axis with a
+//    five-value window and sliding differences, and the interior vP nodes
+//    run without vP masks.
 //  * The rare arrival-time update stays in a (divergent) branch.
 #pragma once
+
+#include <type_traits>
 
 #include "hji5d_stage.h"
 
@@ -29,43 +45,247 @@ struct Axis {
   bool first, last, has_m2, has_p2;
 };
 
-// Load at a launch-uniform base plus a 32-bit byte offset (the SGPR-base +
-// VGPR-offset addressing form of global loads).
+// Load / store at a launch-uniform base plus a 32-bit byte offset (the
+// SGPR-base + VGPR-offset addressing form of global memory instructions).
 __device__ __forceinline__ float ld(const float *__restrict__ base, unsigned byte_off) {
   return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+__device__ __forceinline__ void st(float *__restrict__ base, unsigned byte_off, float v) {
+  *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + byte_off) = v;
 }
 
 __device__ __forceinline__ Axis axis_masks(int pos, int n) {
   return Axis{pos == 0, pos == n - 1, pos >= 2, pos + 2 < n};
 }
 
-// ENO2 slopes from the five values (non-periodic axis, masks per thread).
-__device__ __forceinline__ void eno2(float v, float m2, float m1, float p1, float p2,
-                                     const Axis &a, float inv_h, float inv_hh,
-                                     float half_h, float &dm, float &dp) {
-  const float Dm = (v - m1) * inv_h, Dp = (p1 - v) * inv_h;
-  const float sc = (p1 - 2.f * v + m1) * inv_hh;
-  float sl = (v - 2.f * m1 + m2) * inv_hh;
-  float sr = (p2 - 2.f * p1 + v) * inv_hh;
-  sl = a.has_m2 ? sl : sc;
-  sr = a.has_p2 ? sr : sc;
-  float em = Dm + half_h * (fabsf(sl) < fabsf(sc) ? sl : sc);
-  float ep = Dp - half_h * (fabsf(sc) < fabsf(sr) ? sc : sr);
-  // one-sided raw differences at the ends (native_relative.cpp:47-54)
-  em = a.first ? Dp : (a.last ? Dm : em);
-  ep = a.first ? Dp : (a.last ? Dm : ep);
-  dm = em;
-  dp = ep;
+// ENO2 one-sided differences in value units from the five values; returns
+// s = em + ep and d = ep - em.  MASKED = false for an interior node of a
+// non-periodic axis or any node of the periodic one.
+template <bool MASKED>
+__device__ __forceinline__ void eno2_raw(float v, float m2, float m1, float p1, float p2,
+                                         const Axis &a, float &s, float &d) {
+  float dm = v - m1, dp = p1 - v;
+  if (MASKED) {
+    dm = a.first ? dp : dm;
+    dp = a.last ? dm : dp;
+  }
+  const float sc = dp - dm;
+  float sl = dm - (m1 - m2), sr = (p2 - p1) - dp;
+  if (MASKED) {
+    sl = a.has_m2 ? sl : sc;
+    sr = a.has_p2 ? sr : sc;
+  }
+  const float em = fmaf(.5f, fabsf(sl) < fabsf(sc) ? sl : sc, dm);
+  const float ep = fmaf(-.5f, fabsf(sc) < fabsf(sr) ? sc : sr, dp);
+  s = em + ep;
+  d = ep - em;
 }
 
-__device__ __forceinline__ void eno1(float v, float m1, float p1, const Axis &a,
-                                     float inv_h, float &dm, float &dp) {
-  const float Dm = (v - m1) * inv_h, Dp = (p1 - v) * inv_h;
-  dm = a.first ? Dp : (a.last ? Dm : Dm);
-  dp = a.first ? Dp : (a.last ? Dm : Dp);
+// ENO1 in value units.
+template <bool MASKED>
+__device__ __forceinline__ void eno1_raw(float v, float m1, float p1, const Axis &a, float &s,
+                                         float &d) {
+  float dm = v - m1, dp = p1 - v;
+  if (MASKED) {
+    dm = a.first ? dp : dm;
+    dp = a.last ? dm : dp;
+  }
+  s = dm + dp;
+  d = dp - dm;
 }
 
-template <bool BLEND, int ORDER>
+// Per-thread constants of one (vE, psi, y, x) column.
+struct Column {
+  unsigned j;            // element index of the node at vP index 0
+  float ve_cs;           // ve cos(psi)
+  float cx, cp;          // 1/(2 dx), 1/(2 dphi)
+  float cxy, cxx;        // cx y, cx x
+  float cby, caby;       // cx by, cx |by|
+  float cay, cax;        // cx |y|, cx |x|
+  float wecp;            // we cp
+  float e_lo, e_hi;      // elow ce, ehigh ce (control bounds at the vE ends)
+  float aev;             // ae ce
+  Axis X, Y, E;
+};
+
+// The 16 neighbour values of one node (m2, m1, p1, p2 along x, y, vE, psi)
+// and, in the blend stage, the previous value.
+struct Neighbours {
+  float x[4], y[4], e[4], h[4], prev;
+};
+
+template <bool BLEND, int ORDER, bool EINT, bool PREFETCH>
+__device__ __forceinline__ void hji5d_column(const StageArgs<float> &a, const Hji5dGrid &g,
+                                             const Column &c, const float *__restrict__ hb[4],
+                                             int final_stage, float &inc_max, float &chg_max,
+                                             bool &nonfinite) {
+  const float *__restrict__ s0 = a.src;
+  const int nx = g.nx, se = (int)g.se;
+  const float *__restrict__ sxm2 = s0 - 2, *__restrict__ sxm1 = s0 - 1,
+                            *__restrict__ sxp1 = s0 + 1, *__restrict__ sxp2 = s0 + 2;
+  const float *__restrict__ sym2 = s0 - 2 * nx, *__restrict__ sym1 = s0 - nx,
+                            *__restrict__ syp1 = s0 + nx, *__restrict__ syp2 = s0 + 2 * nx;
+  const float *__restrict__ sem2 = s0 - 2 * se, *__restrict__ sem1 = s0 - se,
+                            *__restrict__ sep1 = s0 + se, *__restrict__ sep2 = s0 + 2 * se;
+  const float dvp = float(g.dvp);
+  const float ap = float(g.ap);
+  const float cv = .5f / dvp, apv = ap * cv;
+  const float dt = a.dt;
+  const int nvp = g.nvp;
+  const unsigned sv = (unsigned)g.sv;
+  unsigned j = c.j;
+  // vP window w2..w5 = V(iv .. iv+3); dpc = w3 - w2 (forward difference at
+  // iv), dpp = the forward difference at iv - 1, c2p / r2p = second
+  // differences at iv - 1 / iv.
+  float w2 = ld(s0, j << 2);
+  float w3 = ld(s0, (j + sv) << 2);
+  float w4 = ld(s0, (j + 2 * sv) << 2);
+  float dpc = w3 - w2, dpp = 0.f, c2p = 0.f, r2p = 0.f;
+  auto load = [&](unsigned jb, Neighbours &n) {
+    if (ORDER == 2) {
+      n.x[0] = ld(sxm2, jb);
+      n.x[3] = ld(sxp2, jb);
+      n.y[0] = ld(sym2, jb);
+      n.y[3] = ld(syp2, jb);
+      n.e[0] = ld(sem2, jb);
+      n.e[3] = ld(sep2, jb);
+      n.h[0] = ld(hb[0], jb);
+      n.h[3] = ld(hb[3], jb);
+    }
+    n.x[1] = ld(sxm1, jb);
+    n.x[2] = ld(sxp1, jb);
+    n.y[1] = ld(sym1, jb);
+    n.y[2] = ld(syp1, jb);
+    n.e[1] = ld(sem1, jb);
+    n.e[2] = ld(sep1, jb);
+    n.h[1] = ld(hb[1], jb);
+    n.h[2] = ld(hb[2], jb);
+    if (BLEND) n.prev = ld(a.dst, jb);
+  };
+  // With PREFETCH the neighbours of the next vP node are requested before
+  // the current node is computed, so each wave overlaps its own load
+  // latency with its arithmetic.
+  Neighbours cur;
+  if (PREFETCH) load(j << 2, cur);
+
+  auto node = [&](int iv, auto interior) {
+    constexpr bool VINT = decltype(interior)::value;
+    const float w5 = iv + 3 < nvp ? ld(s0, (j + 3 * sv) << 2) : 0.f;
+    Neighbours nb;
+    if (PREFETCH) {
+      nb = cur;
+      if (VINT || iv + 1 < nvp) load((j + sv) << 2, cur);
+    } else {
+      load(j << 2, nb);
+    }
+    // vP slopes (sliding): dm = dpp, dp = dpc, centre second difference c2
+    float sv_, dv_;
+    {
+      const float dn = w4 - w3;
+      const float r2 = dn - dpc, c2 = r2p, l2 = c2p;
+      float dm = dpp, dp = dpc;
+      if (ORDER == 1) {
+        if (!VINT) {
+          dm = iv == 0 ? dp : dm;
+          dp = iv == nvp - 1 ? dm : dp;
+        }
+        sv_ = dm + dp;
+        dv_ = dp - dm;
+      } else {
+        const float sl2 = (VINT || iv >= 2) ? l2 : c2;
+        const float sr2 = (VINT || iv + 2 < nvp) ? r2 : c2;
+        float em = fmaf(.5f, fabsf(sl2) < fabsf(c2) ? sl2 : c2, dm);
+        float ep = fmaf(-.5f, fabsf(c2) < fabsf(sr2) ? c2 : sr2, dp);
+        if (!VINT) {
+          const bool first = iv == 0, last = iv == nvp - 1;
+          em = first ? dp : (last ? dm : em);
+          ep = first ? dp : (last ? dm : ep);
+        }
+        sv_ = em + ep;
+        dv_ = ep - em;
+      }
+      dpp = dpc;
+      dpc = dn;
+      c2p = c2;
+      r2p = r2;
+    }
+    const unsigned jb = j << 2;
+    const float v = w2;
+    float Sx, Dx, Sy, Dy, Sp, Dp, Se, De;
+    if (ORDER == 1) {
+      eno1_raw<true>(v, nb.x[1], nb.x[2], c.X, Sx, Dx);
+      eno1_raw<true>(v, nb.y[1], nb.y[2], c.Y, Sy, Dy);
+      eno1_raw<!EINT>(v, nb.e[1], nb.e[2], c.E, Se, De);
+      eno1_raw<false>(v, nb.h[1], nb.h[2], c.E, Sp, Dp);
+    } else {
+      eno2_raw<true>(v, nb.x[0], nb.x[1], nb.x[2], nb.x[3], c.X, Sx, Dx);
+      eno2_raw<true>(v, nb.y[0], nb.y[1], nb.y[2], nb.y[3], c.Y, Sy, Dy);
+      eno2_raw<!EINT>(v, nb.e[0], nb.e[1], nb.e[2], nb.e[3], c.E, Se, De);
+      eno2_raw<false>(v, nb.h[0], nb.h[1], nb.h[2], nb.h[3], c.E, Sp, Dp);
+    }
+    // Hamiltonian (native_relative.cpp:124-165) in terms of S and D:
+    //   bx gx + by gy - wp |y gx - x gy - gp| + we |gp|
+    //   + min(plow gv, phigh gv) + max(elow ge, ehigh ge)
+    //   + 1/2 [(|bx| + wp |y|)(xp - xm) + (|by| + wp |x|)(yp - ym)
+    //          + (wp + we)(pp - pm) + ap (vp+ - vp-) + ae (ve+ - ve-)]
+    const float vp = a.vpk[2 * iv], wp = a.vpk[2 * iv + 1];  // uniform: scalar loads
+    const float bxc = (c.ve_cs - vp) * c.cx;
+    float H = fmaf(bxc, Sx, c.cby * Sy);
+    const float inner = fmaf(c.cxy, Sx, fmaf(-c.cxx, Sy, -c.cp * Sp));
+    H = fmaf(-wp, fabsf(inner), H);
+    H = fmaf(c.wecp, fabsf(Sp), H);
+    // Control terms: at interior speeds min(-a g, a g) = -a |g| and
+    // max(-a g, a g) = a |g| exactly.
+    if (VINT) {
+      H = fmaf(-apv, fabsf(sv_), H);
+    } else {
+      const float p_lo = iv == 0 ? 0.f : -apv, p_hi = iv == nvp - 1 ? 0.f : apv;
+      const float pa = p_lo * sv_, pb = p_hi * sv_;
+      H += pb < pa ? pb : pa;
+    }
+    if (EINT) {
+      H = fmaf(c.aev, fabsf(Se), H);
+    } else {
+      const float ea = c.e_lo * Se, eb = c.e_hi * Se;
+      H += ea < eb ? eb : ea;
+    }
+    H = fmaf(fmaf(wp, c.cay, fabsf(bxc)), Dx, H);
+    H = fmaf(fmaf(wp, c.cax, c.caby), Dy, H);
+    H = fmaf(fmaf(wp, c.cp, c.wecp), Dp, H);
+    H = fmaf(apv, dv_, H);
+    H = fmaf(c.aev, De, H);
+    const float previous = BLEND ? nb.prev : v;
+    float value = fmaf(dt, H < 0.f ? H : 0.f, v);
+    if (BLEND) value = .5f * (previous + value);
+    nonfinite |= !isfinite(value);
+    st(a.dst, jb, value);
+    if (final_stage) {
+      // Maxima on the bit patterns: inc_max >= 0, so signed integer order
+      // agrees with float order against any inc; |inc| >= 0 likewise.
+      const float inc = value - previous;
+      inc_max = __int_as_float(max(__float_as_int(inc_max), __float_as_int(inc)));
+      chg_max = __int_as_float(max(__float_as_int(chg_max), __float_as_int(fabsf(inc))));
+      if (previous > 0.f && value <= 0.f) {
+        const float old_arrival = ld(a.arrival, jb);
+        if (isnan(old_arrival))
+          st(a.arrival, jb,
+             float(a.t + double(a.dt) * double(previous) / (double(previous) - double(value))));
+      }
+    }
+    w2 = w3;
+    w3 = w4;
+    w4 = w5;
+    j += sv;
+  };
+
+  int iv = 0;
+  for (; iv < 2 && iv < nvp; iv++) node(iv, std::false_type{});
+#pragma unroll 2
+  for (; iv + 2 < nvp; iv++) node(iv, std::true_type{});
+  for (; iv < nvp; iv++) node(iv, std::false_type{});
+}
+
+template <bool BLEND, int ORDER, bool PREFETCH>
 __global__ __launch_bounds__(256) void hji5d_stage_fast(StageArgs<float> a, Hji5dGrid g,
                                                         int final_stage) {
   const int hw = blockIdx.x;
@@ -80,125 +300,53 @@ __global__ __launch_bounds__(256) void hji5d_stage_fast(StageArgs<float> a, Hji5
     if (r < g.nx2) {
       const int iy = r / g.nx;
       const int ix = r - iy * g.nx;
-      const float dx = float(g.dx), dphi = float(g.dphi), dvp = float(g.dvp),
-                  dve = float(g.dve);
-      const float ix_h = 1.f / dx, ip_h = 1.f / dphi, iv_h = 1.f / dvp, ie_h = 1.f / dve;
-      const float ix_hh = ix_h * ix_h, ip_hh = ip_h * ip_h, iv_hh = iv_h * iv_h,
-                  ie_hh = ie_h * ie_h;
-      const float hx = .5f * dx, hp = .5f * dphi, hv = .5f * dvp, hev = .5f * dve;
+      const float dx = float(g.dx), dphi = float(g.dphi), dve = float(g.dve);
       const float x = float(-g.extent) + float(ix) * dx;
       const float y = float(-g.extent) + float(iy) * dx;
-      const float ve = float(g.vemin) + float(ie) * dve;
-      const float ap = float(g.ap), ae = float(g.ae), kp = float(g.kp);
-      const float we = fabsf(ve) * float(g.ke);
-      const float cs = a.cs[ih], sn = a.sn[ih];
-      const float by = ve * sn, aby = fabsf(by), ax = fabsf(x), ay = fabsf(y);
-      const float elow = ie == 0 ? 0.f : -ae, ehigh = ie == g.nve - 1 ? 0.f : ae;
-      const bool outside = g.exit_on && double(x * x + y * y) > g.exit_r2;
-      const Axis X = axis_masks(ix, g.nx), Y = axis_masks(iy, g.nx),
-                 E = axis_masks(ie, g.nve);
-      // psi neighbours (periodic), per-thread wrapped offsets.
-      const int hm1 = ih == 0 ? g.nh - 1 : ih - 1;
-      const int hm2 = ih >= 2 ? ih - 2 : ih - 2 + g.nh;
-      const int hp1 = ih == g.nh - 1 ? 0 : ih + 1;
-      const int hp2 = ih + 2 < g.nh ? ih + 2 : ih + 2 - g.nh;
-      const unsigned ohm1_b = (unsigned)((hm1 - ih) * g.nx2) << 2,
-                     ohm2_b = (unsigned)((hm2 - ih) * g.nx2) << 2,
-                     ohp1_b = (unsigned)((hp1 - ih) * g.nx2) << 2,
-                     ohp2_b = (unsigned)((hp2 - ih) * g.nx2) << 2;
-      // Launch-uniform neighbour bases (guard margins keep them in range).
-      const float *__restrict__ s0 = a.src;
-      const int nx = g.nx, se = (int)g.se;
-      const float *__restrict__ sxm2 = s0 - 2, *__restrict__ sxm1 = s0 - 1,
-                                *__restrict__ sxp1 = s0 + 1, *__restrict__ sxp2 = s0 + 2;
-      const float *__restrict__ sym2 = s0 - 2 * nx, *__restrict__ sym1 = s0 - nx,
-                                *__restrict__ syp1 = s0 + nx, *__restrict__ syp2 = s0 + 2 * nx;
-      const float *__restrict__ sem2 = s0 - 2 * se, *__restrict__ sem1 = s0 - se,
-                                *__restrict__ sep1 = s0 + se, *__restrict__ sep2 = s0 + 2 * se;
-      const unsigned sv = (unsigned)g.sv;
-      unsigned j = (unsigned)(ie * se + ih * g.nx2 + r);  // node at vP index 0
-      // vP window w1..w4 = V(iv-1 .. iv+2) and its sliding differences:
-      // at node iv, Dm = D(iv-1/2), Dp = D(iv+1/2), l2 = d2(iv-1),
-      // c2 = d2(iv), r2 = d2(iv+1); each step computes only Dp and r2.
-      float w1 = 0.f, w2 = ld(s0, j << 2);
-      float w3 = ld(s0, (j + sv) << 2);
-      float w4 = ld(s0, (j + 2 * sv) << 2);
-      float Dp_prev = 0.f, c2_prev = 0.f, r2_prev = 0.f;
-      for (int iv = 0; iv < g.nvp; iv++, j += sv) {
-        const float w5 = iv + 3 < g.nvp ? ld(s0, (j + 3 * sv) << 2) : 0.f;
-        const float Dm = Dp_prev, Dp = (w3 - w2) * iv_h;
-        const float l2 = c2_prev, c2 = r2_prev, r2 = (w4 - 2.f * w3 + w2) * iv_hh;
-        Dp_prev = Dp;
-        c2_prev = c2;
-        r2_prev = r2;
-        const bool first = iv == 0, last = iv == g.nvp - 1;
-        float vm, vpl;
-        if (ORDER == 1) {
-          vm = first ? Dp : Dm;
-          vpl = last ? Dm : Dp;
-        } else {
-          const float sl2 = iv >= 2 ? l2 : c2;
-          const float sr2 = iv + 2 < g.nvp ? r2 : c2;
-          const float em = Dm + hv * (fabsf(sl2) < fabsf(c2) ? sl2 : c2);
-          const float ep = Dp - hv * (fabsf(c2) < fabsf(sr2) ? c2 : sr2);
-          vm = first ? Dp : (last ? Dm : em);
-          vpl = first ? Dp : (last ? Dm : ep);
-        }
-        const unsigned jb = j << 2;
-        if (outside) {
-          a.dst[j] = float(g.exit_value);
-        } else {
-          const float v = w2;
-          float xm, xp, ym, yp, pm, pp, em, ep;
-          if (ORDER == 1) {
-            eno1(v, ld(sxm1, jb), ld(sxp1, jb), X, ix_h, xm, xp);
-            eno1(v, ld(sym1, jb), ld(syp1, jb), Y, ix_h, ym, yp);
-            eno1(v, ld(sem1, jb), ld(sep1, jb), E, ie_h, em, ep);
-            const float hm = ld(s0, jb + ohm1_b), hpv = ld(s0, jb + ohp1_b);
-            pm = (v - hm) * ip_h;
-            pp = (hpv - v) * ip_h;
-          } else {
-            eno2(v, ld(sxm2, jb), ld(sxm1, jb), ld(sxp1, jb), ld(sxp2, jb), X, ix_h, ix_hh, hx, xm, xp);
-            eno2(v, ld(sym2, jb), ld(sym1, jb), ld(syp1, jb), ld(syp2, jb), Y, ix_h, ix_hh, hx, ym, yp);
-            eno2(v, ld(sem2, jb), ld(sem1, jb), ld(sep1, jb), ld(sep2, jb), E, ie_h, ie_hh, hev, em, ep);
-            const float hm2v = ld(s0, jb + ohm2_b), hm1v = ld(s0, jb + ohm1_b), hp1v = ld(s0, jb + ohp1_b),
-                        hp2v = ld(s0, jb + ohp2_b);
-            const Axis P{false, false, true, true};
-            eno2(v, hm2v, hm1v, hp1v, hp2v, P, ip_h, ip_hh, hp, pm, pp);
-          }
-          const float vp = float(g.vpmin) + float(iv) * dvp;
-          const float wp = fabsf(vp) * kp;
-          const float bx = ve * cs - vp;
-          const float gx = .5f * (xm + xp), gy = .5f * (ym + yp), gp = .5f * (pm + pp),
-                      gv = .5f * (vm + vpl), ge = .5f * (em + ep);
-          const float plow = iv == 0 ? 0.f : -ap, phigh = iv == g.nvp - 1 ? 0.f : ap;
-          const float pa = plow * gv, pb = phigh * gv, ea = elow * ge, eb = ehigh * ge;
-          float H = bx * gx + by * gy - wp * fabsf(y * gx - x * gy - gp) + we * fabsf(gp) +
-                    (pb < pa ? pb : pa) + (ea < eb ? eb : ea);
-          H += .5f * ((fabsf(bx) + wp * ay) * (xp - xm) + (aby + wp * ax) * (yp - ym) +
-                      (wp + we) * (pp - pm) + ap * (vpl - vm) + ae * (ep - em));
-          const float old = v;
-          const float previous = BLEND ? a.dst[j] : old;
-          float value = old + a.dt * (H < 0.f ? H : 0.f);
-          if (BLEND) value = .5f * (previous + value);
-          nonfinite |= !isfinite(value);
-          a.dst[j] = value;
-          if (final_stage) {
-            const float inc = value - previous;
-            inc_max = fmaxf(inc_max, inc);
-            chg_max = fmaxf(chg_max, fabsf(inc));
-            if (previous > 0.f && value <= 0.f) {
-              const float old_arrival = a.arrival[j];
-              if (isnan(old_arrival))
-                a.arrival[j] = float(a.t + double(a.dt) * double(previous) /
-                                              (double(previous) - double(value)));
-            }
-          }
-        }
-        w1 = w2;
-        w2 = w3;
-        w3 = w4;
-        w4 = w5;
+      const unsigned j0 = (unsigned)(ie * (int)g.se + ih * g.nx2 + r);
+      if (g.exit_on && double(x * x + y * y) > g.exit_r2) {
+        // Outside the local exit radius every vP node takes the exit value
+        // (gpu_relative.py:126-131).
+        const float ev = float(g.exit_value);
+        for (int iv = 0; iv < g.nvp; iv++) a.dst[j0 + (unsigned)iv * (unsigned)g.sv] = ev;
+      } else {
+        const float ve = float(g.vemin) + float(ie) * dve;
+        const float cs = a.cs[ih], sn = a.sn[ih];
+        const float by = ve * sn;
+        Column c;
+        c.j = j0;
+        c.ve_cs = ve * cs;
+        c.cx = .5f / dx;
+        c.cp = .5f / dphi;
+        const float ce = .5f / dve;
+        c.cxy = c.cx * y;
+        c.cxx = c.cx * x;
+        c.cby = c.cx * by;
+        c.caby = c.cx * fabsf(by);
+        c.cay = c.cx * fabsf(y);
+        c.cax = c.cx * fabsf(x);
+        const float we = fabsf(ve) * float(g.ke), ae = float(g.ae);
+        c.wecp = we * c.cp;
+        c.e_lo = (ie == 0 ? 0.f : -ae) * ce;
+        c.e_hi = (ie == g.nve - 1 ? 0.f : ae) * ce;
+        c.aev = ae * ce;
+        c.X = axis_masks(ix, g.nx);
+        c.Y = axis_masks(iy, g.nx);
+        c.E = axis_masks(ie, g.nve);
+        // psi neighbours (periodic; ih is uniform per block)
+        const int nh = g.nh, nx2 = g.nx2;
+        const int hm1 = ih == 0 ? nh - 1 : ih - 1;
+        const int hm2 = ih >= 2 ? ih - 2 : ih - 2 + nh;
+        const int hp1 = ih == nh - 1 ? 0 : ih + 1;
+        const int hp2 = ih + 2 < nh ? ih + 2 : ih + 2 - nh;
+        const float *hb[4] = {a.src + (hm2 - ih) * nx2, a.src + (hm1 - ih) * nx2,
+                              a.src + (hp1 - ih) * nx2, a.src + (hp2 - ih) * nx2};
+        if (ie >= 2 && ie + 2 < g.nve)
+          hji5d_column<BLEND, ORDER, true, PREFETCH>(a, g, c, hb, final_stage, inc_max,
+                                                     chg_max, nonfinite);
+        else
+          hji5d_column<BLEND, ORDER, false, PREFETCH>(a, g, c, hb, final_stage, inc_max,
+                                                      chg_max, nonfinite);
       }
     }
   }

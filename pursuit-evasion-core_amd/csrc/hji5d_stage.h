@@ -47,6 +47,7 @@ struct Hji5dGrid {
   int xytiles;      // ceil(nx * nx / 256): tiles of one (y, x) plane
   int layout;       // block order: 0 psi-plane tiles; 1 (xy tile, psi, vE)
   int fast;         // FP32 launches use hji5d_stage_fast (hji5d_fast.h)
+  int prefetch;     // fast kernel requests the next vP node's neighbours early
   int blocks;       // tiles * nve * nvpc (layout 0) or xytiles*nh*nve*nvpc
   int per_xcd;      // ceil(blocks / 8)
 };
@@ -62,6 +63,7 @@ struct StageArgs {
   R dt;
   double t;
   unsigned long long *slot;  // [increase, |change|, nonfinite] of this step
+  const float *vpk = nullptr;  // [nvp][2]: vP and |vP| kP (FP32 fast kernel)
 };
 
 // ENO1 / ENO2 one-sided differences along one axis

@@ -21,17 +21,21 @@ void hji5d_launch_f32(const Hji5dGrid &g, const StageArgs<float> &a,
                       bool blend, bool final_stage, hipStream_t stream) {
   const dim3 grid((unsigned)(g.per_xcd * 8)), block(256);
   if (g.fast) {
+#define PE_FAST(B, O, P) \
+  hipLaunchKernelGGL((hji5d_stage_fast<B, O, P>), grid, block, 0, stream, a, g, (int)final_stage)
     if (g.order == 2) {
-      if (blend)
-        hipLaunchKernelGGL((hji5d_stage_fast<true, 2>), grid, block, 0, stream, a, g,
-                           (int)final_stage);
-      else
-        hipLaunchKernelGGL((hji5d_stage_fast<false, 2>), grid, block, 0, stream, a, g,
-                           (int)final_stage);
+      if (blend) {
+        if (g.prefetch) PE_FAST(true, 2, true);
+        else PE_FAST(true, 2, false);
+      } else {
+        if (g.prefetch) PE_FAST(false, 2, true);
+        else PE_FAST(false, 2, false);
+      }
     } else {
-      hipLaunchKernelGGL((hji5d_stage_fast<false, 1>), grid, block, 0, stream, a, g,
-                         (int)final_stage);
+      if (g.prefetch) PE_FAST(false, 1, true);
+      else PE_FAST(false, 1, false);
     }
+#undef PE_FAST
     return;
   }
   if (blend)
@@ -108,6 +112,7 @@ struct pe_hji5d {
   float *a_alloc = nullptr, *b_alloc = nullptr;  // a, b with guard margins
   long long margin = 0;                          // floats before a and b
   void *sn = nullptr, *cs = nullptr;
+  float *vpk = nullptr;  // [nvp][2] vP, |vP| kP in float
   // Reduction ring: slot k holds [increase, |change|, nonfinite, pad].
   unsigned long long *ring = nullptr;
   int ring_slots = 4096, ring_used = 0;
@@ -137,7 +142,7 @@ static int validate(const pe_hji5d_config &c) {
       c.p_min >= c.p_max || c.e_min >= c.e_max ||
       (c.precision != 0 && c.precision != 1) ||
       (c.capture != 0 && c.capture != 1) || c.vp_chunk < 0 ||
-      (c.flags & ~(PE_HJI5D_LAYOUT_PLANE | PE_HJI5D_GENERIC)) != 0)
+      (c.flags & ~(PE_HJI5D_LAYOUT_PLANE | PE_HJI5D_GENERIC | PE_HJI5D_NO_PREFETCH)) != 0)
     return fail(PE_ERR_INVALID, "Invalid solver options");
   if (c.exit_radius > 0 && !(c.exit_radius < c.extent))
     return fail(PE_ERR_INVALID,
@@ -218,7 +223,7 @@ static int one_step(pe_hji5d *h, double dt) {
       hji5d_launch_f64(h->g, a, blend, fin, h->stream);
     } else {
       StageArgs<float> a{src, dst, h->arrival, (const float *)h->sn,
-                         (const float *)h->cs, (float)dt, h->t, slot};
+                         (const float *)h->cs, (float)dt, h->t, slot, h->vpk};
       hji5d_launch_f32(h->g, a, blend, fin, h->stream);
     }
     record(h, kind, false);
@@ -309,6 +314,7 @@ extern "C" int pe_hji5d_create(const pe_hji5d_config *cfg, pe_hji5d **out) {
   // 32-bit byte offsets).
   g.fast = c.precision == 0 && g.layout == 1 && g.nvpc == 1 &&
            !(c.flags & PE_HJI5D_GENERIC) && double(h->n) * 4.0 < 4.0e9;
+  g.prefetch = !(c.flags & PE_HJI5D_NO_PREFETCH);
   {  // native_relative.cpp:76-81
     const double VP = std::max(std::fabs(c.p_min), std::fabs(c.p_max));
     const double VE = std::max(std::fabs(c.e_min), std::fabs(c.e_max));
@@ -379,6 +385,15 @@ extern "C" int pe_hji5d_create(const pe_hji5d_config *cfg, pe_hji5d **out) {
     };
     TRY(upload(h->sn, sn));
     TRY(upload(h->cs, cs));
+    // Pursuer speeds and their turn-rate bounds, rounded once from double.
+    std::vector<float> vpk(2 * (size_t)g.nvp);
+    for (int iv = 0; iv < g.nvp; iv++) {
+      const double vp = g.vpmin + iv * g.dvp;
+      vpk[2 * iv] = float(vp);
+      vpk[2 * iv + 1] = float(std::fabs(vp) * g.kp);
+    }
+    TRY(alloc((void **)&h->vpk, vpk.size() * sizeof(float)));
+    PE_HIP(hipMemcpy(h->vpk, vpk.data(), vpk.size() * sizeof(float), hipMemcpyHostToDevice));
   }
   // Initial capture plane on the host, broadcast on the device
   // (native_relative.cpp:93-96; gpu_relative.py:156-179).
@@ -554,7 +569,7 @@ extern "C" void pe_hji5d_destroy(pe_hji5d *h) {
   if (h->stream) hipStreamSynchronize(h->stream);
   for (hipEvent_t e : h->events) hipEventDestroy(e);
   for (void *p : {(void *)h->a_alloc, (void *)h->b_alloc, (void *)h->arrival, h->sn, h->cs,
-                  (void *)h->ring, (void *)h->counter})
+                  (void *)h->vpk, (void *)h->ring, (void *)h->counter})
     if (p) hipFree(p);
   if (h->t0) hipEventDestroy(h->t0);
   if (h->t1) hipEventDestroy(h->t1);

@@ -45,6 +45,7 @@ struct pe_sl {
   std::vector<pe::SlMap> maps;
   pe::SlMap *d_maps = nullptr;
   int np_max = 0, ne_max = 0;
+  size_t lds_bytes = 0;  // G rows of one P state, [na][ne_max] floats (0: no LDS E side)
   long long pairs = 0, g_size = 0, active = 0;
   // car tables
   int *pids = nullptr, *eids = nullptr, *ppos = nullptr, *epos = nullptr;

@@ -21,6 +21,11 @@
 //              lower it) and, optionally, the previous sweep's argmin /
 //              argmax commands tried first so that exit comes sooner.  It
 //              also max-reduces |V' - V| per map (the residual).
+//   sl_e_side_lds  the same per pair, but one 1024-thread block per P state
+//              first copies that state's G_u rows into LDS ([na][ne] floats,
+//              94 KB at 2,612 E states), then walks the whole E row, so the
+//              16-point E gathers hit LDS instead of one cache line per lane.
+//              Used whenever the rows fit; bit-identical to sl_e_side.
 //
 // G holds [maps][u][ip][e] float32 (9 x the field) and is zero-filled once:
 // entries skipped by sl_p_side are never read with a non-zero weight, and
@@ -99,6 +104,86 @@ __device__ __forceinline__ int nth_command(int i, int first) {
   return i == 0 ? first : (i <= first ? i - 1 : i);
 }
 
+// Min over allowed P commands of the max over allowed E commands of one
+// inside pair (ip, ie), starting from its current value; row(u) is the
+// row G_u(ip, .).  Early exit: a P command whose running max over E
+// commands reaches the running min cannot lower it (grid_game.py:409-421).
+// Updates the stored command order of the pair.
+template <bool EARLY, bool ORDERED, typename Row>
+__device__ __forceinline__ float pair_backup(const SlMap &m, int prow, int erow, float start,
+                                             long long pair, Row row,
+                                             const int *__restrict__ eids,
+                                             const float *__restrict__ ew,
+                                             const uint8_t *__restrict__ eallowed,
+                                             const uint8_t *__restrict__ pallowed,
+                                             uint8_t *__restrict__ order) {
+  float best = start;
+  int ufirst = 0, bfirst = 0;
+  if (ORDERED) {
+    const int o = order[pair];
+    if (o != 0xFF) {
+      ufirst = o & 15;
+      bfirst = o >> 4;
+    }
+  }
+  int best_u = -1, best_b = -1;
+  for (int i = 0; i < m.na; i++) {
+    const int u = ORDERED ? nth_command(i, ufirst) : i;
+    if (!pallowed[(long long)prow * m.na + u]) continue;
+    const float *Gu = row(u);
+    float worst = -1e30f;
+    int arg = -1;
+    bool cut = false;
+    for (int j = 0; j < m.na; j++) {
+      const int b = ORDERED ? nth_command(j, bfirst) : j;
+      const long long cmd = (long long)erow * m.na + b;
+      if (!eallowed[cmd]) continue;
+      const int4 *id4 = reinterpret_cast<const int4 *>(eids + cmd * kStencil);
+      const float4 *w4 = reinterpret_cast<const float4 *>(ew + cmd * kStencil);
+      float q = 0.f;
+#pragma unroll
+      for (int c = 0; c < kStencil / 4; c++) {
+        const int4 id = id4[c];
+        const float4 w = w4[c];
+        q = fmaf(w.x, Gu[id.x], q);
+        q = fmaf(w.y, Gu[id.y], q);
+        q = fmaf(w.z, Gu[id.z], q);
+        q = fmaf(w.w, Gu[id.w], q);
+      }
+      if (q > worst) {
+        worst = q;
+        arg = b;
+      }
+      if (EARLY && worst >= best) {
+        cut = true;  // this P command cannot lower the value
+        break;
+      }
+    }
+    if (!cut && worst < best) {
+      best = worst;
+      best_u = u;
+      best_b = arg;
+    }
+  }
+  if (ORDERED && best_u >= 0) order[pair] = (uint8_t)(best_u | (best_b << 4));
+  return best;
+}
+
+// Block max of `change` into the map's residual slot (blockDim.x <= 1024).
+__device__ __forceinline__ void block_residual(float change, unsigned long long *slot) {
+  __shared__ float red[16];
+  const float w = wave_max(change);
+  const int waves = (int)(blockDim.x + 63) >> 6;
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = w;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float b = red[0];
+    for (int k = 1; k < waves; k++) b = fmaxf(b, red[k]);
+    raise_max(slot, (double)b);
+  }
+}
+
+// E side with G gathered from global memory: one thread per pair.
 template <bool EARLY, bool ORDERED>
 __global__ __launch_bounds__(256) void sl_e_side(
     const SlMap *__restrict__ maps, const float *__restrict__ V,
@@ -120,68 +205,63 @@ __global__ __launch_bounds__(256) void sl_e_side(
       float best = start;
       const int prow = m.p_row + ip;
       if (inside[m.tab_off + (long long)ppos[prow] * m.npos_e + ep]) {
-        int ufirst = 0, bfirst = 0;
-        if (ORDERED) {
-          const int o = order[pair];
-          if (o != 0xFF) {
-            ufirst = o & 15;
-            bfirst = o >> 4;
-          }
-        }
-        int best_u = -1, best_b = -1;
-        for (int i = 0; i < m.na; i++) {
-          const int u = ORDERED ? nth_command(i, ufirst) : i;
-          if (!pallowed[(long long)prow * m.na + u]) continue;
-          const float *__restrict__ Gu =
-              G + m.g_off + ((long long)u * m.np + ip) * m.ne;
-          float worst = -1e30f;
-          int arg = -1;
-          bool cut = false;
-          for (int j = 0; j < m.na; j++) {
-            const int b = ORDERED ? nth_command(j, bfirst) : j;
-            const long long cmd = (long long)erow * m.na + b;
-            if (!eallowed[cmd]) continue;
-            const int4 *id4 = reinterpret_cast<const int4 *>(eids + cmd * kStencil);
-            const float4 *w4 = reinterpret_cast<const float4 *>(ew + cmd * kStencil);
-            float q = 0.f;
-#pragma unroll
-            for (int c = 0; c < kStencil / 4; c++) {
-              const int4 id = id4[c];
-              const float4 w = w4[c];
-              q = fmaf(w.x, Gu[id.x], q);
-              q = fmaf(w.y, Gu[id.y], q);
-              q = fmaf(w.z, Gu[id.z], q);
-              q = fmaf(w.w, Gu[id.w], q);
-            }
-            if (q > worst) {
-              worst = q;
-              arg = b;
-            }
-            if (EARLY && worst >= best) {
-              cut = true;  // this P command cannot lower the value
-              break;
-            }
-          }
-          if (!cut && worst < best) {
-            best = worst;
-            best_u = u;
-            best_b = arg;
-          }
-        }
-        if (ORDERED && best_u >= 0) order[pair] = (uint8_t)(best_u | (best_b << 4));
+        auto row = [&](int u) { return G + m.g_off + ((long long)u * m.np + ip) * m.ne; };
+        best = pair_backup<EARLY, ORDERED>(m, prow, erow, start, pair, row, eids, ew,
+                                           eallowed, pallowed, order);
       }
       Vn[pair] = best;
       change = fmaxf(change, fabsf(best - start));
     }
   }
-  __shared__ float red[4];
-  const float w = wave_max(change);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = w;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const float b = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
-    raise_max(slot + blockIdx.z, (double)b);
+  block_residual(change, slot + blockIdx.z);
+}
+
+// E side with the G rows of one P state staged in LDS: one block per
+// (P state, map) walks the whole E row, so the 16-point E gathers of every
+// pair read LDS instead of scattered cache lines.  Dynamic LDS holds
+// [na][ne] floats; only the rows of allowed P commands are filled, and only
+// those are read.  Same arithmetic as sl_e_side, value for value.
+template <bool EARLY, bool ORDERED>
+__global__ __launch_bounds__(1024) void sl_e_side_lds(
+    const SlMap *__restrict__ maps, const float *__restrict__ V,
+    float *__restrict__ Vn, const float *__restrict__ G,
+    const int *__restrict__ eids, const float *__restrict__ ew,
+    const uint8_t *__restrict__ eallowed, const uint8_t *__restrict__ pallowed,
+    const int *__restrict__ ppos, const int *__restrict__ epos,
+    const uint8_t *__restrict__ inside, uint8_t *__restrict__ order,
+    unsigned long long *__restrict__ slot) {
+  extern __shared__ float tile[];
+  const SlMap m = maps[blockIdx.z];
+  const int ip = blockIdx.x;  // uniform per block
+  const bool active = ip < m.np;
+  const int prow = m.p_row + ip;
+  if (active) {
+    for (int u = 0; u < m.na; u++) {
+      if (!pallowed[(long long)prow * m.na + u]) continue;
+      const float *__restrict__ src = G + m.g_off + ((long long)u * m.np + ip) * m.ne;
+      float *dst = tile + u * m.ne;
+      for (int e = threadIdx.x; e < m.ne; e += blockDim.x) dst[e] = src[e];
+    }
   }
+  __syncthreads();
+  float change = 0.f;
+  if (active) {
+    const int pp = ppos[prow];
+    for (int ie = threadIdx.x; ie < m.ne; ie += blockDim.x) {
+      const int erow = m.e_row + ie;
+      const long long pair = m.v_off + (long long)ip * m.ne + ie;
+      const float start = V[pair];
+      float best = start;
+      if (inside[m.tab_off + (long long)pp * m.npos_e + epos[erow]]) {
+        auto row = [&](int u) { return tile + u * m.ne; };
+        best = pair_backup<EARLY, ORDERED>(m, prow, erow, start, pair, row, eids, ew,
+                                           eallowed, pallowed, order);
+      }
+      Vn[pair] = best;
+      change = fmaxf(change, fabsf(best - start));
+    }
+  }
+  block_residual(change, slot + blockIdx.z);
 }
 
 // Field sampled at continuous pair states through the product of the two
@@ -251,10 +331,18 @@ static void enqueue_sweep(pe_sl *h, const float *src, float *dst, int flags,
   if (record) sl_record(h, 0, false);
   if (record) sl_record(h, 1, true);
   const bool early = flags & PE_SL_EARLY_EXIT, ordered = flags & PE_SL_ORDERED;
-#define PE_E_SIDE(E, O)                                                        \
-  hipLaunchKernelGGL((sl_e_side<E, O>), grid, block, 0, h->stream, h->d_maps,  \
-                     src, dst, h->G, h->eids, h->ew, h->eallowed, h->pallowed, \
-                     h->ppos, h->epos, h->inside, h->order, slot)
+  const bool lds = h->lds_bytes > 0 && !(flags & PE_SL_GLOBAL_GATHER);
+  const dim3 lds_grid((unsigned)h->np_max, 1, (unsigned)h->B), lds_block(1024);
+#define PE_E_SIDE(E, O)                                                           \
+  if (lds)                                                                        \
+    hipLaunchKernelGGL((sl_e_side_lds<E, O>), lds_grid, lds_block, h->lds_bytes,  \
+                       h->stream, h->d_maps, src, dst, h->G, h->eids, h->ew,      \
+                       h->eallowed, h->pallowed, h->ppos, h->epos, h->inside,     \
+                       h->order, slot);                                           \
+  else                                                                            \
+    hipLaunchKernelGGL((sl_e_side<E, O>), grid, block, 0, h->stream, h->d_maps,   \
+                       src, dst, h->G, h->eids, h->ew, h->eallowed, h->pallowed,  \
+                       h->ppos, h->epos, h->inside, h->order, slot)
   if (early && ordered) PE_E_SIDE(true, true);
   else if (early) PE_E_SIDE(true, false);
   else if (ordered) PE_E_SIDE(false, true);
@@ -451,6 +539,25 @@ extern "C" int pe_sl_create(const pe_sl_game *games, int32_t n_maps,
   };
   auto body = [&]() -> int {
     PE_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    // LDS E side when the G rows of one P state fit a workgroup's LDS.
+    {
+      int limit = 0;
+      PE_HIP(hipDeviceGetAttribute(&limit, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
+      const size_t bytes = sizeof(float) * (size_t)na * (size_t)h->ne_max;
+      if (bytes + 256 <= (size_t)limit) {
+        // If the runtime refuses the LDS size for any variant, the global-gather
+        // E side is used instead.
+        bool ok = true;
+        for (const void *f : {(const void *)sl_e_side_lds<true, true>,
+                              (const void *)sl_e_side_lds<true, false>,
+                              (const void *)sl_e_side_lds<false, true>,
+                              (const void *)sl_e_side_lds<false, false>})
+          ok = ok && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)bytes) == hipSuccess;
+        if (ok) h->lds_bytes = bytes;
+        else (void)hipGetLastError();
+      }
+    }
     PE_HIP(hipEventCreate(&h->t0));
     PE_HIP(hipEventCreate(&h->t1));
     PE_HIP(hipMalloc(&h->d_maps, sizeof(SlMap) * n_maps));
@@ -646,6 +753,7 @@ extern "C" int pe_sl_get_stats(pe_sl *h, pe_sl_stats *s) {
   std::memset(s, 0, sizeof(*s));
   s->sweeps = h->sweeps;
   s->maps = h->B;
+  s->lds_e_side = h->lds_bytes > 0;
   s->pairs = h->pairs;
   s->active_pairs = h->active;
   s->update_seconds = h->update_seconds;

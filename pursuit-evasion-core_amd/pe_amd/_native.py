@@ -96,6 +96,7 @@ class HJI5DConfig(ctypes.Structure):
 
 PE_HJI5D_LAYOUT_PLANE = 1
 PE_HJI5D_GENERIC = 2
+PE_HJI5D_NO_PREFETCH = 4
 
 
 class HJI5DStats(ctypes.Structure):
@@ -144,7 +145,7 @@ class SLStats(ctypes.Structure):
     _fields_ = [
         ("sweeps", ctypes.c_int64),
         ("maps", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("lds_e_side", ctypes.c_int32),
         ("pairs", ctypes.c_int64),
         ("active_pairs", ctypes.c_int64),
         ("update_seconds", ctypes.c_double),
@@ -178,6 +179,7 @@ class MLPConfig(ctypes.Structure):
 PE_SL_EARLY_EXIT = 1
 PE_SL_ORDERED = 2
 PE_SL_GRAPH = 4
+PE_SL_GLOBAL_GATHER = 8
 
 P = ctypes.c_void_p
 I32 = ctypes.c_int32
@@ -217,6 +219,9 @@ SIGNATURES = {
     "pe_sl_destroy": (None, [P]),
     "pe_sl_warm_start": (ctypes.c_int, [P, P, ctypes.c_float]),
     "pe_sl_pair_inputs": (ctypes.c_int, [P, I32, I64, I64, P]),
+    "pe_history_create": (ctypes.c_int, [P, I64, I64, P]),
+    "pe_history_query": (ctypes.c_int, [P, I64, I32, P, P, P, ctypes.c_float, P]),
+    "pe_history_destroy": (None, [P]),
     "pe_mlp_create": (ctypes.c_int, [ctypes.POINTER(MLPConfig), P]),
     "pe_mlp_param_count": (ctypes.c_int, [P, P]),
     "pe_mlp_get_params": (ctypes.c_int, [P, P, I64]),
@@ -439,7 +444,7 @@ class SL:
     def stats(self):
         s = SLStats()
         check(lib().pe_sl_get_stats(self._h, ctypes.byref(s)))
-        return {k: getattr(s, k) for k, _ in SLStats._fields_ if k != "reserved"}
+        return {k: getattr(s, k) for k, _ in SLStats._fields_}
 
     def set_profiling(self, on):
         check(lib().pe_sl_set_profiling(self._h, int(bool(on))))
@@ -451,3 +456,47 @@ class SL:
 
     def warm_start(self, mlp, delta):
         check(lib().pe_sl_warm_start(self._h, mlp._h, float(delta)))
+
+
+class History:
+    """Owning wrapper of a pe_history handle: a packed float16 field history
+    [frames, states] resident on the device, queried through product
+    stencils."""
+
+    def __init__(self, values):
+        v = np.ascontiguousarray(values, np.float16)
+        if v.ndim != 2:
+            raise ValueError("history [frames, states] required")
+        self.frames, self.states = v.shape
+        self._h = ctypes.c_void_p()
+        check(lib().pe_history_create(ptr(v.view(np.uint16)), self.frames, self.states,
+                                      ctypes.byref(self._h)))
+
+    def query(self, ids, weights, kept, exit_value):
+        """[frames, n] sums over the terms of each query row."""
+        ids = np.ascontiguousarray(ids, np.int64)
+        n = ids.shape[0]
+        terms = int(np.prod(ids.shape[1:])) if ids.ndim > 1 else 1
+        w = np.ascontiguousarray(weights, np.float32).reshape(n, terms)
+        k = np.ascontiguousarray(kept, np.uint8).reshape(n, terms)
+        out = np.empty((self.frames, n), np.float32)
+        check(lib().pe_history_query(self._h, n, terms, ptr(ids), ptr(w), ptr(k),
+                                     float(exit_value), ptr(out)))
+        return out
+
+    def close(self):
+        if self._h:
+            lib().pe_history_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -33,6 +33,7 @@ import time
 import numpy as np
 
 from .. import _native
+from ..reversible.map_game_field import first_crossing
 from ..reversible.model import ReversibleConfig, wrap_angle
 from .geometry import CircleArena
 from .grid_game import GameSolver, PairDomain, VehicleGrid, game_tables, make_grid
@@ -336,7 +337,12 @@ def benchmark_game(seed, *, positions=11, headings=12, speeds=5, half_width=3.0,
 
 
 class ContactReference:
-    """Reader of a completed solve with retained history (reference.py:372-454)."""
+    """Reader of a completed solve with retained history (reference.py:372-454).
+
+    The float16 history stays on the GPU; `fields` sums the 256-term product
+    stencil of every query at every saved budget in one launch
+    (pe_history_query).  Stencils, the payoff and the final masking are
+    computed on the host as in the reference."""
 
     def __init__(self, folder):
         self.folder = Path(folder)
@@ -350,9 +356,66 @@ class ContactReference:
         self.p = VehicleGrid.load(self.folder / "p_grid.npz")
         self.e = VehicleGrid.load(self.folder / "e_grid.npz")
         self.domain = PairDomain.load(self.folder / "domain.npz")
+        dt = self.config.control_dt
+        self.p.bind_geometry(self.arena, self.config.pursuer, "pursuer", dt)
+        self.e.bind_geometry(self.arena, self.config.evader, "evader", dt)
         self.times = np.array(self.meta["times"])
         self.horizon = self.times[-1]
         if (self.folder / "fields.f16").stat().st_size != len(self.times) * self.domain.states * 2:
             raise ValueError("Incomplete history length")
         self.values = np.memmap(self.folder / "fields.f16", np.float16, mode="r",
                                 shape=(len(self.times), self.domain.states))
+        self._history = None
+
+    def history(self):
+        """The device-resident history (uploaded on first use)."""
+        if self._history is None:
+            self._history = _native.History(np.asarray(self.values))
+        return self._history
+
+    def stencil(self, s):
+        """Packed pair ids [n,16,16], product weights, stored-pair mask,
+        validity, inside-radius mask and the query batch shape."""
+        s = np.asarray(s, float)
+        shape = s.shape[:-1]
+        flat = s.reshape(-1, 8)
+        pi, pw, pv, _, _ = self.p.stencil(flat[:, :4])
+        ei, ew, ev, _, _ = self.e.stencil(flat[:, 4:])
+        rank = self.domain.lookup[self.domain.p_position[pi][:, :, None], ei[:, None, :]]
+        ids = self.domain.offsets[pi][:, :, None] + np.maximum(rank, 0)
+        weights = pw[:, :, None] * ew[:, None, :]
+        valid = pv & ev
+        inside = np.linalg.norm(flat[:, :2] - flat[:, 4:6], axis=1) <= self.domain.exit_radius
+        return ids, weights, rank >= 0, valid, inside, shape
+
+    def fields(self, s):
+        """Values at every saved budget [budgets, n], the valid-and-inside
+        mask and the batch shape."""
+        ids, w, kept, valid, inside, shape = self.stencil(s)
+        n = len(ids)
+        values = self.history().query(ids.reshape(n, -1), w.reshape(n, -1),
+                                      kept.reshape(n, -1), self.domain.exit_radius)
+        phi = self.arena.capture_value(np.asarray(s, float).reshape(-1, 8))
+        values = np.minimum(values, phi[None])
+        values = np.where(inside[None], values, self.domain.exit_radius)
+        values[:, ~valid] = np.nan
+        values[0, valid & inside] = phi[valid & inside]
+        return values, valid & inside, shape
+
+    def field(self, s, budget=None):
+        values, valid, shape = self.fields(s)
+        budget = self.horizon if budget is None else budget
+        if not 0 <= budget <= self.horizon:
+            raise ValueError("Budget outside reference")
+        i = min(np.searchsorted(self.times, budget, side="right") - 1, len(self.times) - 2)
+        f = (budget - self.times[i]) / (self.times[i + 1] - self.times[i])
+        return ((1 - f) * values[i] + f * values[i + 1]).reshape(shape)
+
+    def time_value(self, s):
+        values, valid, shape = self.fields(s)
+        return first_crossing(self.times, values, valid).reshape(shape)
+
+    def close(self):
+        if self._history is not None:
+            self._history.close()
+            self._history = None

@@ -1,0 +1,100 @@
+"""End-to-end contact solve through the reference API on the GPU, and the
+reader of its saved history.
+
+* `solve(config, arena, out, ...)` writes the reference's folder
+  (meta.json, p_grid.npz, e_grid.npz, domain.npz, queries.npz, labels.npz,
+  fields.f16; the checkpoint removed on completion) and a second call on a
+  complete folder returns its meta without work (reference.py:176-184).
+* ContactReference (reference.py:372-454) answers from the device-resident
+  float16 history.  Pinned by what the stencil fixes: at a grid-node pair
+  state the stencil is a single vertex of weight one, so every budget's value
+  is the stored float16 value itself (clipped by the payoff); frame 0 is the
+  payoff.  At the sampled continuous queries the reader agrees with the
+  solver's own FP32 labels (written by pe_sl_query from the unrounded field)
+  to float16 rounding: |dV| <= 2e-3 for |V| <= 4.
+* time_value is the first budget where the field reaches zero.
+"""
+
+import json
+
+import numpy as np
+import pytest
+
+from pe_amd.local_game.geometry import Footprint, random_rectangle_arena
+from pe_amd.local_game.reference import ContactReference, contact_config, solve
+from pe_amd.reversible.map_game_field import first_crossing
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def solved(tmp_path_factory):
+    out = tmp_path_factory.mktemp("contact") / "run"
+    config = contact_config()
+    body = Footprint(0.5, 0.3)
+    arena = random_rectangle_arena(5, pursuer=body, evader=body, count=8)
+    print("[gpu] contact solve (7 positions, 8 headings, 3 speeds)", flush=True)
+    meta = solve(config, arena, out, headings=8, positions=7, half_width=3.0, p_speeds=3,
+                 e_speeds=3, horizon=0.5, output_dt=0.25, retain_history=True, frames=4,
+                 per_frame=16)
+    return out, meta
+
+
+def test_solve_writes_reference_folder_and_resumes(gpu, solved):
+    out, meta = solved
+    assert meta["status"] == "complete"
+    for name in ("meta.json", "p_grid.npz", "e_grid.npz", "domain.npz", "queries.npz",
+                 "labels.npz", "fields.f16"):
+        assert (out / name).exists(), name
+    assert not list(out.glob("checkpoint_*"))
+    assert meta["times"] == pytest.approx([0.0, 0.25, 0.5])
+    stored = json.loads((out / "meta.json").read_text())
+    assert stored["schema"] == "local_contact_compact8d_v1"
+    size = (out / "fields.f16").stat().st_size
+    assert size == 3 * stored["pair_states"] * 2
+    again = solve(contact_config(), random_rectangle_arena(5, count=8), out, headings=8,
+                  positions=7, half_width=3.0, p_speeds=3, e_speeds=3, horizon=0.5,
+                  output_dt=0.25, retain_history=True, frames=4, per_frame=16)
+    assert again["status"] == "complete" and again["signature"] == stored["signature"]
+
+
+def test_reader_at_grid_nodes_returns_stored_values(gpu, solved):
+    out, _ = solved
+    ref = ContactReference(out)
+    d = ref.domain
+    rng = np.random.default_rng(0)
+    ip = rng.integers(len(ref.p.states), size=200)
+    rows = []
+    for i in ip:
+        pos = d.p_position[i]
+        ie = d.e_indices[pos, rng.integers(d.counts[pos])]
+        rows.append((i, ie))
+    ip, ie = np.array(rows).T
+    s = np.concatenate([ref.p.states[ip], ref.e.states[ie]], axis=1)
+    values, valid, _ = ref.fields(s)
+    assert valid.all()
+    packed = d.offsets[ip] + d.lookup[d.p_position[ip], ie]
+    stored = np.asarray(ref.values[:, packed], np.float32)
+    phi = ref.arena.capture_value(s)
+    np.testing.assert_allclose(values[1:], np.minimum(stored[1:], phi[None]), atol=1e-6)
+    np.testing.assert_allclose(values[0], phi, atol=1e-6)
+    np.testing.assert_allclose(ref.field(s, 0.25), values[1], atol=1e-6)
+    np.testing.assert_allclose(ref.field(s, 0.375), 0.5 * (values[1] + values[2]), atol=1e-6)
+    np.testing.assert_array_equal(np.isnan(ref.time_value(s)),
+                                  np.isnan(first_crossing(ref.times, values, valid)))
+    ref.close()
+
+
+def test_reader_matches_solver_labels(gpu, solved):
+    out, _ = solved
+    ref = ContactReference(out)
+    q = np.load(out / "queries.npz")["states"]
+    labels = np.load(out / "labels.npz")
+    values, valid, _ = ref.fields(q)
+    phi = ref.arena.capture_value(q)
+    keep = valid
+    want = np.minimum(labels["values"][1:], phi[None])
+    d = np.abs(values[1:, keep] - want[:, keep])
+    print(f"reader vs FP32 labels at {keep.sum()} queries: max|dV|={d.max():.2e}")
+    assert d.max() <= 2e-3
+    ref.close()

@@ -9,8 +9,9 @@
   a small real map over several sweeps, and C2 at full size (one sweep
   compared on every pair, 40 sweeps of properties).  Tolerance stated in
   each test; capture-set membership must agree except within tolerance of 0.
-* The early exit, the control ordering and the hipGraph replay change
-  nothing: results are bit-identical to the plain sweep.
+* The early exit, the control ordering, the hipGraph replay and the choice
+  of E-side kernel (G rows staged in LDS, or gathered from global memory)
+  change nothing: results are bit-identical to the plain sweep.
 """
 
 from types import SimpleNamespace
@@ -32,6 +33,7 @@ from pe_amd.reversible.model import VehicleLimits
 
 pytestmark = pytest.mark.gpu
 EARLY, ORDERED, GRAPH = _native.PE_SL_EARLY_EXIT, _native.PE_SL_ORDERED, _native.PE_SL_GRAPH
+GLOBAL = _native.PE_SL_GLOBAL_GATHER
 CONTROLS = np.array([[a, k] for a in (-3, 0, 3) for k in (-1, 0, 1)], float)
 
 
@@ -115,7 +117,8 @@ def test_sweeps_match_fp64_oracle_on_random_tables(gpu, oracle_lib, seed, np_, n
 def run_variants(tables, sweeps):
     out = {}
     for name, flags in [("plain", 0), ("early", EARLY), ("ordered", EARLY | ORDERED),
-                        ("graph", EARLY | ORDERED | GRAPH)]:
+                        ("graph", EARLY | ORDERED | GRAPH), ("global", EARLY | ORDERED | GLOBAL),
+                        ("global_plain", GLOBAL)]:
         with GameSolver(tables) as s:
             s.sweep(sweeps, flags)
             out[name] = (s.read(), s.trace()[:, 0])
@@ -129,7 +132,7 @@ def test_early_exit_ordering_and_graph_are_exact(gpu):
     print("[gpu] flag variants", flush=True)
     res = run_variants(tables, 6)
     base, trace = res["plain"]
-    for name in ("early", "ordered", "graph"):
+    for name in ("early", "ordered", "graph", "global", "global_plain"):
         np.testing.assert_array_equal(res[name][0], base, err_msg=name)
         np.testing.assert_array_equal(res[name][1], trace, err_msg=name)
 
@@ -161,6 +164,36 @@ def test_small_map_sweeps_match_oracle(gpu, oracle_lib, small_game):
     assert d.max() <= 1e-5
     flip = np.flatnonzero((got.ravel() <= 0) != (V.ravel() <= 0))
     assert (np.abs(V.ravel()[flip]) <= 1e-5).all()
+
+
+def test_small_map_converged_matches_fp64_oracle(gpu, oracle_lib, small_game):
+    """Converged solve: the FP32 GPU iteration run to residual max|dV| <= 1e-4
+    against the FP64 oracle after the same number of sweeps.  Bar:
+    max|dV| <= 1e-3, capture-set (V <= 0) membership equal except where the
+    FP64 value is within 1e-3 of zero (those pairs are printed)."""
+    import time
+
+    config, arena, p, e, tables = small_game
+    inside = dense_inside(p, e, 3.0)
+    with GameSolver(tables) as s:
+        done, ok = s.run(1e-4, 20000, 1, EARLY | ORDERED)
+        got = s.read()
+        gpu_res = s.trace()[-1, 0]
+    print(f"[gpu] small map converged in {done} sweeps (residual {gpu_res:.2e})", flush=True)
+    assert ok
+    V = oracle_lib.sl_init(p.states, e.states, inside, 3.0, arena.pursuer, arena.evader)
+    t0 = time.perf_counter()
+    for k in range(done):
+        V, change = oracle_lib.sl_sweep(V, p, e, inside, threads=16)
+        if k % 500 == 499:
+            print(f"oracle sweep {k + 1}/{done} ({time.perf_counter() - t0:.0f} s)", flush=True)
+    d = np.abs(got - V)
+    flip = np.flatnonzero((got.ravel() <= 0) != (V.ravel() <= 0))
+    print(f"converged small map: max|dV|={d.max():.3e} mean={d.mean():.3e}, FP64 residual "
+          f"{change:.2e}, capture-set flips {len(flip)} at |V64| "
+          f"{np.abs(V.ravel()[flip]).tolist()[:10]}")
+    assert d.max() <= 1e-3
+    assert (np.abs(V.ravel()[flip]) <= 1e-3).all()
 
 
 def test_query_and_packed_layout(gpu, small_game):
