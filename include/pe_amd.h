@@ -78,6 +78,9 @@ typedef struct pe_hji5d_config {
 /* pe_hji5d_config.flags: the FP32 kernel loads each node's neighbours just
    before computing it instead of one vP node ahead (comparison runs). */
 #define PE_HJI5D_NO_PREFETCH 4
+/* pe_hji5d_config.flags: FP32 kernel block order (psi, xy tile, vE), so that
+   y-adjacent tiles run close together (comparison runs). */
+#define PE_HJI5D_PSI_OUTER 8
 
 typedef struct pe_hji5d_stats {
   int64_t states;           /* grid nodes                                  */

@@ -293,10 +293,13 @@ __global__ __launch_bounds__(256) void hji5d_stage_fast(StageArgs<float> a, Hji5
   float inc_max = 0.f, chg_max = 0.f;
   bool nonfinite = false;
   if (block < g.blocks) {
+    // Block order, fastest first: vE, then psi and the xy tile (psi before
+    // the tile by default; the tile before psi with psi_outer).
     const int ie = block % g.nve;
     const int rest = block / g.nve;
-    const int ih = rest % g.nh;
-    const int r = (rest / g.nh) * 256 + (int)threadIdx.x;  // (y, x) index
+    const int ih = g.psi_outer ? rest / g.xytiles : rest % g.nh;
+    const int tile = g.psi_outer ? rest % g.xytiles : rest / g.nh;
+    const int r = tile * 256 + (int)threadIdx.x;  // (y, x) index
     if (r < g.nx2) {
       const int iy = r / g.nx;
       const int ix = r - iy * g.nx;

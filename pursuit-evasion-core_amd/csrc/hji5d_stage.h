@@ -48,6 +48,7 @@ struct Hji5dGrid {
   int layout;       // block order: 0 psi-plane tiles; 1 (xy tile, psi, vE)
   int fast;         // FP32 launches use hji5d_stage_fast (hji5d_fast.h)
   int prefetch;     // fast kernel requests the next vP node's neighbours early
+  int psi_outer;    // fast kernel block order (psi, xy tile, vE) instead of (xy tile, psi, vE)
   int blocks;       // tiles * nve * nvpc (layout 0) or xytiles*nh*nve*nvpc
   int per_xcd;      // ceil(blocks / 8)
 };

@@ -142,7 +142,8 @@ static int validate(const pe_hji5d_config &c) {
       c.p_min >= c.p_max || c.e_min >= c.e_max ||
       (c.precision != 0 && c.precision != 1) ||
       (c.capture != 0 && c.capture != 1) || c.vp_chunk < 0 ||
-      (c.flags & ~(PE_HJI5D_LAYOUT_PLANE | PE_HJI5D_GENERIC | PE_HJI5D_NO_PREFETCH)) != 0)
+      (c.flags & ~(PE_HJI5D_LAYOUT_PLANE | PE_HJI5D_GENERIC | PE_HJI5D_NO_PREFETCH |
+                   PE_HJI5D_PSI_OUTER)) != 0)
     return fail(PE_ERR_INVALID, "Invalid solver options");
   if (c.exit_radius > 0 && !(c.exit_radius < c.extent))
     return fail(PE_ERR_INVALID,
@@ -315,6 +316,7 @@ extern "C" int pe_hji5d_create(const pe_hji5d_config *cfg, pe_hji5d **out) {
   g.fast = c.precision == 0 && g.layout == 1 && g.nvpc == 1 &&
            !(c.flags & PE_HJI5D_GENERIC) && double(h->n) * 4.0 < 4.0e9;
   g.prefetch = !(c.flags & PE_HJI5D_NO_PREFETCH);
+  g.psi_outer = (c.flags & PE_HJI5D_PSI_OUTER) != 0;
   {  // native_relative.cpp:76-81
     const double VP = std::max(std::fabs(c.p_min), std::fabs(c.p_max));
     const double VE = std::max(std::fabs(c.e_min), std::fabs(c.e_max));
