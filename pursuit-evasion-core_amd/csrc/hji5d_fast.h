@@ -7,8 +7,16 @@
 //   vP-walking threads + guard margins + selects              178 / 151 us
 //     (about 200 VALU per node; wave-uniform branches around the boundary
 //     cases instead of selects were slower, 196 / 176 us)
-//   this version: raw differences, folded scale factors,
-//     mask-free interior vE blocks and vP iterations
+//   raw differences, folded scale factors, mask-free interior
+//     vE blocks and vP iterations (~130 VALU per node)        176 / 146 us
+//   + next vP node's neighbours requested one node early      169 / 141 us
+// So the stage is not VALU bound.  Nor is it HBM bound: the (psi, tile, vE)
+// block order fetches 2.2x the bytes from memory in the same time.  What
+// scales is the number of vector memory instructions per node (about 19):
+// variants that omit neighbour loads run faster in proportion.  Taking the x
+// neighbours from adjacent lanes with DPP wave shifts instead of loads was
+// bit-identical but slower (173 / 148 us): the shifts, their wait states
+// and the halo lanes cost more than the four loads.
 //
 // Same scheme as hji5d_stage<float, false, *> node by node
 // (reversible/native_relative.cpp:38-67 slopes, :124-172 update), with the

@@ -253,3 +253,36 @@ def test_profiling_counts_launches(gpu):
         st = h.stats()
     assert st["stage_launches"] == [5, 5]
     assert all(ms > 0 for ms in st["stage_ms"])
+
+
+@pytest.mark.parametrize("grid,steps", [(RelativeGrid(17, 12, 7, 7, 4.0), 18),
+                                        (RelativeGrid(33, 24, 9, 9, 4.0), 12),
+                                        (RelativeGrid(65, 36, 15, 15, 8.0), 3)])
+def test_fp32_kernel_variants_are_bit_identical(gpu, grid, steps):
+    """The FP32 kernel's launch variants do the same arithmetic on the same
+    neighbour values: the next vP node's neighbours requested early or not,
+    and either block order.  Fields and arrival times must agree bit for bit (the generic
+    template, a different operation order, is compared by tolerance in the
+    oracle tests)."""
+    variants = {
+        "default": 0,
+        "no_prefetch": _native.PE_HJI5D_NO_PREFETCH,
+        "psi_outer": _native.PE_HJI5D_PSI_OUTER,
+        "no_prefetch_psi_outer": _native.PE_HJI5D_NO_PREFETCH | _native.PE_HJI5D_PSI_OUTER,
+    }
+    out = {}
+    for name, flags in variants.items():
+        cfg = native_config(a3(0.5), grid, order=2, cfl=0.8)
+        cfg.flags = flags
+        print(f"[gpu] variant {name} on {grid.positions}x{grid.headings}x{grid.p_speeds}x"
+              f"{grid.e_speeds}", flush=True)
+        with _native.HJI5D(cfg) as h:
+            dt = h.stats()["cfl_dt_limit"]
+            h.step(dt, steps)
+            out[name] = h.read()
+    f0, a0 = out["default"]
+    for name, (f, arr) in out.items():
+        np.testing.assert_array_equal(f.view(np.uint32), f0.view(np.uint32), err_msg=name)
+        np.testing.assert_array_equal(np.isnan(arr), np.isnan(a0), err_msg=name)
+        np.testing.assert_array_equal(np.nan_to_num(arr).view(np.uint32),
+                                      np.nan_to_num(a0).view(np.uint32), err_msg=name)
