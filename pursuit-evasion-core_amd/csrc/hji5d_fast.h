@@ -16,7 +16,11 @@
 // variants that omit neighbour loads run faster in proportion.  Taking the x
 // neighbours from adjacent lanes with DPP wave shifts instead of loads was
 // bit-identical but slower (173 / 148 us): the shifts, their wait states
-// and the halo lanes cost more than the four loads.
+// and the halo lanes cost more than the four loads.  Staging each vP node's
+// xy tile with its halo rows in LDS (eight x/y neighbours from LDS, about two
+// global loads per node for the tile) was bit-identical and much slower
+// (228 / 202 us): the block barrier per vP node stops the waves from running
+// ahead of one another, which is what hides the load latency here.
 //
 // Same scheme as hji5d_stage<float, false, *> node by node
 // (reversible/native_relative.cpp:38-67 slopes, :124-172 update), with the
