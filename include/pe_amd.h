@@ -209,16 +209,27 @@ typedef struct pe_sl_stats {
 typedef struct pe_sl pe_sl;
 
 int pe_sl_create(const pe_sl_game *maps, int32_t n_maps, pe_sl **out);
+/* As pe_sl_create with the value precision: 0 = FP32 (the product path),
+   1 = FP64, the oracle's arithmetic on the GPU (each stencil sum a separate
+   multiply and add over the positive weights, left to right), for
+   full-size precision checks.  FP64 handles sweep, run, read and
+   initialise; queries and warm starts need FP32. */
+int pe_sl_create_ex(const pe_sl_game *maps, int32_t n_maps, int32_t precision,
+                    pe_sl **out);
 /* v0 == NULL: the rectangle-contact payoff inside, exit value outside
    (init_packed, grid_game.py:390-398); else the concatenated [p.n*e.n]
    fields of all maps (a warm start). */
 int pe_sl_initialize(pe_sl *h, const float *v0);
+/* The same with a double field (converted to the handle's precision). */
+int pe_sl_initialize_f64(pe_sl *h, const double *v0);
 int pe_sl_sweep(pe_sl *h, int32_t sweeps, int32_t flags);
 /* Sweeps until every map's residual max|dV| <= tol (checked every
    check_every sweeps) or max_sweeps. */
 int pe_sl_run(pe_sl *h, double tol, int32_t max_sweeps, int32_t check_every,
               int32_t flags, int32_t *sweeps_done, int32_t *converged);
 int pe_sl_read(pe_sl *h, int32_t map, float *field);
+/* The same into doubles (exact for FP64 handles). */
+int pe_sl_read_f64(pe_sl *h, int32_t map, double *field);
 /* Field of `map` at n continuous pair states given as the P and E stencils
    (ids [n,16] into the cars' legal states, weights [n,16]):
    out[i] = sum_a pw[i,a] sum_b ew[i,b] V[pi[i,a], ei[i,b]]

@@ -45,6 +45,8 @@ struct pe_sl {
   std::vector<pe::SlMap> maps;
   pe::SlMap *d_maps = nullptr;
   int np_max = 0, ne_max = 0;
+  bool f64 = false;      // FP64 values (va, vb, G hold doubles)
+  size_t esize = 4;      // bytes per value
   size_t lds_bytes = 0;  // G rows of one P state, [na][ne_max] floats (0: no LDS E side)
   long long pairs = 0, g_size = 0, active = 0;
   // car tables

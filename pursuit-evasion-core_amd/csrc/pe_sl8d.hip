@@ -32,237 +32,15 @@
 // zero weights multiply finite values only, so no per-term branch is needed.
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
-#include "pe_internal.h"
+#include "sl8d_kernels.h"
 
 namespace pe {
 
 
 
-
-__global__ __launch_bounds__(256) void sl_initialize(
-    const SlMap *__restrict__ maps, const double *__restrict__ pstates,
-    const double *__restrict__ estates, const int *__restrict__ ppos,
-    const int *__restrict__ epos, const uint8_t *__restrict__ inside,
-    float *__restrict__ V, uint8_t *__restrict__ order) {
-  const SlMap m = maps[blockIdx.z];
-  const int ie = blockIdx.x * 256 + threadIdx.x;
-  if (ie >= m.ne) return;
-  for (int ip = blockIdx.y; ip < m.np; ip += gridDim.y) {
-    const long long pair = m.v_off + (long long)ip * m.ne + ie;
-    order[pair] = 0xFF;
-    const int pp = ppos[m.p_row + ip], ep = epos[m.e_row + ie];
-    if (!inside[m.tab_off + (long long)pp * m.npos_e + ep]) {
-      V[pair] = m.exit_value;
-      continue;
-    }
-    // Rectangle-contact payoff (local_game/geometry.py:15-36), in double.
-    const double *a = pstates + 4ll * (m.p_row + ip);
-    const double *b = estates + 4ll * (m.e_row + ie);
-    const double dx = b[0] - a[0], dy = b[1] - a[1], rel = b[2] - a[2];
-    const double ac = fabs(cos(rel)), as = fabs(sin(rel));
-    const double cp = cos(a[2]), sp = sin(a[2]), ce = cos(b[2]), se = sin(b[2]);
-    const double g0 = fabs(dx * cp + dy * sp) - .5 * (m.pl + m.el * ac + m.ew * as);
-    const double g1 = fabs(-dx * sp + dy * cp) - .5 * (m.pw + m.el * as + m.ew * ac);
-    const double g2 = fabs(dx * ce + dy * se) - .5 * (m.el + m.pl * ac + m.pw * as);
-    const double g3 = fabs(-dx * se + dy * ce) - .5 * (m.ew + m.pl * as + m.pw * ac);
-    V[pair] = float(fmax(fmax(g0, g1), fmax(g2, g3)));
-  }
-}
-
-__global__ __launch_bounds__(256) void sl_p_side(
-    const SlMap *__restrict__ maps, const float *__restrict__ V,
-    float *__restrict__ G, const int *__restrict__ pids,
-    const float *__restrict__ pw, const uint8_t *__restrict__ pallowed,
-    const int *__restrict__ ppos, const int *__restrict__ epos,
-    const uint8_t *__restrict__ near) {
-  const SlMap m = maps[blockIdx.z];
-  const int e = blockIdx.x * 256 + threadIdx.x;
-  if (e >= m.ne) return;
-  const float *__restrict__ Vm = V + m.v_off;
-  const int ep = epos[m.e_row + e];
-  for (int ip = blockIdx.y; ip < m.np; ip += gridDim.y) {
-    const int row = m.p_row + ip;
-    if (!near[m.tab_off + (long long)ppos[row] * m.npos_e + ep]) continue;
-    for (int u = 0; u < m.na; u++) {
-      const long long cmd = (long long)row * m.na + u;
-      if (!pallowed[cmd]) continue;
-      const int *id = pids + cmd * kStencil;
-      const float *w = pw + cmd * kStencil;
-      float acc = 0.f;
-#pragma unroll
-      for (int k = 0; k < kStencil; k++)
-        acc = fmaf(w[k], Vm[(long long)id[k] * m.ne + e], acc);
-      G[m.g_off + ((long long)u * m.np + ip) * m.ne + e] = acc;
-    }
-  }
-}
-
-// i-th command of the order that puts `first` in front (others ascending).
-__device__ __forceinline__ int nth_command(int i, int first) {
-  return i == 0 ? first : (i <= first ? i - 1 : i);
-}
-
-// Min over allowed P commands of the max over allowed E commands of one
-// inside pair (ip, ie), starting from its current value; row(u) is the
-// row G_u(ip, .).  Early exit: a P command whose running max over E
-// commands reaches the running min cannot lower it (grid_game.py:409-421).
-// Updates the stored command order of the pair.
-template <bool EARLY, bool ORDERED, typename Row>
-__device__ __forceinline__ float pair_backup(const SlMap &m, int prow, int erow, float start,
-                                             long long pair, Row row,
-                                             const int *__restrict__ eids,
-                                             const float *__restrict__ ew,
-                                             const uint8_t *__restrict__ eallowed,
-                                             const uint8_t *__restrict__ pallowed,
-                                             uint8_t *__restrict__ order) {
-  float best = start;
-  int ufirst = 0, bfirst = 0;
-  if (ORDERED) {
-    const int o = order[pair];
-    if (o != 0xFF) {
-      ufirst = o & 15;
-      bfirst = o >> 4;
-    }
-  }
-  int best_u = -1, best_b = -1;
-  for (int i = 0; i < m.na; i++) {
-    const int u = ORDERED ? nth_command(i, ufirst) : i;
-    if (!pallowed[(long long)prow * m.na + u]) continue;
-    const float *Gu = row(u);
-    float worst = -1e30f;
-    int arg = -1;
-    bool cut = false;
-    for (int j = 0; j < m.na; j++) {
-      const int b = ORDERED ? nth_command(j, bfirst) : j;
-      const long long cmd = (long long)erow * m.na + b;
-      if (!eallowed[cmd]) continue;
-      const int4 *id4 = reinterpret_cast<const int4 *>(eids + cmd * kStencil);
-      const float4 *w4 = reinterpret_cast<const float4 *>(ew + cmd * kStencil);
-      float q = 0.f;
-#pragma unroll
-      for (int c = 0; c < kStencil / 4; c++) {
-        const int4 id = id4[c];
-        const float4 w = w4[c];
-        q = fmaf(w.x, Gu[id.x], q);
-        q = fmaf(w.y, Gu[id.y], q);
-        q = fmaf(w.z, Gu[id.z], q);
-        q = fmaf(w.w, Gu[id.w], q);
-      }
-      if (q > worst) {
-        worst = q;
-        arg = b;
-      }
-      if (EARLY && worst >= best) {
-        cut = true;  // this P command cannot lower the value
-        break;
-      }
-    }
-    if (!cut && worst < best) {
-      best = worst;
-      best_u = u;
-      best_b = arg;
-    }
-  }
-  if (ORDERED && best_u >= 0) order[pair] = (uint8_t)(best_u | (best_b << 4));
-  return best;
-}
-
-// Block max of `change` into the map's residual slot (blockDim.x <= 1024).
-__device__ __forceinline__ void block_residual(float change, unsigned long long *slot) {
-  __shared__ float red[16];
-  const float w = wave_max(change);
-  const int waves = (int)(blockDim.x + 63) >> 6;
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = w;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    float b = red[0];
-    for (int k = 1; k < waves; k++) b = fmaxf(b, red[k]);
-    raise_max(slot, (double)b);
-  }
-}
-
-// E side with G gathered from global memory: one thread per pair.
-template <bool EARLY, bool ORDERED>
-__global__ __launch_bounds__(256) void sl_e_side(
-    const SlMap *__restrict__ maps, const float *__restrict__ V,
-    float *__restrict__ Vn, const float *__restrict__ G,
-    const int *__restrict__ eids, const float *__restrict__ ew,
-    const uint8_t *__restrict__ eallowed, const uint8_t *__restrict__ pallowed,
-    const int *__restrict__ ppos, const int *__restrict__ epos,
-    const uint8_t *__restrict__ inside, uint8_t *__restrict__ order,
-    unsigned long long *__restrict__ slot) {
-  const SlMap m = maps[blockIdx.z];
-  const int ie = blockIdx.x * 256 + threadIdx.x;
-  float change = 0.f;
-  if (ie < m.ne) {
-    const int erow = m.e_row + ie;
-    const int ep = epos[erow];
-    for (int ip = blockIdx.y; ip < m.np; ip += gridDim.y) {
-      const long long pair = m.v_off + (long long)ip * m.ne + ie;
-      const float start = V[pair];
-      float best = start;
-      const int prow = m.p_row + ip;
-      if (inside[m.tab_off + (long long)ppos[prow] * m.npos_e + ep]) {
-        auto row = [&](int u) { return G + m.g_off + ((long long)u * m.np + ip) * m.ne; };
-        best = pair_backup<EARLY, ORDERED>(m, prow, erow, start, pair, row, eids, ew,
-                                           eallowed, pallowed, order);
-      }
-      Vn[pair] = best;
-      change = fmaxf(change, fabsf(best - start));
-    }
-  }
-  block_residual(change, slot + blockIdx.z);
-}
-
-// E side with the G rows of one P state staged in LDS: one block per
-// (P state, map) walks the whole E row, so the 16-point E gathers of every
-// pair read LDS instead of scattered cache lines.  Dynamic LDS holds
-// [na][ne] floats; only the rows of allowed P commands are filled, and only
-// those are read.  Same arithmetic as sl_e_side, value for value.
-template <bool EARLY, bool ORDERED>
-__global__ __launch_bounds__(1024) void sl_e_side_lds(
-    const SlMap *__restrict__ maps, const float *__restrict__ V,
-    float *__restrict__ Vn, const float *__restrict__ G,
-    const int *__restrict__ eids, const float *__restrict__ ew,
-    const uint8_t *__restrict__ eallowed, const uint8_t *__restrict__ pallowed,
-    const int *__restrict__ ppos, const int *__restrict__ epos,
-    const uint8_t *__restrict__ inside, uint8_t *__restrict__ order,
-    unsigned long long *__restrict__ slot) {
-  extern __shared__ float tile[];
-  const SlMap m = maps[blockIdx.z];
-  const int ip = blockIdx.x;  // uniform per block
-  const bool active = ip < m.np;
-  const int prow = m.p_row + ip;
-  if (active) {
-    for (int u = 0; u < m.na; u++) {
-      if (!pallowed[(long long)prow * m.na + u]) continue;
-      const float *__restrict__ src = G + m.g_off + ((long long)u * m.np + ip) * m.ne;
-      float *dst = tile + u * m.ne;
-      for (int e = threadIdx.x; e < m.ne; e += blockDim.x) dst[e] = src[e];
-    }
-  }
-  __syncthreads();
-  float change = 0.f;
-  if (active) {
-    const int pp = ppos[prow];
-    for (int ie = threadIdx.x; ie < m.ne; ie += blockDim.x) {
-      const int erow = m.e_row + ie;
-      const long long pair = m.v_off + (long long)ip * m.ne + ie;
-      const float start = V[pair];
-      float best = start;
-      if (inside[m.tab_off + (long long)pp * m.npos_e + epos[erow]]) {
-        auto row = [&](int u) { return tile + u * m.ne; };
-        best = pair_backup<EARLY, ORDERED>(m, prow, erow, start, pair, row, eids, ew,
-                                           eallowed, pallowed, order);
-      }
-      Vn[pair] = best;
-      change = fmaxf(change, fabsf(best - start));
-    }
-  }
-  block_residual(change, slot + blockIdx.z);
-}
 
 // Field sampled at continuous pair states through the product of the two
 // cars' 16-vertex stencils (reference.py:127-135, query_field).  Pairs
@@ -306,49 +84,13 @@ __global__ void sl_fill_f32(float *p, long long n) {
 
 using namespace pe;
 
-static dim3 pair_grid(const pe_sl *h) {
-  const unsigned gy = (unsigned)std::min(h->np_max, 65535);
-  return dim3((unsigned)((h->ne_max + 255) / 256), gy, (unsigned)h->B);
-}
-
-static void sl_record(pe_sl *h, int kind, bool start) {
-  if (!h->profiling) return;
-  hipEvent_t e;
-  if (hipEventCreate(&e) != hipSuccess) return;
-  hipEventRecord(e, h->stream);
-  h->events.push_back(e);
-  if (start) h->kinds.push_back(kind);
-}
-
-// Enqueues one sweep va -> vb writing its residuals to slot[0..B).
+// One sweep va -> vb in the handle's precision.
 static void enqueue_sweep(pe_sl *h, const float *src, float *dst, int flags,
                           unsigned long long *slot, bool record) {
-  const dim3 grid = pair_grid(h), block(256);
-  if (record) sl_record(h, 0, true);
-  hipLaunchKernelGGL(sl_p_side, grid, block, 0, h->stream, h->d_maps, src,
-                     h->G, h->pids, h->pw, h->pallowed, h->ppos, h->epos,
-                     h->near);
-  if (record) sl_record(h, 0, false);
-  if (record) sl_record(h, 1, true);
-  const bool early = flags & PE_SL_EARLY_EXIT, ordered = flags & PE_SL_ORDERED;
-  const bool lds = h->lds_bytes > 0 && !(flags & PE_SL_GLOBAL_GATHER);
-  const dim3 lds_grid((unsigned)h->np_max, 1, (unsigned)h->B), lds_block(1024);
-#define PE_E_SIDE(E, O)                                                           \
-  if (lds)                                                                        \
-    hipLaunchKernelGGL((sl_e_side_lds<E, O>), lds_grid, lds_block, h->lds_bytes,  \
-                       h->stream, h->d_maps, src, dst, h->G, h->eids, h->ew,      \
-                       h->eallowed, h->pallowed, h->ppos, h->epos, h->inside,     \
-                       h->order, slot);                                           \
-  else                                                                            \
-    hipLaunchKernelGGL((sl_e_side<E, O>), grid, block, 0, h->stream, h->d_maps,   \
-                       src, dst, h->G, h->eids, h->ew, h->eallowed, h->pallowed,  \
-                       h->ppos, h->epos, h->inside, h->order, slot)
-  if (early && ordered) PE_E_SIDE(true, true);
-  else if (early) PE_E_SIDE(true, false);
-  else if (ordered) PE_E_SIDE(false, true);
-  else PE_E_SIDE(false, false);
-#undef PE_E_SIDE
-  if (record) sl_record(h, 1, false);
+  if (h->f64)
+    sl_enqueue_f64(h, src, dst, flags, slot, record);
+  else
+    enqueue_sweep_t<float>(h, src, dst, flags, slot, record);
 }
 
 static int sl_flush(pe_sl *h) {
@@ -463,9 +205,16 @@ static int do_sweeps(pe_sl *h, int count, int flags) {
 
 extern "C" int pe_sl_create(const pe_sl_game *games, int32_t n_maps,
                             pe_sl **out) {
+  return pe_sl_create_ex(games, n_maps, 0, out);
+}
+
+extern "C" int pe_sl_create_ex(const pe_sl_game *games, int32_t n_maps, int32_t precision,
+                               pe_sl **out) {
   clear_error();
   if (!games || !out || n_maps < 1 || n_maps > 65535)
     return fail(PE_ERR_INVALID, "invalid map list");
+  if (precision != 0 && precision != 1)
+    return fail(PE_ERR_INVALID, "precision must be 0 (FP32) or 1 (FP64)");
   *out = nullptr;
   const int na = games[0].p.actions;
   if (na < 1 || na > kMaxActions)
@@ -493,6 +242,8 @@ extern "C" int pe_sl_create(const pe_sl_game *games, int32_t n_maps,
   if (rc) return rc;
   pe_sl *h = new pe_sl();
   h->device = dev;
+  h->f64 = precision == 1;
+  h->esize = h->f64 ? sizeof(double) : sizeof(float);
   h->B = n_maps;
   h->na = na;
   long long prow = 0, erow = 0, voff = 0, goff = 0, toff = 0;
@@ -543,20 +294,11 @@ extern "C" int pe_sl_create(const pe_sl_game *games, int32_t n_maps,
     {
       int limit = 0;
       PE_HIP(hipDeviceGetAttribute(&limit, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
-      const size_t bytes = sizeof(float) * (size_t)na * (size_t)h->ne_max;
-      if (bytes + 256 <= (size_t)limit) {
-        // If the runtime refuses the LDS size for any variant, the global-gather
-        // E side is used instead.
-        bool ok = true;
-        for (const void *f : {(const void *)sl_e_side_lds<true, true>,
-                              (const void *)sl_e_side_lds<true, false>,
-                              (const void *)sl_e_side_lds<false, true>,
-                              (const void *)sl_e_side_lds<false, false>})
-          ok = ok && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)bytes) == hipSuccess;
-        if (ok) h->lds_bytes = bytes;
-        else (void)hipGetLastError();
-      }
+      const size_t bytes = h->esize * (size_t)na * (size_t)h->ne_max;
+      // If the runtime refuses the LDS size, the global-gather E side is used.
+      if (bytes + 256 <= (size_t)limit &&
+          (h->f64 ? sl_allow_lds_f64(bytes) : sl_allow_lds<float>(bytes)))
+        h->lds_bytes = bytes;
     }
     PE_HIP(hipEventCreate(&h->t0));
     PE_HIP(hipEventCreate(&h->t1));
@@ -609,14 +351,14 @@ extern "C" int pe_sl_create(const pe_sl_game *games, int32_t n_maps,
       PE_HIP(hipMemcpy(h->near + at, games[b].near, sz, hipMemcpyHostToDevice));
       at += sz;
     }
-    PE_HIP(hipMalloc((void **)&h->va, sizeof(float) * h->pairs));
-    PE_HIP(hipMalloc((void **)&h->vb, sizeof(float) * h->pairs));
+    PE_HIP(hipMalloc((void **)&h->va, h->esize * h->pairs));
+    PE_HIP(hipMalloc((void **)&h->vb, h->esize * h->pairs));
     PE_HIP(hipMalloc((void **)&h->order, h->pairs));
-    PE_HIP(hipMalloc((void **)&h->G, sizeof(float) * h->g_size));
+    PE_HIP(hipMalloc((void **)&h->G, h->esize * h->g_size));
     PE_HIP(hipMalloc((void **)&h->ring, 8ll * h->ring_slots * n_maps));
     PE_HIP(hipMalloc((void **)&h->gslots, 8ll * h->graph_sweeps * n_maps));
     hipLaunchKernelGGL(sl_fill_f32, dim3(4096), dim3(256), 0, h->stream, h->G,
-                       h->g_size);
+                       h->g_size * (long long)(h->esize / sizeof(float)));
     PE_LAUNCHED(sl_fill_f32);
     const long long nr = 1ll * h->ring_slots * n_maps;
     hipLaunchKernelGGL(sl_fill_u64, dim3((unsigned)((nr + 255) / 256)), dim3(256),
@@ -633,25 +375,44 @@ extern "C" int pe_sl_create(const pe_sl_game *games, int32_t n_maps,
   return PE_OK;
 }
 
-extern "C" int pe_sl_initialize(pe_sl *h, const float *v0) {
+// Payoff initialisation plus an optional given field (float or double
+// values, converted to the handle's precision on the host).
+template <typename S>
+static int initialize_from(pe_sl *h, const S *v0) {
   clear_error();
   if (!h) return fail(PE_ERR_INVALID, "null handle");
   PE_HIP(hipSetDevice(h->device));
-  hipLaunchKernelGGL(sl_initialize, pair_grid(h), dim3(256), 0, h->stream,
-                     h->d_maps, h->pstates, h->estates, h->ppos, h->epos,
-                     h->inside, h->va, h->order);
+  if (h->f64) {
+    sl_initialize_f64(h);
+  } else {
+    hipLaunchKernelGGL(sl_initialize<float>, pair_grid(h), dim3(256), 0, h->stream,
+                       h->d_maps, h->pstates, h->estates, h->ppos, h->epos, h->inside, h->va,
+                       h->order);
+  }
   PE_LAUNCHED(sl_initialize);
   if (v0) {
     for (long long k = 0; k < h->pairs; k++)
       if (!std::isfinite(v0[k])) return fail(PE_ERR_INVALID, "non-finite initial field");
-    PE_HIP(hipMemcpyAsync(h->va, v0, sizeof(float) * h->pairs,
-                          hipMemcpyHostToDevice, h->stream));
+    PE_HIP(hipStreamSynchronize(h->stream));
+    if (h->f64) {
+      std::vector<double> v(v0, v0 + h->pairs);
+      PE_HIP(hipMemcpy(h->va, v.data(), sizeof(double) * h->pairs, hipMemcpyHostToDevice));
+    } else {
+      std::vector<float> v(v0, v0 + h->pairs);
+      PE_HIP(hipMemcpy(h->va, v.data(), sizeof(float) * h->pairs, hipMemcpyHostToDevice));
+    }
   }
   PE_HIP(hipStreamSynchronize(h->stream));
   h->trace.clear();
   h->sweeps = 0;
   h->last = 0;
   return PE_OK;
+}
+
+extern "C" int pe_sl_initialize(pe_sl *h, const float *v0) { return initialize_from(h, v0); }
+
+extern "C" int pe_sl_initialize_f64(pe_sl *h, const double *v0) {
+  return initialize_from(h, v0);
 }
 
 extern "C" int pe_sl_sweep(pe_sl *h, int32_t sweeps, int32_t flags) {
@@ -684,16 +445,38 @@ extern "C" int pe_sl_run(pe_sl *h, double tol, int32_t max_sweeps,
   return PE_OK;
 }
 
-extern "C" int pe_sl_read(pe_sl *h, int32_t map, float *field) {
+// Copies map `map` of the current field out, converting between the
+// handle's precision and the caller's.
+template <typename S>
+static int read_into(pe_sl *h, int32_t map, S *field) {
   clear_error();
   if (!h || !field || map < 0 || map >= h->B)
     return fail(PE_ERR_INVALID, "invalid argument");
   PE_HIP(hipSetDevice(h->device));
   const SlMap &m = h->maps[map];
-  PE_HIP(hipMemcpyAsync(field, h->va + m.v_off, sizeof(float) * m.np * m.ne,
-                        hipMemcpyDeviceToHost, h->stream));
+  const long long n = (long long)m.np * m.ne;
   PE_HIP(hipStreamSynchronize(h->stream));
+  if (h->f64) {
+    std::vector<double> v((size_t)n);
+    PE_HIP(hipMemcpy(v.data(), reinterpret_cast<double *>(h->va) + m.v_off, sizeof(double) * n,
+                     hipMemcpyDeviceToHost));
+    for (long long k = 0; k < n; k++) field[k] = (S)v[k];
+  } else if (std::is_same<S, float>::value) {
+    PE_HIP(hipMemcpy(field, h->va + m.v_off, sizeof(float) * n, hipMemcpyDeviceToHost));
+  } else {
+    std::vector<float> v((size_t)n);
+    PE_HIP(hipMemcpy(v.data(), h->va + m.v_off, sizeof(float) * n, hipMemcpyDeviceToHost));
+    for (long long k = 0; k < n; k++) field[k] = (S)v[k];
+  }
   return PE_OK;
+}
+
+extern "C" int pe_sl_read(pe_sl *h, int32_t map, float *field) {
+  return read_into(h, map, field);
+}
+
+extern "C" int pe_sl_read_f64(pe_sl *h, int32_t map, double *field) {
+  return read_into(h, map, field);
 }
 
 extern "C" int pe_sl_query(pe_sl *h, int32_t map, int64_t n,
@@ -704,6 +487,7 @@ extern "C" int pe_sl_query(pe_sl *h, int32_t map, int64_t n,
   if (!h || map < 0 || map >= h->B || n < 0 || (n && (!p_ids || !p_weights ||
                                                       !e_ids || !e_weights || !out)))
     return fail(PE_ERR_INVALID, "invalid argument");
+  if (h->f64) return fail(PE_ERR_INVALID, "field queries need an FP32 handle");
   const SlMap &m = h->maps[map];
   for (long long k = 0; k < n * kStencil; k++)
     if (p_ids[k] < 0 || p_ids[k] >= m.np || e_ids[k] < 0 || e_ids[k] >= m.ne)

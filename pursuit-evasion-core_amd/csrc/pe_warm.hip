@@ -97,6 +97,7 @@ extern "C" int pe_sl_warm_start(pe_sl *sl, pe_mlp *mlp, float delta) {
   clear_error();
   if (!sl || !mlp || !std::isfinite(delta)) return fail(PE_ERR_INVALID, "invalid argument");
   if (mlp->shape.raw != 11) return fail(PE_ERR_INVALID, "the network needs extra = 3 inputs");
+  if (sl->f64) return fail(PE_ERR_INVALID, "warm starts need an FP32 solver handle");
   if (sl->device != mlp->device) return fail(PE_ERR_INVALID, "handles on different devices");
   PE_HIP(hipSetDevice(sl->device));
   PE_HIP(hipStreamSynchronize(sl->stream));

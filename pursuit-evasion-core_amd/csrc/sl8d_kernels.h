@@ -1,0 +1,318 @@
+// Kernels of the compact 8D semi-Lagrangian backup, templated on the value
+// type.  pe_sl8d.hip instantiates them in FP32 (the product path) and
+// pe_sl8d_f64.hip in FP64, compiled without contraction so that every
+// stencil sum is the oracle's: zero weights skipped, then, left to right,
+// a multiply and an add (oracle/sl8d.c follows local_game/grid_game.py).
+#pragma once
+
+#include "pe_internal.h"
+
+namespace pe {
+
+// acc + w * v: one FP32 fma, or the FP64 multiply-then-add of positive
+// weights only.
+__device__ __forceinline__ float madd(float w, float v, float acc) { return fmaf(w, v, acc); }
+__device__ __forceinline__ double madd(float w, double v, double acc) {
+  return w > 0.f ? acc + (double)w * v : acc;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void sl_initialize(
+    const SlMap *__restrict__ maps, const double *__restrict__ pstates,
+    const double *__restrict__ estates, const int *__restrict__ ppos,
+    const int *__restrict__ epos, const uint8_t *__restrict__ inside,
+    T *__restrict__ V, uint8_t *__restrict__ order) {
+  const SlMap m = maps[blockIdx.z];
+  const int ie = blockIdx.x * 256 + threadIdx.x;
+  if (ie >= m.ne) return;
+  for (int ip = blockIdx.y; ip < m.np; ip += gridDim.y) {
+    const long long pair = m.v_off + (long long)ip * m.ne + ie;
+    order[pair] = 0xFF;
+    const int pp = ppos[m.p_row + ip], ep = epos[m.e_row + ie];
+    if (!inside[m.tab_off + (long long)pp * m.npos_e + ep]) {
+      V[pair] = m.exit_value;
+      continue;
+    }
+    // Rectangle-contact payoff (local_game/geometry.py:15-36), in double.
+    const double *a = pstates + 4ll * (m.p_row + ip);
+    const double *b = estates + 4ll * (m.e_row + ie);
+    const double dx = b[0] - a[0], dy = b[1] - a[1], rel = b[2] - a[2];
+    const double ac = fabs(cos(rel)), as = fabs(sin(rel));
+    const double cp = cos(a[2]), sp = sin(a[2]), ce = cos(b[2]), se = sin(b[2]);
+    const double g0 = fabs(dx * cp + dy * sp) - .5 * (m.pl + m.el * ac + m.ew * as);
+    const double g1 = fabs(-dx * sp + dy * cp) - .5 * (m.pw + m.el * as + m.ew * ac);
+    const double g2 = fabs(dx * ce + dy * se) - .5 * (m.el + m.pl * ac + m.pw * as);
+    const double g3 = fabs(-dx * se + dy * ce) - .5 * (m.ew + m.pl * as + m.pw * ac);
+    V[pair] = T(fmax(fmax(g0, g1), fmax(g2, g3)));
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void sl_p_side(
+    const SlMap *__restrict__ maps, const T *__restrict__ V,
+    T *__restrict__ G, const int *__restrict__ pids,
+    const float *__restrict__ pw, const uint8_t *__restrict__ pallowed,
+    const int *__restrict__ ppos, const int *__restrict__ epos,
+    const uint8_t *__restrict__ near) {
+  const SlMap m = maps[blockIdx.z];
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= m.ne) return;
+  const T *__restrict__ Vm = V + m.v_off;
+  const int ep = epos[m.e_row + e];
+  for (int ip = blockIdx.y; ip < m.np; ip += gridDim.y) {
+    const int row = m.p_row + ip;
+    if (!near[m.tab_off + (long long)ppos[row] * m.npos_e + ep]) continue;
+    for (int u = 0; u < m.na; u++) {
+      const long long cmd = (long long)row * m.na + u;
+      if (!pallowed[cmd]) continue;
+      const int *id = pids + cmd * kStencil;
+      const float *w = pw + cmd * kStencil;
+      T acc = 0;
+#pragma unroll
+      for (int k = 0; k < kStencil; k++) acc = madd(w[k], Vm[(long long)id[k] * m.ne + e], acc);
+      G[m.g_off + ((long long)u * m.np + ip) * m.ne + e] = acc;
+    }
+  }
+}
+
+// i-th command of the order that puts `first` in front (others ascending).
+__device__ __forceinline__ int nth_command(int i, int first) {
+  return i == 0 ? first : (i <= first ? i - 1 : i);
+}
+
+// Min over allowed P commands of the max over allowed E commands of one
+// inside pair (ip, ie), starting from its current value; row(u) is the
+// row G_u(ip, .).  Early exit: a P command whose running max over E
+// commands reaches the running min cannot lower it (grid_game.py:409-421).
+// Updates the stored command order of the pair.
+template <bool EARLY, bool ORDERED, typename T, typename Row>
+__device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T start,
+                                             long long pair, Row row,
+                                             const int *__restrict__ eids,
+                                             const float *__restrict__ ew,
+                                             const uint8_t *__restrict__ eallowed,
+                                             const uint8_t *__restrict__ pallowed,
+                                             uint8_t *__restrict__ order) {
+  T best = start;
+  int ufirst = 0, bfirst = 0;
+  if (ORDERED) {
+    const int o = order[pair];
+    if (o != 0xFF) {
+      ufirst = o & 15;
+      bfirst = o >> 4;
+    }
+  }
+  int best_u = -1, best_b = -1;
+  for (int i = 0; i < m.na; i++) {
+    const int u = ORDERED ? nth_command(i, ufirst) : i;
+    if (!pallowed[(long long)prow * m.na + u]) continue;
+    const T *Gu = row(u);
+    T worst = T(-1e30);
+    int arg = -1;
+    bool cut = false;
+    for (int j = 0; j < m.na; j++) {
+      const int b = ORDERED ? nth_command(j, bfirst) : j;
+      const long long cmd = (long long)erow * m.na + b;
+      if (!eallowed[cmd]) continue;
+      const int4 *id4 = reinterpret_cast<const int4 *>(eids + cmd * kStencil);
+      const float4 *w4 = reinterpret_cast<const float4 *>(ew + cmd * kStencil);
+      T q = 0;
+#pragma unroll
+      for (int c = 0; c < kStencil / 4; c++) {
+        const int4 id = id4[c];
+        const float4 w = w4[c];
+        q = madd(w.x, Gu[id.x], q);
+        q = madd(w.y, Gu[id.y], q);
+        q = madd(w.z, Gu[id.z], q);
+        q = madd(w.w, Gu[id.w], q);
+      }
+      if (q > worst) {
+        worst = q;
+        arg = b;
+      }
+      if (EARLY && worst >= best) {
+        cut = true;  // this P command cannot lower the value
+        break;
+      }
+    }
+    if (!cut && worst < best) {
+      best = worst;
+      best_u = u;
+      best_b = arg;
+    }
+  }
+  if (ORDERED && best_u >= 0) order[pair] = (uint8_t)(best_u | (best_b << 4));
+  return best;
+}
+
+// Block max of `change` into the map's residual slot (blockDim.x <= 1024).
+template <typename T>
+__device__ __forceinline__ void block_residual(T change, unsigned long long *slot) {
+  __shared__ T red[16];
+  const T w = wave_max(change);
+  const int waves = (int)(blockDim.x + 63) >> 6;
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = w;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    T b = red[0];
+    for (int k = 1; k < waves; k++) b = b > red[k] ? b : red[k];
+    raise_max(slot, (double)b);
+  }
+}
+
+// E side with G gathered from global memory: one thread per pair.
+template <typename T, bool EARLY, bool ORDERED>
+__global__ __launch_bounds__(256) void sl_e_side(
+    const SlMap *__restrict__ maps, const T *__restrict__ V,
+    T *__restrict__ Vn, const T *__restrict__ G,
+    const int *__restrict__ eids, const float *__restrict__ ew,
+    const uint8_t *__restrict__ eallowed, const uint8_t *__restrict__ pallowed,
+    const int *__restrict__ ppos, const int *__restrict__ epos,
+    const uint8_t *__restrict__ inside, uint8_t *__restrict__ order,
+    unsigned long long *__restrict__ slot) {
+  const SlMap m = maps[blockIdx.z];
+  const int ie = blockIdx.x * 256 + threadIdx.x;
+  T change = 0;
+  if (ie < m.ne) {
+    const int erow = m.e_row + ie;
+    const int ep = epos[erow];
+    for (int ip = blockIdx.y; ip < m.np; ip += gridDim.y) {
+      const long long pair = m.v_off + (long long)ip * m.ne + ie;
+      const T start = V[pair];
+      T best = start;
+      const int prow = m.p_row + ip;
+      if (inside[m.tab_off + (long long)ppos[prow] * m.npos_e + ep]) {
+        auto row = [&](int u) { return G + m.g_off + ((long long)u * m.np + ip) * m.ne; };
+        best = pair_backup<EARLY, ORDERED>(m, prow, erow, start, pair, row, eids, ew,
+                                           eallowed, pallowed, order);
+      }
+      Vn[pair] = best;
+      const T d = best > start ? best - start : start - best;
+      change = change > d ? change : d;
+    }
+  }
+  block_residual(change, slot + blockIdx.z);
+}
+
+// E side with the G rows of one P state staged in LDS: one block per
+// (P state, map) walks the whole E row, so the 16-point E gathers of every
+// pair read LDS instead of scattered cache lines.  Dynamic LDS holds
+// [na][ne] floats; only the rows of allowed P commands are filled, and only
+// those are read.  Same arithmetic as sl_e_side, value for value.
+template <typename T, bool EARLY, bool ORDERED>
+__global__ __launch_bounds__(1024) void sl_e_side_lds(
+    const SlMap *__restrict__ maps, const T *__restrict__ V,
+    T *__restrict__ Vn, const T *__restrict__ G,
+    const int *__restrict__ eids, const float *__restrict__ ew,
+    const uint8_t *__restrict__ eallowed, const uint8_t *__restrict__ pallowed,
+    const int *__restrict__ ppos, const int *__restrict__ epos,
+    const uint8_t *__restrict__ inside, uint8_t *__restrict__ order,
+    unsigned long long *__restrict__ slot) {
+  extern __shared__ __align__(16) unsigned char tile_bytes[];
+  T *tile = reinterpret_cast<T *>(tile_bytes);
+  const SlMap m = maps[blockIdx.z];
+  const int ip = blockIdx.x;  // uniform per block
+  const bool active = ip < m.np;
+  const int prow = m.p_row + ip;
+  if (active) {
+    for (int u = 0; u < m.na; u++) {
+      if (!pallowed[(long long)prow * m.na + u]) continue;
+      const T *__restrict__ src = G + m.g_off + ((long long)u * m.np + ip) * m.ne;
+      T *dst = tile + u * m.ne;
+      for (int e = threadIdx.x; e < m.ne; e += blockDim.x) dst[e] = src[e];
+    }
+  }
+  __syncthreads();
+  T change = 0;
+  if (active) {
+    const int pp = ppos[prow];
+    for (int ie = threadIdx.x; ie < m.ne; ie += blockDim.x) {
+      const int erow = m.e_row + ie;
+      const long long pair = m.v_off + (long long)ip * m.ne + ie;
+      const T start = V[pair];
+      T best = start;
+      if (inside[m.tab_off + (long long)pp * m.npos_e + epos[erow]]) {
+        auto row = [&](int u) { return tile + u * m.ne; };
+        best = pair_backup<EARLY, ORDERED>(m, prow, erow, start, pair, row, eids, ew,
+                                           eallowed, pallowed, order);
+      }
+      Vn[pair] = best;
+      const T d = best > start ? best - start : start - best;
+      change = change > d ? change : d;
+    }
+  }
+  block_residual(change, slot + blockIdx.z);
+}
+
+
+inline dim3 pair_grid(const pe_sl *h) {
+  const unsigned gy = (unsigned)std::min(h->np_max, 65535);
+  return dim3((unsigned)((h->ne_max + 255) / 256), gy, (unsigned)h->B);
+}
+
+inline void sl_record(pe_sl *h, int kind, bool start) {
+  if (!h->profiling) return;
+  hipEvent_t e;
+  if (hipEventCreate(&e) != hipSuccess) return;
+  hipEventRecord(e, h->stream);
+  h->events.push_back(e);
+  if (start) h->kinds.push_back(kind);
+}
+
+// Enqueues one sweep src -> dst (T values) writing its residuals to
+// slot[0..B).
+template <typename T>
+void enqueue_sweep_t(pe_sl *h, const void *src_, void *dst_, int flags,
+                     unsigned long long *slot, bool record) {
+  const T *src = static_cast<const T *>(src_);
+  T *dst = static_cast<T *>(dst_);
+  T *G = reinterpret_cast<T *>(h->G);
+  const dim3 grid = pair_grid(h), block(256);
+  if (record) sl_record(h, 0, true);
+  hipLaunchKernelGGL(sl_p_side<T>, grid, block, 0, h->stream, h->d_maps, src, G,
+                     h->pids, h->pw, h->pallowed, h->ppos, h->epos, h->near);
+  if (record) sl_record(h, 0, false);
+  if (record) sl_record(h, 1, true);
+  const bool early = flags & PE_SL_EARLY_EXIT, ordered = flags & PE_SL_ORDERED;
+  const bool lds = h->lds_bytes > 0 && !(flags & PE_SL_GLOBAL_GATHER);
+  const dim3 lds_grid((unsigned)h->np_max, 1, (unsigned)h->B), lds_block(1024);
+#define PE_E_SIDE(E, O)                                                           \
+  if (lds)                                                                        \
+    hipLaunchKernelGGL((sl_e_side_lds<T, E, O>), lds_grid, lds_block, h->lds_bytes, \
+                       h->stream, h->d_maps, src, dst, G, h->eids, h->ew,           \
+                       h->eallowed, h->pallowed, h->ppos, h->epos, h->inside,       \
+                       h->order, slot);                                             \
+  else                                                                              \
+    hipLaunchKernelGGL((sl_e_side<T, E, O>), grid, block, 0, h->stream, h->d_maps,  \
+                       src, dst, G, h->eids, h->ew, h->eallowed, h->pallowed,       \
+                       h->ppos, h->epos, h->inside, h->order, slot)
+  if (early && ordered) PE_E_SIDE(true, true);
+  else if (early) PE_E_SIDE(true, false);
+  else if (ordered) PE_E_SIDE(false, true);
+  else PE_E_SIDE(false, false);
+#undef PE_E_SIDE
+  if (record) sl_record(h, 1, false);
+}
+
+// Lets the LDS E side use `bytes` of dynamic LDS for every flag variant;
+// false if the runtime refuses.
+template <typename T>
+bool sl_allow_lds(size_t bytes) {
+  bool ok = true;
+  for (const void *f : {(const void *)sl_e_side_lds<T, true, true>,
+                        (const void *)sl_e_side_lds<T, true, false>,
+                        (const void *)sl_e_side_lds<T, false, true>,
+                        (const void *)sl_e_side_lds<T, false, false>})
+    ok = ok && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) ==
+                   hipSuccess;
+  if (!ok) (void)hipGetLastError();
+  return ok;
+}
+
+// FP64 instances (pe_sl8d_f64.hip, built without contraction).
+void sl_enqueue_f64(pe_sl *h, const void *src, void *dst, int flags, unsigned long long *slot,
+                    bool record);
+void sl_initialize_f64(pe_sl *h);
+bool sl_allow_lds_f64(size_t bytes);
+
+}  // namespace pe

@@ -209,6 +209,9 @@ SIGNATURES = {
     "pe_geom_actions": (ctypes.c_int, [P, I64, P, I32, F64, P, I32, F64, P, P, P]),
     "pe_geom_connections": (ctypes.c_int, [P, P, I64, P, I32, F64, P, P]),
     "pe_sl_create": (ctypes.c_int, [ctypes.POINTER(SLGame), I32, P]),
+    "pe_sl_create_ex": (ctypes.c_int, [ctypes.POINTER(SLGame), I32, I32, P]),
+    "pe_sl_initialize_f64": (ctypes.c_int, [P, P]),
+    "pe_sl_read_f64": (ctypes.c_int, [P, I32, P]),
     "pe_sl_initialize": (ctypes.c_int, [P, P]),
     "pe_sl_sweep": (ctypes.c_int, [P, I32, I32]),
     "pe_sl_run": (ctypes.c_int, [P, F64, I32, I32, I32, P, P]),
@@ -357,8 +360,9 @@ class SL:
     pe_amd.local_game.grid_game.game_tables); they are copied to the device
     at construction."""
 
-    def __init__(self, games):
+    def __init__(self, games, precision=0):
         self._keep = []
+        self.precision = precision
         arr = (SLGame * len(games))()
         for k, g in enumerate(games):
             for side in ("p", "e"):
@@ -385,7 +389,7 @@ class SL:
             arr[k].p_length, arr[k].p_width = g["p_body"]
             arr[k].e_length, arr[k].e_width = g["e_body"]
         self._h = ctypes.c_void_p()
-        check(lib().pe_sl_create(arr, len(games), ctypes.byref(self._h)))
+        check(lib().pe_sl_create_ex(arr, len(games), int(precision), ctypes.byref(self._h)))
         self._keep = []
         self.shapes = [(len(g["p"]["states"]), len(g["e"]["states"])) for g in games]
 
@@ -407,10 +411,13 @@ class SL:
         self.close()
 
     def initialize(self, v0=None):
+        dtype = np.float64 if (v0 is not None and np.asarray(
+            v0[0] if isinstance(v0, (list, tuple)) else v0).dtype == np.float64) else np.float32
         a = None if v0 is None else np.ascontiguousarray(
             np.concatenate([np.ravel(v) for v in v0]) if isinstance(v0, (list, tuple)) else v0,
-            np.float32)
-        check(lib().pe_sl_initialize(self._h, ptr(a)))
+            dtype)
+        fn = lib().pe_sl_initialize_f64 if dtype == np.float64 else lib().pe_sl_initialize
+        check(fn(self._h, ptr(a)))
 
     def sweep(self, count=1, flags=PE_SL_EARLY_EXIT):
         check(lib().pe_sl_sweep(self._h, int(count), int(flags)))
@@ -421,10 +428,14 @@ class SL:
                               int(flags), ctypes.byref(done), ctypes.byref(conv)))
         return done.value, bool(conv.value)
 
-    def read(self, index=0):
+    def read(self, index=0, dtype=None):
+        """Field of map `index` [p.n, e.n]; float32, or float64 for FP64
+        handles (or when dtype=np.float64 is asked for)."""
         np_, ne = self.shapes[index]
-        out = np.empty((np_, ne), np.float32)
-        check(lib().pe_sl_read(self._h, int(index), ptr(out)))
+        dtype = dtype or (np.float64 if self.precision == 1 else np.float32)
+        out = np.empty((np_, ne), dtype)
+        fn = lib().pe_sl_read_f64 if out.dtype == np.float64 else lib().pe_sl_read
+        check(fn(self._h, int(index), ptr(out)))
         return out
 
     def query(self, p_ids, p_weights, e_ids, e_weights, index=0):

@@ -319,5 +319,5 @@ def dense_inside(p, e, exit_radius):
 class GameSolver(_native.SL):
     """Device-resident value iteration of one or more contact-game maps."""
 
-    def __init__(self, tables):
-        super().__init__(tables if isinstance(tables, (list, tuple)) else [tables])
+    def __init__(self, tables, precision=0):
+        super().__init__(tables if isinstance(tables, (list, tuple)) else [tables], precision)

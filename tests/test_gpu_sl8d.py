@@ -12,6 +12,11 @@
 * The early exit, the control ordering, the hipGraph replay and the choice
   of E-side kernel (G rows staged in LDS, or gathered from global memory)
   change nothing: results are bit-identical to the plain sweep.
+* FP64 solver handles do the oracle's arithmetic and match it bit for bit
+  (seeded tables over several sweeps, one full C2 sweep).  With them, the
+  FP32 solve of C2 run to residual 1e-4 is compared with the FP64 solve
+  after the same sweeps: max|dV| <= 1e-3, capture set equal except within
+  1e-3 of zero.
 """
 
 from types import SimpleNamespace
@@ -263,3 +268,64 @@ def test_batched_maps_equal_individual_solves(gpu):
             s1.sweep(10, EARLY | ORDERED)
             np.testing.assert_array_equal(s1.read(), batched[k])
             np.testing.assert_array_equal(s1.trace()[:, 0], traces[:, k])
+
+
+@pytest.mark.parametrize("flags", [0, EARLY | ORDERED, EARLY | ORDERED | GRAPH,
+                                   EARLY | ORDERED | GLOBAL])
+def test_fp64_sweeps_are_bit_exact_with_oracle(gpu, oracle_lib, flags):
+    """FP64 handles do the oracle's arithmetic (each stencil sum a separate
+    multiply and add, left to right over the positive weights), so from the
+    oracle's own initial field every sweep agrees bit for bit, whatever the
+    early exit, control order, graph replay or E-side kernel."""
+    rng = np.random.default_rng(8)
+    p, e = fake_grid(300, 9, rng), fake_grid(257, 9, rng)
+    tables, inside = fake_tables(p, e, 1.2)
+    V = oracle_lib.sl_init(p.states, e.states, inside, 1.2, Footprint(0.8, 0.5),
+                           Footprint(0.4, 0.25))
+    print(f"[gpu] fp64 sweeps flags={flags}", flush=True)
+    with GameSolver(tables, precision=1) as s:
+        s.initialize(V)
+        for k in range(4):
+            V, change = oracle_lib.sl_sweep(V, p, e, inside)
+            s.sweep(1, flags & ~GRAPH)
+        s.sweep(2, flags)
+        V, _ = oracle_lib.sl_sweep(V, p, e, inside)
+        V, change = oracle_lib.sl_sweep(V, p, e, inside)
+        got = s.read()
+        trace = s.trace()[:, 0]
+    assert got.dtype == np.float64
+    np.testing.assert_array_equal(got, V)
+    assert trace[-1] == change
+
+
+def test_c2_fp64_sweep_bit_exact_and_converged_fp32_vs_fp64(gpu, oracle_lib, c2_game):
+    """Full C2: one FP64 GPU sweep equals the FP64 oracle bit for bit; then
+    the FP32 solve run to residual max|dV| <= 1e-4 against the FP64 solve
+    after the same number of sweeps.  Bar: max|dV| <= 1e-3, capture-set
+    (V <= 0) membership equal except where |V64| <= 1e-3 (listed)."""
+    config, arena, p, e, tables = c2_game
+    inside = dense_inside(p, e, 3.0)
+    V0 = oracle_lib.sl_init(p.states, e.states, inside, 3.0, arena.pursuer, arena.evader)
+    print("[gpu] C2 FP64 sweep vs oracle", flush=True)
+    with GameSolver(tables, precision=1) as s64:
+        s64.initialize(V0)
+        s64.sweep(1, EARLY | ORDERED)
+        ref1, _ = oracle_lib.sl_sweep(V0, p, e, inside, threads=16)
+        np.testing.assert_array_equal(s64.read(), ref1)
+        print("[gpu] C2 FP32 to residual 1e-4", flush=True)
+        with GameSolver(tables) as s32:
+            s32.initialize(V0.astype(np.float32))
+            done, ok = s32.run(1e-4, 20000, 1, EARLY | ORDERED)
+            v32 = s32.read()
+        assert ok
+        s64.initialize(V0)
+        s64.sweep(done, EARLY | ORDERED | GRAPH)
+        v64 = s64.read()
+        res64 = s64.trace()[-1, 0]
+    d = np.abs(v32 - v64)
+    flip = np.flatnonzero((v32.ravel() <= 0) != (v64.ravel() <= 0))
+    print(f"C2 converged ({done} sweeps, FP64 residual {res64:.2e}): FP32 vs FP64 "
+          f"max|dV|={d.max():.3e} mean={d.mean():.3e}, capture-set flips {len(flip)} "
+          f"at |V64| {np.abs(v64.ravel()[flip])[:10].tolist()}")
+    assert d.max() <= 1e-3
+    assert (np.abs(v64.ravel()[flip]) <= 1e-3).all()
