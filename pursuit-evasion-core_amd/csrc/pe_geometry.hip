@@ -10,9 +10,14 @@
 // signed speed; obstacles [nc,3] = x, y, r; param = [low, high, a, k, L, W,
 // dt]) and return a status instead of taking a thread count.
 //
-// One thread per state (or per state x command).  The reference's recursive
-// Lipschitz bisection becomes a depth-first loop over an explicit stack,
-// visiting the same intervals in the same order, so every decision matches.
+// Clearance queries run one thread per state.  The certified-motion
+// queries (backups, actions, connections) run one wave per item: the
+// reference's recursive Lipschitz bisection becomes a depth-first loop over
+// an explicit stack, visiting the same intervals in the same order, and
+// every clearance evaluation in it is split over the wave's 64 lanes (each
+// lane a strided share of the circles, then a wave minimum).  The minimum
+// does not depend on the order the circles are visited in, so every
+// decision matches the serial walk.
 #include <cmath>
 #include <vector>
 
@@ -40,6 +45,9 @@ __device__ __forceinline__ void held_step(const double *s, double a, double k,
   out[3] = v;
 }
 
+// WAVE: the calling wave's lanes share the circles and all receive the
+// minimum.
+template <bool WAVE = false>
 __device__ double clearance(const double *s, const double *__restrict__ obs,
                             int nc, double half, const Car &m) {
   const double c = cos(s[2]), q = sin(s[2]);
@@ -47,7 +55,8 @@ __device__ double clearance(const double *s, const double *__restrict__ obs,
   const double ey = .5 * (m.L * fabs(q) + m.W * fabs(c));
   double best = fmin(half - fabs(s[0]) - ex, half - fabs(s[1]) - ey);
   const double reach = .5 * hypot(m.L, m.W);
-  for (int j = 0; j < nc; j++) {
+  const int first = WAVE ? (int)(threadIdx.x & 63) : 0, stride = WAVE ? 64 : 1;
+  for (int j = first; j < nc; j += stride) {
     const double dx = obs[3 * j] - s[0], dy = obs[3 * j + 1] - s[1];
     const double r = obs[3 * j + 2];
     const double far = reach + r + fmax(0., best);
@@ -55,6 +64,10 @@ __device__ double clearance(const double *s, const double *__restrict__ obs,
     const double lx = fabs(dx * c + dy * q) - m.L * .5;
     const double ly = fabs(-dx * q + dy * c) - m.W * .5;
     best = fmin(best, hypot(fmax(0., lx), fmax(0., ly)) + fmin(fmax(lx, ly), 0.) - r);
+  }
+  if (WAVE) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) best = fmin(best, __shfl_xor(best, off, 64));
   }
   return best;
 }
@@ -66,6 +79,7 @@ struct Piece {
 
 // All pieces of [0, t] along the held arc keep clearance > 1e-7 and are
 // certified by the Lipschitz bound lip within bisection depth 18.
+template <bool WAVE>
 __device__ bool arc_clear(const double *s, double a, double k, double t,
                           double lip, double c0, double c1,
                           const double *__restrict__ obs, int nc, double half,
@@ -82,13 +96,14 @@ __device__ bool arc_clear(const double *s, double a, double k, double t,
     const double mid = .5 * (p.lo + p.hi);
     double z[4];
     held_step(s, a, k, mid, m, z);
-    const double cm = clearance(z, obs, nc, half, m);
+    const double cm = clearance<WAVE>(z, obs, nc, half, m);
     stack[top++] = {mid, p.hi, cm, p.cr, p.depth + 1};  // visited second
     stack[top++] = {p.lo, mid, p.cl, cm, p.depth + 1};  // visited first
   }
   return true;
 }
 
+template <bool WAVE>
 __device__ bool motion_safe(const double *s, double a, double k, double t,
                             const double *__restrict__ obs, int nc, double half,
                             const Car &m) {
@@ -96,16 +111,17 @@ __device__ bool motion_safe(const double *s, double a, double k, double t,
   held_step(s, a, k, t, m, end);
   const double lip =
       fmax(fabs(s[3]), fabs(end[3])) * (1 + .5 * hypot(m.L, m.W) * fabs(k));
-  return arc_clear(s, a, k, t, lip, clearance(s, obs, nc, half, m),
-                   clearance(end, obs, nc, half, m), obs, nc, half, m);
+  return arc_clear<WAVE>(s, a, k, t, lip, clearance<WAVE>(s, obs, nc, half, m),
+                         clearance<WAVE>(end, obs, nc, half, m), obs, nc, half, m);
 }
 
 // Held-command braking to rest along one of k in {0, -K, +K}
 // (native_geometry.cpp:66-88): full-deceleration periods, then a final
 // period with a = -v/dt; the same arc is swept by uniform deceleration.
+template <bool WAVE>
 __device__ bool backup_safe(const double *s, const double *__restrict__ obs,
                             int nc, double half, const Car &m) {
-  if (clearance(s, obs, nc, half, m) <= 1e-7) return false;
+  if (clearance<WAVE>(s, obs, nc, half, m) <= 1e-7) return false;
   if (fabs(s[3]) < 1e-10) return true;
   const double speed = fabs(s[3]);
   int periods = (int)ceil(speed / (m.a * m.dt) - 1e-10);
@@ -116,7 +132,7 @@ __device__ bool backup_safe(const double *s, const double *__restrict__ obs,
   const double t = 2 * dist / speed, a = -s[3] / t;
   const double curv[3] = {0., -m.k, m.k};
   for (int j = 0; j < 3; j++)
-    if (motion_safe(s, a, curv[j], t, obs, nc, half, m)) return true;
+    if (motion_safe<WAVE>(s, a, curv[j], t, obs, nc, half, m)) return true;
   return false;
 }
 
@@ -130,23 +146,29 @@ __global__ __launch_bounds__(128) void geom_clearance(
   out[i] = clearance(s, obs, nc, half, m);
 }
 
-__global__ __launch_bounds__(128) void geom_backups(
-    const double *__restrict__ states, long long n,
-    const double *__restrict__ obs, int nc, double half, Car m, uint8_t *out) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double s[4] = {states[4 * i], states[4 * i + 1], states[4 * i + 2],
-                 states[4 * i + 3]};
-  out[i] = backup_safe(s, obs, nc, half, m) ? 1 : 0;
+// Item of the calling wave (4 waves per 256-thread block).
+__device__ __forceinline__ long long wave_item() {
+  return (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
 }
 
-__global__ __launch_bounds__(128) void geom_actions(
+__global__ __launch_bounds__(256) void geom_backups(
+    const double *__restrict__ states, long long n,
+    const double *__restrict__ obs, int nc, double half, Car m, uint8_t *out) {
+  const long long i = wave_item();
+  if (i >= n) return;  // uniform per wave
+  double s[4] = {states[4 * i], states[4 * i + 1], states[4 * i + 2],
+                 states[4 * i + 3]};
+  const bool good = backup_safe<true>(s, obs, nc, half, m);
+  if ((threadIdx.x & 63) == 0) out[i] = good ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void geom_actions(
     const double *__restrict__ states, long long n,
     const double *__restrict__ controls, int na, double dt,
     const double *__restrict__ obs, int nc, double half, Car m, uint8_t *ok,
     double *next) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n * na) return;
+  const long long t = wave_item();
+  if (t >= n * na) return;  // uniform per wave
   const long long i = t / na;
   const int u = (int)(t - i * na);
   double s[4] = {states[4 * i], states[4 * i + 1], states[4 * i + 2],
@@ -154,21 +176,22 @@ __global__ __launch_bounds__(128) void geom_actions(
   const double a = controls[2 * u], k = controls[2 * u + 1];
   double z[4];
   held_step(s, a, k, dt, m, z);
-  for (int c = 0; c < 4; c++) next[4 * t + c] = z[c];
-  ok[t] = (motion_safe(s, a, k, dt, obs, nc, half, m) &&
-           backup_safe(z, obs, nc, half, m))
-              ? 1
-              : 0;
+  const bool good = motion_safe<true>(s, a, k, dt, obs, nc, half, m) &&
+                    backup_safe<true>(z, obs, nc, half, m);
+  if ((threadIdx.x & 63) == 0) {
+    for (int c = 0; c < 4; c++) next[4 * t + c] = z[c];
+    ok[t] = good ? 1 : 0;
+  }
 }
 
 // Straight configuration-space segment a -> b (heading by the short way)
 // stays clear, certified by bisection to depth 20 (native_geometry.cpp:213-252).
-__global__ __launch_bounds__(128) void geom_connections(
+__global__ __launch_bounds__(256) void geom_connections(
     const double *__restrict__ from, const double *__restrict__ to,
     long long n, const double *__restrict__ obs, int nc, double half, Car m,
     uint8_t *out) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const long long i = wave_item();
+  if (i >= n) return;  // uniform per wave
   const double *a = from + 4 * i, *b = to + 4 * i;
   const double turn = remainder(b[2] - a[2], 2 * M_PI);
   const double lip =
@@ -176,7 +199,7 @@ __global__ __launch_bounds__(128) void geom_connections(
   auto at = [&](double f) {
     const double s[4] = {a[0] + f * (b[0] - a[0]), a[1] + f * (b[1] - a[1]),
                          a[2] + f * turn, 0.};
-    return clearance(s, obs, nc, half, m);
+    return clearance<true>(s, obs, nc, half, m);
   };
   Piece stack[22];
   int top = 0;
@@ -193,7 +216,7 @@ __global__ __launch_bounds__(128) void geom_connections(
     stack[top++] = {mid, p.hi, cm, p.cr, p.depth + 1};
     stack[top++] = {p.lo, mid, p.cl, cm, p.depth + 1};
   }
-  out[i] = good ? 1 : 0;
+  if ((threadIdx.x & 63) == 0) out[i] = good ? 1 : 0;
 }
 
 // Device buffers of one synchronous geometry call.
@@ -230,6 +253,11 @@ static Car read_car(const double *p) {
 
 static unsigned blocks_for(long long n) {
   return (unsigned)((n + 127) / 128 > 0 ? (n + 127) / 128 : 1);
+}
+
+// 256-thread blocks of four one-wave items.
+static unsigned wave_blocks_for(long long n) {
+  return (unsigned)((n + 3) / 4 > 0 ? (n + 3) / 4 : 1);
 }
 
 }  // namespace pe
@@ -270,7 +298,7 @@ extern "C" int pe_geom_backups(const double *states, int64_t n,
   if ((rc = s.upload(&d_s, states, n * 32)) || (rc = s.upload(&d_o, obstacles, nc * 24)) ||
       (rc = s.alloc(&d_out, n)))
     return rc;
-  hipLaunchKernelGGL(geom_backups, dim3(blocks_for(n)), dim3(128), 0, 0,
+  hipLaunchKernelGGL(geom_backups, dim3(wave_blocks_for(n)), dim3(256), 0, 0,
                      (const double *)d_s, (long long)n, (const double *)d_o, nc,
                      half, read_car(param), (uint8_t *)d_out);
   PE_LAUNCHED(geom_backups);
@@ -297,7 +325,7 @@ extern "C" int pe_geom_actions(const double *states, int64_t n,
       (rc = s.upload(&d_o, obstacles, nc * 24)) || (rc = s.alloc(&d_ok, m)) ||
       (rc = s.alloc(&d_next, m * 32)))
     return rc;
-  hipLaunchKernelGGL(geom_actions, dim3(blocks_for(m)), dim3(128), 0, 0,
+  hipLaunchKernelGGL(geom_actions, dim3(wave_blocks_for(m)), dim3(256), 0, 0,
                      (const double *)d_s, (long long)n, (const double *)d_u, na,
                      dt, (const double *)d_o, nc, half, read_car(param),
                      (uint8_t *)d_ok, (double *)d_next);
@@ -320,7 +348,7 @@ extern "C" int pe_geom_connections(const double *from, const double *to,
   if ((rc = s.upload(&d_a, from, n * 32)) || (rc = s.upload(&d_b, to, n * 32)) ||
       (rc = s.upload(&d_o, obstacles, nc * 24)) || (rc = s.alloc(&d_out, n)))
     return rc;
-  hipLaunchKernelGGL(geom_connections, dim3(blocks_for(n)), dim3(128), 0, 0,
+  hipLaunchKernelGGL(geom_connections, dim3(wave_blocks_for(n)), dim3(256), 0, 0,
                      (const double *)d_a, (const double *)d_b, (long long)n,
                      (const double *)d_o, nc, half, read_car(param),
                      (uint8_t *)d_out);
