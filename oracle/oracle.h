@@ -12,6 +12,15 @@ typedef struct {
   double vpmin, vpmax, vemin, vemax, ap, ae, kp, ke;
 } pe_oracle_hji5d_cfg;
 
+/* Dense 8D world game options; the argv flags of
+ * reversible/native_world.cpp:77-104 (float parameters are floats there). */
+typedef struct {
+  int positions, headings, p_speeds, e_speeds, order, linear_speed_boundary, threads;
+  float extent, arena, radius, p_min, p_max, e_min, e_max, p_accel, e_accel;
+  float p_curvature, e_curvature;
+  double cfl;
+} pe_oracle_world8d_cfg;
+
 /* Vehicle model parameters [low, high, a, k, L, W, dt]
  * (local_game/native_geometry.cpp:5-7,89-91). */
 typedef struct {

@@ -64,6 +64,15 @@ class HJI5DCfg(ctypes.Structure):
     ]
 
 
+class World8DCfg(ctypes.Structure):
+    """pe_oracle_world8d_cfg: native_world.cpp's argv options."""
+    _fields_ = [(k, ctypes.c_int) for k in (
+        "positions", "headings", "p_speeds", "e_speeds", "order", "linear_speed_boundary",
+        "threads")] + [(k, ctypes.c_float) for k in (
+        "extent", "arena", "radius", "p_min", "p_max", "e_min", "e_max", "p_accel", "e_accel",
+        "p_curvature", "e_curvature")] + [("cfl", ctypes.c_double)]
+
+
 def lib():
     global _LIB
     if _LIB is None:
@@ -103,6 +112,16 @@ def _declare(L):
     L.pe_oracle_hji5d_steps.restype = ctypes.c_double
     L.pe_oracle_hji5d_init.argtypes = [ctypes.POINTER(HJI5DCfg), P, P]
     L.pe_oracle_hji5d_init.restype = None
+    W = ctypes.POINTER(World8DCfg)
+    L.pe_oracle_world8d_cfl_limit.argtypes = [W]
+    L.pe_oracle_world8d_cfl_limit.restype = ctypes.c_double
+    L.pe_oracle_world8d_states.argtypes = [W]
+    L.pe_oracle_world8d_states.restype = ctypes.c_int64
+    L.pe_oracle_world8d_init.argtypes = [W, P, P]
+    L.pe_oracle_world8d_init.restype = None
+    L.pe_oracle_world8d_solve.argtypes = [W, P, P, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_double, P, P]
+    L.pe_oracle_world8d_solve.restype = ctypes.c_int64
     _declare_8d(L)
 
 
@@ -227,6 +246,39 @@ def hji5d_time_steps(cfg: HJI5DCfg, field, arrival, dt, count, t=0.0) -> float:
     return lib().pe_oracle_hji5d_steps(
         ctypes.byref(cfg), _ptr(field), _ptr(arrival), dt, t, count
     )
+
+
+# ----------------------------------------------------- dense 8D world game
+
+def world8d_cfl_limit(cfg: World8DCfg) -> float:
+    return lib().pe_oracle_world8d_cfl_limit(ctypes.byref(cfg))
+
+
+def world8d_init(cfg: World8DCfg):
+    n = lib().pe_oracle_world8d_states(ctypes.byref(cfg))
+    field = np.empty(n, np.float32)
+    arrival = np.empty(n, np.float32)
+    lib().pe_oracle_world8d_init(ctypes.byref(cfg), _ptr(field), _ptr(arrival))
+    return field, arrival
+
+
+def world8d_solve(cfg: World8DCfg, horizon, output_dt, start=0.0, field=None, arrival=None,
+                  history=True):
+    """(frames [k+1, N] including the initial field when history, final,
+    arrival, steps, cfl dt limit)."""
+    if field is None:
+        field, arrival = world8d_init(cfg)
+    field = np.ascontiguousarray(field, np.float32).copy()
+    arrival = np.ascontiguousarray(arrival, np.float32).copy()
+    count = int(np.ceil((horizon - start) / output_dt - 1e-10))
+    frames = np.empty((count, field.size), np.float32) if history else None
+    first = field.copy()
+    stats = np.zeros(1, np.float64)
+    steps = lib().pe_oracle_world8d_solve(ctypes.byref(cfg), _ptr(field), _ptr(arrival), start,
+                                          horizon, output_dt, _ptr(frames), _ptr(stats))
+    if history:
+        frames = np.concatenate([first[None], frames]) if start == 0 else frames
+    return frames, field, arrival, int(steps), float(stats[0])
 
 
 # ------------------------------------------------------------------ 8D game

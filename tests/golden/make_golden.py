@@ -1,4 +1,5 @@
-"""Generate the golden vectors for the 5D relative HJI path.
+"""Generate the golden vectors for the 5D relative HJI and the dense 8D world
+game paths.
 
 Runs the reference's own `relative5d` executable, compiled unmodified from
 reversible/native_relative.cpp into oracle/_ref by oracle/ref.mk, on
@@ -9,8 +10,12 @@ reversible/native_relative.cpp into oracle/_ref by oracle/ref.mk, on
     default ReversibleConfig, horizon 0.06, output_dt 0.02, orders 1 and 2
     (tests/test_reversible_free_dp.py:97-129);
 
-and stores its output files (fields, final, arrival, timing.json numbers) as
-compressed .npz fixtures next to this script.  Needs /root/reference, so it
+and the reference's `world8d` executable (reversible/native_world.cpp, also
+compiled unmodified by oracle/ref.mk) on small dense 8D grids with the
+default ReversibleConfig (arena 3 m, extent 3.5 m), RK2 with the linear
+speed boundary and RK1 with the legacy signed one; and stores the output
+files (fields, final, arrival, timing numbers) as compressed .npz fixtures
+next to this script.  Needs /root/reference, so it
 only runs in the build container:
 
     python tests/golden/make_golden.py
@@ -86,6 +91,44 @@ def run_case(binary: Path, case: dict, out: Path) -> dict:
     )
 
 
+# argv flag values of reversible/native_world.cpp:77-104 (default config).
+WORLD = dict(p_min=-1.0, p_max=2.0, e_min=-0.5, e_max=1.0, p_accel=2.0, e_accel=1.0,
+             p_curvature=1.0, e_curvature=2.0, radius=0.3, arena=3.0, extent=3.5)
+WORLD_CASES = {
+    "world8d_5x4x3x3_order2_linear": dict(positions=5, headings=4, p_speeds=3, e_speeds=3,
+                                          horizon=0.4, output_dt=0.1, order=2, cfl=0.8,
+                                          linear_speed_boundary=1, history=1),
+    "world8d_5x4x3x3_order1_legacy": dict(positions=5, headings=4, p_speeds=3, e_speeds=3,
+                                          horizon=0.4, output_dt=0.1, order=1, cfl=0.8,
+                                          linear_speed_boundary=0, history=1),
+    "world8d_6x6x4x3_order2_linear": dict(positions=6, headings=6, p_speeds=4, e_speeds=3,
+                                          horizon=0.3, output_dt=0.1, order=2, cfl=0.7,
+                                          linear_speed_boundary=1, history=0),
+}
+
+
+def run_world(binary: Path, case: dict, out: Path) -> dict:
+    flags = dict(WORLD)
+    flags.update(case)
+    argv = [str(binary)]
+    for k, v in flags.items():
+        argv += ["--" + k.replace("_", "-"), repr(v)]
+    argv += ["--threads", "4", "--output", str(out)]
+    subprocess.run(argv, check=True, stdout=subprocess.DEVNULL)
+    n = case["positions"] ** 4 * case["headings"] ** 2 * case["p_speeds"] * case["e_speeds"]
+    timing = json.loads((out / "native_timing.json").read_text())
+    data = dict(
+        final=np.fromfile(out / "final.f32", np.float32),
+        arrival=np.fromfile(out / "arrival.f32", np.float32),
+        states=timing["states"], internal_steps=timing["internal_steps"],
+        frames=timing["frames"], max_internal_dt=timing["max_internal_dt"],
+        argv=np.array(argv[1:]), case=json.dumps(flags),
+    )
+    if case["history"]:
+        data["fields"] = np.fromfile(out / "fields.f32", np.float32).reshape(-1, n)
+    return data
+
+
 def main() -> int:
     subprocess.run(["make", "-f", str(ROOT / "oracle" / "ref.mk")], check=True,
                    stdout=subprocess.DEVNULL)
@@ -100,6 +143,14 @@ def main() -> int:
             # and the last equals final.f32.
             data["fields"] = data["fields"][1:-1]
         np.savez_compressed(HERE / f"hji5d_{name}.npz", **data)
+        print(name, data["internal_steps"], "steps", data["frames"], "frames")
+    world = ROOT / "oracle" / "_ref" / "world8d"
+    for name, case in WORLD_CASES.items():
+        with tempfile.TemporaryDirectory() as tmp:
+            out = Path(tmp) / "out"
+            out.mkdir()
+            data = run_world(world, case, out)
+        np.savez_compressed(HERE / f"{name}.npz", **data)
         print(name, data["internal_steps"], "steps", data["frames"], "frames")
     return 0
 
