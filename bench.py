@@ -91,6 +91,53 @@ def cpu_baseline(cfg_native, sample_steps: int):
     }
 
 
+def world8d_cpu_baseline(grid_dims, sample_steps: int):
+    """cpu_baseline leg of the dense 8D line: the oracle (C/OpenMP, bit-exact
+    with the reference executable) timed for a few RK2 steps of the grid."""
+    import oracle
+
+    threads = min(os.cpu_count() or 1, 16)
+    n, h, np_, ne = grid_dims
+    ocfg = oracle.World8DCfg(n, h, np_, ne, 2, 1, threads, 3.5, 3.0, 0.3, -1.0, 2.0, -0.5, 1.0,
+                             2.0, 1.0, 1.0, 2.0, 0.8)
+    dt = oracle.world8d_cfl_limit(ocfg)
+    field, arrival = oracle.world8d_init(ocfg)
+    t0 = time.perf_counter()
+    oracle.world8d_solve(ocfg, sample_steps * dt, sample_steps * dt, field=field,
+                         arrival=arrival, history=False)
+    seconds = time.perf_counter() - t0
+    return {"value": field.size * sample_steps / seconds, "unit": "node-updates/s",
+            "cores": threads, "kind": "port",
+            "sample": f"{sample_steps} RK2 steps ({seconds:.2f} s, float arithmetic, OpenMP)"}
+
+
+def run_world8d(grid_dims, steps: int, warmup: int, cpu_steps: int = 0):
+    """Dense 8D square game (default ReversibleConfig, RK2, linear speed
+    boundary, CFL 0.8): CFL-limited steps of the whole grid."""
+    from pe_amd import _native
+    from pe_amd.reversible.model import ReversibleConfig
+    from pe_amd.reversible.world_dp import WorldGrid, native_config
+
+    grid = WorldGrid(*grid_dims)
+    with _native.World8D(native_config(ReversibleConfig(), grid, order=2)) as w:
+        dt = w.stats()["max_internal_dt"]
+        w.advance_to(warmup * dt)
+        s0 = w.stats()
+        w.advance_to((warmup + steps) * dt)
+        s1 = w.stats()
+    seconds = s1["update_seconds"] - s0["update_seconds"]
+    taken = s1["internal_steps"] - s0["internal_steps"]
+    out = {"grid": list(grid_dims), "nodes": grid.states, "steps": taken,
+           "ms_per_step": seconds / taken * 1e3,
+           "node_updates_per_s": grid.states * taken / seconds,
+           "stage_bytes_per_node": list(STAGE_BYTES),
+           "hbm_fraction_algorithmic": grid.states * sum(STAGE_BYTES) * taken / seconds
+                                       / (HBM_PEAK_GBS * 1e9)}
+    if cpu_steps:
+        out["cpu_baseline"] = world8d_cpu_baseline(grid_dims, cpu_steps)
+    return out
+
+
 def run_c2(args):
     """8D contact game C2: sweeps of the compact min-max backup."""
     from pe_amd import _native
@@ -304,7 +351,12 @@ def main() -> int:
         print("[bench] secondary: 8D contact game (seed 0)", file=sys.stderr, flush=True)
         c2 = argparse.Namespace(seed=0, warmup=10, steps=200, to_residual=1e-4,
                                 max_sweeps=20000)
-        out["secondary"] = {"hji5d_c1_to_residual": res5, "contact_game_c2": run_c2(c2)}
+        secondary = {"hji5d_c1_to_residual": res5, "contact_game_c2": run_c2(c2)}
+        print("[bench] secondary: dense 8D square game", file=sys.stderr, flush=True)
+        secondary["world8d_default"] = run_world8d(
+            (9, 8, 7, 7), 40, 2, 0 if args.no_cpu_baseline else 2)
+        secondary["world8d_13x12x7x7"] = run_world8d((13, 12, 7, 7), 20, 2)
+        out["secondary"] = secondary
     print(json.dumps(out), flush=True)
     return 0
 

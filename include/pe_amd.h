@@ -326,6 +326,77 @@ int pe_sl_warm_start(pe_sl *sl, pe_mlp *mlp, float delta);
 int pe_sl_pair_inputs(pe_sl *sl, int32_t map, int64_t first, int64_t count,
                       float *raw);
 
+/* ======================================================================
+ * Dense 8D square game: state [xP, yP, thP, vP, xE, yE, thE, vE] float32,
+ * row-major with vE fastest; ENO1/ENO2 + local Lax-Friedrichs + RK1/RK2,
+ * tube projection max(min(value, previous), avoid) on completed updates,
+ * float local arithmetic in the reference's operation order.
+ *
+ * Replaces the argv/process ABI of the reference executable
+ * reversible/native_world.cpp:76-114 (flags --positions ... --linear-speed-
+ * boundary, run() at :115-366) that reversible/world_dp.py:329-413 spawns.
+ * ====================================================================== */
+typedef struct pe_world8d_config {
+  int32_t positions, headings, p_speeds, e_speeds; /* --positions ...      */
+  int32_t order;                 /* 1 or 2 (--order)                        */
+  int32_t linear_speed_boundary; /* 1: inward one-sided slopes at the speed
+                                    ends; 0: signed extrapolation (legacy) */
+  int32_t flags;                 /* PE_WORLD8D_* launch options (0: default) */
+  int32_t reserved;
+  /* Physical values; rounded to float as the executable parses them. */
+  double extent, arena, radius;  /* --extent --arena --radius               */
+  double p_min, p_max, e_min, e_max;   /* --p-min ... --e-max              */
+  double p_accel, e_accel;             /* --p-accel --e-accel              */
+  double p_curvature, e_curvature;     /* --p-curvature --e-curvature      */
+  double cfl;                          /* --cfl, in (0, 1] (double)        */
+} pe_world8d_config;
+
+/* pe_world8d_config.flags: one thread per node everywhere (the default
+   stages each pursuer slab of evader states in LDS when it fits in 64 KiB);
+   512-thread slab blocks instead of 256.  Results are identical. */
+#define PE_WORLD8D_GENERIC 1
+#define PE_WORLD8D_WIDE_SLAB 2
+
+typedef struct pe_world8d_stats {
+  int64_t states;           /* grid nodes                                  */
+  int64_t internal_steps;   /* RK steps taken                              */
+  double time;              /* current solve time                          */
+  double max_internal_dt;   /* CFL step limit (native_world.cpp:133-135)   */
+  double update_seconds;    /* device time of the stepping                 */
+  int64_t probes;           /* probe stencils set                          */
+  int32_t kernel;           /* 0: thread per node; 1, 2: LDS slab, 256/512 */
+  int32_t reserved;
+} pe_world8d_stats;
+
+typedef struct pe_world8d pe_world8d;
+
+/* Allocate the fields on the current device and write the initial field
+ * max(target, avoid) and arrival (0 where it is <= 0, else NaN)
+ * (native_world.cpp:136-170). */
+int pe_world8d_create(const pe_world8d_config *cfg, pe_world8d **out);
+/* Continuation: replace the field (N floats) and the arrival times (NULL:
+ * `time` where the field is <= 0, else NaN), and set the solve time
+ * (native_world.cpp:171-190). */
+int pe_world8d_set_field(pe_world8d *h, const float *field, const float *arrival,
+                         double time);
+/* CFL-limited RK steps until the solve time reaches goal
+ * (native_world.cpp:317-329).  max_steps <= 0: no limit. */
+int pe_world8d_advance_to(pe_world8d *h, double goal, int64_t max_steps);
+/* Copy the current field and/or arrival times (N floats each; NULL skips). */
+int pe_world8d_read(pe_world8d *h, float *field, float *arrival);
+/* Number of non-finite values of the current field
+ * (native_world.cpp:336-344). */
+int pe_world8d_count_nonfinite(pe_world8d *h, int64_t *count);
+/* Probe stencils, 256 node indices and weights per probe, validated as in
+ * native_world.cpp:205-227 ("Bad probe stencil", "Bad probe weights"). */
+int pe_world8d_set_probes(pe_world8d *h, const uint64_t *indices,
+                          const double *weights, int64_t count);
+/* values[q] = sum_k double(field[index[q,k]]) * weight[q,k], summed in k
+ * order in float64 over the current field (native_world.cpp:241-248). */
+int pe_world8d_probe(pe_world8d *h, double *values);
+int pe_world8d_get_stats(pe_world8d *h, pe_world8d_stats *out);
+void pe_world8d_destroy(pe_world8d *h);
+
 #ifdef __cplusplus
 }
 #endif

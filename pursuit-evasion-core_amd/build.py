@@ -39,6 +39,7 @@ COMMON = [
 PER_FILE = {
     "pe_hji5d_f64.hip": ["-ffp-contract=off"],
     "pe_sl8d_f64.hip": ["-ffp-contract=off"],
+    "pe_world8d.hip": ["-ffp-contract=off"],
 }
 
 

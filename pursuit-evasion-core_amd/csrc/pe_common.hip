@@ -45,6 +45,8 @@ extern "C" int pe_abi_struct_size(const char *name) {
       {"pe_sl_game", (int)sizeof(pe_sl_game)},
       {"pe_sl_stats", (int)sizeof(pe_sl_stats)},
       {"pe_mlp_config", (int)sizeof(pe_mlp_config)},
+      {"pe_world8d_config", (int)sizeof(pe_world8d_config)},
+      {"pe_world8d_stats", (int)sizeof(pe_world8d_stats)},
   };
   for (const auto &t : table)
     if (std::strcmp(t.name, name) == 0) return t.size;

@@ -1,0 +1,615 @@
+// Dense eight-state reversible-car game in the fixed square on the GPU
+// (the native_world8d backend, reversible/native_world.cpp:32-314, behind
+// reversible/world_dp.py:177-426).
+//
+// State [xP, yP, thP, vP, xE, yE, thE, vE], row-major with vE fastest, float
+// fields and float local arithmetic.  This file is built without
+// floating-point contraction and keeps the reference's operation order, so
+// every node update is the reference's, bit for bit:
+//   * ENO1/ENO2 slopes per axis; the ENO2 selection uses <= (not <); the
+//     position axes extrapolate past their ends with the signed edge slope,
+//     headings are periodic, speeds use inward one-sided slopes at their
+//     ends (linear speed boundary) or the signed extrapolation (legacy);
+//   * Hamiltonian with local Lax-Friedrichs dissipation; RK1 or RK2;
+//   * on the completed update, the tube projection max(min(value, previous),
+//     avoid) with avoid the pursuer wall fault, and arrival times at the
+//     crossing of zero.
+// The initial field (capture or evader wall fault, max with the avoid
+// surface) is computed on the host with the same float operations.
+//
+// Layout of a launch: one thread per node; a block covers 256 consecutive
+// nodes of one (xP, yP, thP, vP) slab, so the pursuer coordinates and
+// everything derived from them are uniform per block.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#include "pe_common.h"
+
+namespace pe {
+
+struct W8 {
+  int n[8];              // axis lengths
+  long long stride[8];   // element strides
+  float h[8];            // spacings
+  long long N;           // nodes
+  int inner;             // nodes of one pursuer slab: n * n * nh * ne
+  int inner_blocks;      // ceil(inner / 256)
+  int order, linear;     // ENO order, linear speed boundary
+  float L, ap, ae, kp, ke;
+  // Inner-index decoding by multiply-high: r / n[k] = umulhi(r, magic[k])
+  // for every r < inner when fast_decode (inner * n[k] < 2^32).
+  unsigned magic[8];
+  int fast_decode;
+};
+
+__device__ __forceinline__ float w8_mn(float a, float b) { return b < a ? b : a; }
+__device__ __forceinline__ float w8_mx(float a, float b) { return a < b ? b : a; }
+
+// Field accessors: global memory by node index, or a slab staged in LDS by
+// index within the slab.  Same values, so the same arithmetic either way.
+struct W8Global {
+  const float *__restrict__ p;
+  __device__ __forceinline__ float operator()(long long i) const { return p[i]; }
+};
+struct W8Lds {
+  const float *p;
+  __device__ __forceinline__ float operator()(long long i) const { return p[(int)i]; }
+};
+
+// Value `off` cells from node j along one axis: periodic wrap, or the signed
+// linear extrapolation of the edge values past a non-periodic end.
+template <typename V>
+__device__ __forceinline__ float w8_at(const V &v, long long j, long long stride, int pos, int n,
+                                       int off, bool periodic) {
+  int q = pos + off;
+  if (periodic) {  // |off| <= 2 < n
+    q = q < 0 ? q + n : q >= n ? q - n : q;
+    return v(j + (long long)(q - pos) * stride);
+  }
+  if (q < 0) {
+    const long long edge = j - (long long)pos * stride;
+    const float a = v(edge), b = v(edge + stride);
+    const float sign = (float)((a > 0) - (a < 0));
+    return a + (float)(-q) * fabsf(b - a) * sign;
+  }
+  if (q >= n) {
+    const long long edge = j + (long long)(n - 1 - pos) * stride;
+    const float a = v(edge), b = v(edge - stride);
+    const float sign = (float)((a > 0) - (a < 0));
+    return a + (float)(q - n + 1) * fabsf(a - b) * sign;
+  }
+  return v(j + (long long)off * stride);
+}
+
+// Left and right ENO slopes of the node j (value f) along one axis.
+template <typename V>
+__device__ __forceinline__ void w8_slopes(const V &v, long long j, float f, long long stride,
+                                          int pos, int n, float h, bool periodic, bool linear,
+                                          int order, float &left, float &right) {
+  if (linear && pos == 0) {
+    left = right = (v(j + stride) - f) / h;
+    return;
+  }
+  if (linear && pos == n - 1) {
+    left = right = (f - v(j - stride)) / h;
+    return;
+  }
+  const float fm = w8_at(v, j, stride, pos, n, -1, periodic),
+              fp = w8_at(v, j, stride, pos, n, 1, periodic);
+  float l = (f - fm) / h, r = (fp - f) / h;
+  if (order == 2) {
+    const float hh = h * h;
+    const float dc = (fp - 2 * f + fm) / hh;
+    const float dl =
+        linear && pos < 2 ? dc : (f - 2 * fm + w8_at(v, j, stride, pos, n, -2, periodic)) / hh;
+    const float dr = linear && pos + 2 >= n
+                         ? dc
+                         : (w8_at(v, j, stride, pos, n, 2, periodic) - 2 * fp + f) / hh;
+    l += .5f * h * (fabsf(dl) <= fabsf(dc) ? dl : dc);
+    r -= .5f * h * (fabsf(dc) <= fabsf(dr) ? dc : dr);
+  }
+  left = l;
+  right = r;
+}
+
+// Pursuer-side quantities of a slab (uniform over its nodes).
+struct W8Slab {
+  int q[4];
+  float palo, pahi, fpx, fpy, wp, avoid;
+};
+
+__device__ __forceinline__ W8Slab w8_slab(long long s, const W8 &g, const float *__restrict__ x,
+                                          const float *__restrict__ cs,
+                                          const float *__restrict__ sn,
+                                          const float *__restrict__ pv) {
+  W8Slab p;
+  p.q[3] = (int)(s % g.n[3]);
+  s /= g.n[3];
+  p.q[2] = (int)(s % g.n[2]);
+  s /= g.n[2];
+  p.q[1] = (int)(s % g.n[1]);
+  p.q[0] = (int)(s / g.n[1]);
+  const float vp = pv[p.q[3]];
+  p.palo = p.q[3] == 0 ? 0 : -g.ap;
+  p.pahi = p.q[3] == g.n[3] - 1 ? 0 : g.ap;
+  p.fpx = vp * cs[p.q[2]];
+  p.fpy = vp * sn[p.q[2]];
+  p.wp = fabsf(vp) * g.kp;
+  p.avoid = w8_mx(fabsf(x[p.q[0]]), fabsf(x[p.q[1]])) - g.L + 1e-6f;
+  return p;
+}
+
+// The update of node j = slab * inner + i (native_world.cpp:261-314): axes
+// 0-3 read `outer`, axes 4-7 read `inner_v` at index `ii` (i, or j when
+// both are the global field).
+template <typename VO, typename VI>
+__device__ __forceinline__ void w8_node(const VO &outer, const VI &inner_v, long long j,
+                                        long long ii, float center, const W8Slab &p, int i,
+                                        float *dst, const float *base,
+                                        float *__restrict__ arrival, const float *__restrict__ cs,
+                                        const float *__restrict__ sn,
+                                        const float *__restrict__ ev, const W8 &g, float dt,
+                                        double tau) {
+  int q[8];
+  q[0] = p.q[0];
+  q[1] = p.q[1];
+  q[2] = p.q[2];
+  q[3] = p.q[3];
+  if (g.fast_decode) {
+    const int r6 = (int)__umulhi((unsigned)i, g.magic[7]);
+    const int r5 = (int)__umulhi((unsigned)r6, g.magic[6]);
+    q[4] = (int)__umulhi((unsigned)r5, g.magic[5]);
+    q[7] = i - r6 * g.n[7];
+    q[6] = r6 - r5 * g.n[6];
+    q[5] = r5 - q[4] * g.n[5];
+  } else {
+    int r = i;
+    q[7] = r % g.n[7];
+    r /= g.n[7];
+    q[6] = r % g.n[6];
+    r /= g.n[6];
+    q[5] = r % g.n[5];
+    q[4] = r / g.n[5];
+  }
+  const int eh = q[6], iv = q[7];
+  const float ve = ev[iv], we = fabsf(ve) * g.ke, ealo = iv == 0 ? 0 : -g.ae,
+              eahi = iv == g.n[7] - 1 ? 0 : g.ae;
+  const float f[8] = {p.fpx, p.fpy, 0, 0, ve * cs[eh], ve * sn[eh], 0, 0};
+  float gk[8], H = 0, diss = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    float lm, rp;
+    if (k < 4)
+      w8_slopes(outer, j, center, g.stride[k], q[k], g.n[k], g.h[k], k == 2,
+                g.linear && k == 3, g.order, lm, rp);
+    else
+      w8_slopes(inner_v, ii, center, g.stride[k], q[k], g.n[k], g.h[k], k == 6,
+                g.linear && k == 7, g.order, lm, rp);
+    gk[k] = .5f * (lm + rp);
+    const float alpha = k == 2 ? p.wp : k == 3 ? g.ap : k == 6 ? we : k == 7 ? g.ae : fabsf(f[k]);
+    H += f[k] * gk[k];
+    diss += .5f * alpha * (rp - lm);
+  }
+  H += -p.wp * fabsf(gk[2]) + we * fabsf(gk[6]) + w8_mn(p.palo * gk[3], p.pahi * gk[3]) +
+       w8_mx(ealo * gk[7], eahi * gk[7]);
+  float value = center + dt * (H + diss);
+  if (base) value = .5f * (base[j] + value);
+  if (g.order == 1 || base) {
+    const float previous = base ? base[j] : center;
+    value = w8_mx(w8_mn(value, previous), p.avoid);
+    if (previous > 0 && value <= 0 && isnan(arrival[j]))
+      arrival[j] = (float)tau + dt * previous / (previous - value);
+  }
+  dst[j] = value;
+}
+
+// One RK stage src -> dst, one thread per node; base = the previous field in
+// the blend stage (may alias dst: each node reads base[j] before writing
+// dst[j]).  Blocks cover 256 nodes of one slab.
+__global__ __launch_bounds__(256) void world8d_stage(
+    const float *__restrict__ src, float *dst, const float *base, float *__restrict__ arrival,
+    const float *__restrict__ x, const float *__restrict__ cs, const float *__restrict__ sn,
+    const float *__restrict__ pv, const float *__restrict__ ev, W8 g, float dt, double tau) {
+  const long long b = blockIdx.x;
+  const long long slab = b / g.inner_blocks;  // (xP, yP, thP, vP), uniform
+  const int i = (int)(b - slab * g.inner_blocks) * 256 + (int)threadIdx.x;
+  if (i >= g.inner) return;
+  const W8Slab p = w8_slab(slab, g, x, cs, sn, pv);
+  const long long j = slab * g.inner + i;
+  const W8Global v{src};
+  w8_node(v, v, j, j, src[j], p, i, dst, base, arrival, cs, sn, ev, g, dt, tau);
+}
+
+// The same stage with one block per slab: the slab's inner (xE, yE, thE, vE)
+// values are staged in LDS, so the evader-axis neighbours are LDS reads and
+// only the pursuer-axis neighbours (block-uniform boundary cases) are global
+// loads.
+template <int THREADS>
+__global__ __launch_bounds__(THREADS) void world8d_stage_slab(
+    const float *__restrict__ src, float *dst, const float *base, float *__restrict__ arrival,
+    const float *__restrict__ x, const float *__restrict__ cs, const float *__restrict__ sn,
+    const float *__restrict__ pv, const float *__restrict__ ev, W8 g, float dt, double tau) {
+  extern __shared__ float slab_values[];
+  const long long slab = blockIdx.x;
+  const float *__restrict__ row = src + slab * g.inner;
+  for (int i = threadIdx.x; i < g.inner; i += THREADS) slab_values[i] = row[i];
+  __syncthreads();
+  const W8Slab p = w8_slab(slab, g, x, cs, sn, pv);
+  const W8Global outer{src};
+  const W8Lds inner_v{slab_values};
+#pragma unroll 1
+  for (int i = threadIdx.x; i < g.inner; i += THREADS)
+    w8_node(outer, inner_v, slab * g.inner + i, i, slab_values[i], p, i, dst, base, arrival,
+            cs, sn, ev, g, dt, tau);
+}
+
+// Count of non-finite values (grid-stride; one atomic per non-zero wave).
+__global__ __launch_bounds__(256) void world8d_nonfinite(const float *__restrict__ v, long long n,
+                                                         unsigned long long *__restrict__ count) {
+  unsigned long long bad = 0;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    bad += !isfinite(v[i]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) bad += __shfl_xor(bad, off, 64);
+  if ((threadIdx.x & 63) == 0 && bad) atomicAdd(count, bad);
+}
+
+// Probe values: one thread per probe, the 256 products summed in order in
+// float64 (native_world.cpp:241-248).
+__global__ __launch_bounds__(256) void world8d_probe(const float *__restrict__ v,
+                                                     const unsigned long long *__restrict__ index,
+                                                     const double *__restrict__ weight,
+                                                     long long count, double *__restrict__ out) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= count) return;
+  double sum = 0;
+  for (int k = 0; k < 256; k++) sum += (double)v[index[q * 256 + k]] * weight[q * 256 + k];
+  out[q] = sum;
+}
+
+}  // namespace pe
+
+using namespace pe;
+
+struct pe_world8d {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  pe_world8d_config cfg{};
+  W8 g{};
+  float *a = nullptr, *b = nullptr, *arrival = nullptr;
+  float *tables = nullptr;  // x | cs | sn | pv | ev
+  unsigned long long *count = nullptr;   // non-finite counter
+  unsigned long long *probe_index = nullptr;
+  double *probe_weight = nullptr, *probe_out = nullptr;
+  long long probes = 0;
+  int variant = 0;  // 0: thread per node; 1, 2: LDS slab kernel, 256 / 512 threads
+  double maxdt = 0, tau = 0, update_seconds = 0;
+  long long steps = 0;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  std::vector<float> x, cs, sn, pv, ev;
+};
+
+// Host tables and grid, in the reference's float / double operations
+// (native_world.cpp:114-153).
+static void world8d_setup(pe_world8d *h) {
+  const pe_world8d_config &c = h->cfg;
+  const int n = c.positions, nh = c.headings, np = c.p_speeds, ne = c.e_speeds;
+  const float extent = (float)c.extent, pmin = (float)c.p_min, pmax = (float)c.p_max,
+              emin = (float)c.e_min, emax = (float)c.e_max;
+  const double pi = std::acos(-1.);
+  const float dx = 2 * extent / (n - 1), dh = float(2 * pi / nh), dvp = (pmax - pmin) / (np - 1),
+              dve = (emax - emin) / (ne - 1);
+  W8 &g = h->g;
+  const int shape[8] = {n, n, nh, np, n, n, nh, ne};
+  const float spacing[8] = {dx, dx, dh, dvp, dx, dx, dh, dve};
+  g.stride[7] = 1;
+  for (int k = 6; k >= 0; k--) g.stride[k] = g.stride[k + 1] * shape[k + 1];
+  for (int k = 0; k < 8; k++) {
+    g.n[k] = shape[k];
+    g.h[k] = spacing[k];
+  }
+  g.N = g.stride[0] * n;
+  g.inner = n * n * nh * ne;
+  g.inner_blocks = (g.inner + 255) / 256;
+  // floor(2^32 / d) + 1 = 2^32 / d + e with 0 < e <= 1, so
+  // floor(r * magic / 2^32) = floor(r / d + r e / 2^32) = floor(r / d)
+  // whenever r d < 2^32 (the fractional part of r / d is <= 1 - 1 / d).
+  g.fast_decode = 1;
+  for (int k = 0; k < 8; k++) {
+    g.magic[k] = (unsigned)((1ull << 32) / (unsigned)shape[k] + 1);
+    if (k >= 5 && (unsigned long long)g.inner * (unsigned)shape[k] >= (1ull << 32))
+      g.fast_decode = 0;
+  }
+  g.order = c.order;
+  g.linear = c.linear_speed_boundary;
+  g.L = (float)c.arena;
+  g.ap = (float)c.p_accel;
+  g.ae = (float)c.e_accel;
+  g.kp = (float)c.p_curvature;
+  g.ke = (float)c.e_curvature;
+  h->x.resize(n);
+  h->cs.resize(nh);
+  h->sn.resize(nh);
+  h->pv.resize(np);
+  h->ev.resize(ne);
+  for (int i = 0; i < n; i++) h->x[i] = -extent + i * dx;
+  for (int i = 0; i < nh; i++) {
+    h->cs[i] = (float)std::cos(-pi + i * 2 * pi / nh);
+    h->sn[i] = (float)std::sin(-pi + i * 2 * pi / nh);
+  }
+  for (int i = 0; i < np; i++) h->pv[i] = pmin + i * dvp;
+  for (int i = 0; i < ne; i++) h->ev[i] = emin + i * dve;
+  const float pabs = std::max(std::fabs(pmin), std::fabs(pmax)),
+              eabs = std::max(std::fabs(emin), std::fabs(emax));
+  h->maxdt = c.cfl / (2 * double(pabs + eabs) / dx + double(pabs * g.kp + eabs * g.ke) / dh +
+                      double(g.ap) / dvp + double(g.ae) / dve);
+}
+
+// Initial field max(target, avoid) on the host (native_world.cpp:154-178).
+static void world8d_initial(const pe_world8d *h, std::vector<float> &field,
+                            std::vector<float> &arrival) {
+  const W8 &g = h->g;
+  const float L = (float)h->cfg.arena, radius = (float)h->cfg.radius;
+  field.resize(g.N);
+  arrival.resize(g.N);
+  const long long inner = (long long)g.n[6] * g.n[7];
+  const float *x = h->x.data();
+  for (long long block = 0; block < g.N / inner; block++) {
+    const long long begin = block * inner;
+    int q[8];
+    for (int k = 0; k < 8; k++) q[k] = (int)((begin / g.stride[k]) % g.n[k]);
+    const float t1 = std::hypot(x[q[0]] - x[q[4]], x[q[1]] - x[q[5]]) - radius;
+    const float t2 = L - std::max(std::fabs(x[q[4]]), std::fabs(x[q[5]]));
+    const float target = t2 < t1 ? t2 : t1;
+    const float fa = std::fabs(x[q[0]]), fb = std::fabs(x[q[1]]);
+    const float avoid = (fa < fb ? fb : fa) - L + 1e-6f;
+    const float initial = target < avoid ? avoid : target;
+    for (long long k = 0; k < inner; k++) {
+      field[begin + k] = initial;
+      arrival[begin + k] = initial <= 0 ? 0.f : std::numeric_limits<float>::quiet_NaN();
+    }
+  }
+}
+
+extern "C" int pe_world8d_create(const pe_world8d_config *cfg, pe_world8d **out) {
+  clear_error();
+  if (!cfg || !out) return fail(PE_ERR_INVALID, "null argument");
+  *out = nullptr;
+  const pe_world8d_config &c = *cfg;
+  // native_world.cpp:104-111
+  if (c.positions < 5 || c.headings < 4 || c.p_speeds < 3 || c.e_speeds < 3 ||
+      !(c.extent > c.arena) || c.order < 1 || c.order > 2 || !(c.cfl > 0) || c.cfl > 1 ||
+      !(c.p_min < c.p_max) || !(c.e_min < c.e_max) || !(c.p_accel > 0) || !(c.e_accel > 0) ||
+      !(c.p_curvature > 0) || !(c.e_curvature > 0) || !(c.radius >= 0) ||
+      (c.linear_speed_boundary != 0 && c.linear_speed_boundary != 1) ||
+      (c.flags & ~(PE_WORLD8D_GENERIC | PE_WORLD8D_WIDE_SLAB)) != 0)
+    return fail(PE_ERR_INVALID, "Invalid arguments");
+  const double nodes = std::pow((double)c.positions, 4) * c.headings * c.headings *
+                       c.p_speeds * c.e_speeds;
+  if (nodes > 4.0e9) return fail(PE_ERR_INVALID, "grid too large");
+  int dev = 0;
+  int rc = require_device(&dev);
+  if (rc) return rc;
+  pe_world8d *h = new pe_world8d();
+  h->device = dev;
+  h->cfg = c;
+  world8d_setup(h);
+  // The slab kernel needs the slab in LDS; 64 KiB keeps two blocks per CU.
+  if (!(c.flags & PE_WORLD8D_GENERIC) && sizeof(float) * (size_t)h->g.inner <= 65536)
+    h->variant = (c.flags & PE_WORLD8D_WIDE_SLAB) ? 2 : 1;
+  auto body = [&]() -> int {
+    PE_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    PE_HIP(hipEventCreate(&h->t0));
+    PE_HIP(hipEventCreate(&h->t1));
+    const size_t bytes = sizeof(float) * (size_t)h->g.N;
+    if (hipMalloc(&h->a, bytes) != hipSuccess || hipMalloc(&h->b, bytes) != hipSuccess ||
+        hipMalloc(&h->arrival, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(PE_ERR_MEMORY, "cannot allocate the 8D fields");
+    }
+    std::vector<float> tab;
+    for (auto *v : {&h->x, &h->cs, &h->sn, &h->pv, &h->ev}) tab.insert(tab.end(), v->begin(), v->end());
+    PE_HIP(hipMalloc(&h->tables, sizeof(float) * tab.size()));
+    PE_HIP(hipMemcpy(h->tables, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice));
+    PE_HIP(hipMalloc(&h->count, 8));
+    std::vector<float> field, arrival;
+    world8d_initial(h, field, arrival);
+    PE_HIP(hipMemcpy(h->a, field.data(), bytes, hipMemcpyHostToDevice));
+    PE_HIP(hipMemcpy(h->arrival, arrival.data(), bytes, hipMemcpyHostToDevice));
+    return PE_OK;
+  };
+  rc = body();
+  if (rc) {
+    pe_world8d_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return PE_OK;
+}
+
+extern "C" int pe_world8d_set_field(pe_world8d *h, const float *field, const float *arrival,
+                                    double time) {
+  clear_error();
+  if (!h || !field || !(time >= 0)) return fail(PE_ERR_INVALID, "invalid argument");
+  PE_HIP(hipSetDevice(h->device));
+  const long long N = h->g.N;
+  PE_HIP(hipStreamSynchronize(h->stream));
+  const size_t bytes = sizeof(float) * (size_t)N;
+  PE_HIP(hipMemcpy(h->a, field, bytes, hipMemcpyHostToDevice));
+  if (arrival) {
+    PE_HIP(hipMemcpy(h->arrival, arrival, bytes, hipMemcpyHostToDevice));
+  } else {
+    // Captured nodes of a continuation arrive at its start (native_world.cpp:185-189).
+    std::vector<float> arr((size_t)N);
+    for (long long k = 0; k < N; k++)
+      arr[k] = field[k] <= 0 ? (float)time : std::numeric_limits<float>::quiet_NaN();
+    PE_HIP(hipMemcpy(h->arrival, arr.data(), bytes, hipMemcpyHostToDevice));
+  }
+  h->tau = time;
+  return PE_OK;
+}
+
+// One RK1 or RK2 step of size dt from the current time.
+static int world8d_launch(pe_world8d *h, const float *src, float *dst, const float *base,
+                          double dt) {
+  const W8 &g = h->g;
+  const float *x = h->tables, *cs = x + g.n[0], *sn = cs + g.n[2], *pv = sn + g.n[2],
+              *ev = pv + g.n[3];
+  const unsigned slabs = (unsigned)(g.N / g.inner);
+  const size_t lds = sizeof(float) * (size_t)g.inner;
+  if (h->variant == 0) {
+    hipLaunchKernelGGL(world8d_stage, dim3(slabs * g.inner_blocks), dim3(256), 0, h->stream,
+                       src, dst, base, h->arrival, x, cs, sn, pv, ev, g, (float)dt, h->tau);
+    PE_LAUNCHED(world8d_stage);
+  } else if (h->variant == 1) {
+    hipLaunchKernelGGL(world8d_stage_slab<256>, dim3(slabs), dim3(256), lds, h->stream, src,
+                       dst, base, h->arrival, x, cs, sn, pv, ev, g, (float)dt, h->tau);
+    PE_LAUNCHED(world8d_stage_slab);
+  } else {
+    hipLaunchKernelGGL(world8d_stage_slab<512>, dim3(slabs), dim3(512), lds, h->stream, src,
+                       dst, base, h->arrival, x, cs, sn, pv, ev, g, (float)dt, h->tau);
+    PE_LAUNCHED(world8d_stage_slab);
+  }
+  return PE_OK;
+}
+
+static int world8d_step(pe_world8d *h, double dt) {
+  int rc;
+  if (h->cfg.order == 1) {
+    if ((rc = world8d_launch(h, h->a, h->b, nullptr, dt))) return rc;
+    std::swap(h->a, h->b);
+  } else {
+    if ((rc = world8d_launch(h, h->a, h->b, nullptr, dt))) return rc;
+    if ((rc = world8d_launch(h, h->b, h->a, h->a, dt))) return rc;
+  }
+  h->tau += dt;
+  h->steps++;
+  return PE_OK;
+}
+
+extern "C" int pe_world8d_advance_to(pe_world8d *h, double goal, int64_t max_steps) {
+  clear_error();
+  if (!h || !std::isfinite(goal)) return fail(PE_ERR_INVALID, "invalid argument");
+  PE_HIP(hipSetDevice(h->device));
+  PE_HIP(hipEventRecord(h->t0, h->stream));
+  int64_t taken = 0;
+  // native_world.cpp:324-329
+  while (h->tau < goal - 1e-10 && (max_steps <= 0 || taken < max_steps)) {
+    const double dt = std::min(h->maxdt, goal - h->tau);
+    int rc = world8d_step(h, dt);
+    if (rc) return rc;
+    taken++;
+  }
+  PE_HIP(hipEventRecord(h->t1, h->stream));
+  PE_HIP(hipEventSynchronize(h->t1));
+  float ms = 0;
+  PE_HIP(hipEventElapsedTime(&ms, h->t0, h->t1));
+  h->update_seconds += ms * 1e-3;
+  return PE_OK;
+}
+
+extern "C" int pe_world8d_count_nonfinite(pe_world8d *h, int64_t *count) {
+  clear_error();
+  if (!h || !count) return fail(PE_ERR_INVALID, "invalid argument");
+  PE_HIP(hipSetDevice(h->device));
+  PE_HIP(hipMemsetAsync(h->count, 0, 8, h->stream));
+  hipLaunchKernelGGL(world8d_nonfinite, dim3(2048), dim3(256), 0, h->stream, h->a, h->g.N,
+                     h->count);
+  PE_LAUNCHED(world8d_nonfinite);
+  unsigned long long bad = 0;
+  PE_HIP(hipMemcpyAsync(&bad, h->count, 8, hipMemcpyDeviceToHost, h->stream));
+  PE_HIP(hipStreamSynchronize(h->stream));
+  *count = (int64_t)bad;
+  return PE_OK;
+}
+
+extern "C" int pe_world8d_set_probes(pe_world8d *h, const uint64_t *indices,
+                                     const double *weights, int64_t count) {
+  clear_error();
+  if (!h || count < 0 || (count > 0 && (!indices || !weights)))
+    return fail(PE_ERR_INVALID, "invalid argument");
+  // native_world.cpp:215-226
+  for (int64_t q = 0; q < count; q++) {
+    double sum = 0;
+    for (int k = 0; k < 256; k++) {
+      const int64_t j = q * 256 + k;
+      if (indices[j] >= (uint64_t)h->g.N || !std::isfinite(weights[j]) || weights[j] < 0)
+        return fail(PE_ERR_INVALID, "Bad probe stencil");
+      sum += weights[j];
+    }
+    if (std::fabs(sum - 1) > 1e-8) return fail(PE_ERR_INVALID, "Bad probe weights");
+  }
+  PE_HIP(hipSetDevice(h->device));
+  PE_HIP(hipStreamSynchronize(h->stream));
+  for (void *p : {(void *)h->probe_index, (void *)h->probe_weight, (void *)h->probe_out})
+    if (p) PE_HIP(hipFree(p));
+  h->probe_index = nullptr;
+  h->probe_weight = h->probe_out = nullptr;
+  h->probes = 0;
+  if (count == 0) return PE_OK;
+  const size_t terms = (size_t)count * 256;
+  PE_HIP(hipMalloc(&h->probe_index, terms * 8));
+  PE_HIP(hipMalloc(&h->probe_weight, terms * 8));
+  PE_HIP(hipMalloc(&h->probe_out, (size_t)count * 8));
+  PE_HIP(hipMemcpy(h->probe_index, indices, terms * 8, hipMemcpyHostToDevice));
+  PE_HIP(hipMemcpy(h->probe_weight, weights, terms * 8, hipMemcpyHostToDevice));
+  h->probes = count;
+  return PE_OK;
+}
+
+extern "C" int pe_world8d_probe(pe_world8d *h, double *values) {
+  clear_error();
+  if (!h || (h->probes && !values)) return fail(PE_ERR_INVALID, "invalid argument");
+  if (!h->probes) return PE_OK;
+  PE_HIP(hipSetDevice(h->device));
+  hipLaunchKernelGGL(world8d_probe, dim3((unsigned)((h->probes + 255) / 256)), dim3(256), 0,
+                     h->stream, h->a, h->probe_index, h->probe_weight, h->probes, h->probe_out);
+  PE_LAUNCHED(world8d_probe);
+  PE_HIP(hipMemcpyAsync(values, h->probe_out, (size_t)h->probes * 8, hipMemcpyDeviceToHost,
+                        h->stream));
+  PE_HIP(hipStreamSynchronize(h->stream));
+  return PE_OK;
+}
+
+extern "C" int pe_world8d_read(pe_world8d *h, float *field, float *arrival) {
+  clear_error();
+  if (!h) return fail(PE_ERR_INVALID, "null handle");
+  PE_HIP(hipSetDevice(h->device));
+  PE_HIP(hipStreamSynchronize(h->stream));
+  const size_t bytes = sizeof(float) * (size_t)h->g.N;
+  if (field) PE_HIP(hipMemcpy(field, h->a, bytes, hipMemcpyDeviceToHost));
+  if (arrival) PE_HIP(hipMemcpy(arrival, h->arrival, bytes, hipMemcpyDeviceToHost));
+  return PE_OK;
+}
+
+extern "C" int pe_world8d_get_stats(pe_world8d *h, pe_world8d_stats *s) {
+  clear_error();
+  if (!h || !s) return fail(PE_ERR_INVALID, "invalid argument");
+  std::memset(s, 0, sizeof(*s));
+  s->states = h->g.N;
+  s->internal_steps = h->steps;
+  s->time = h->tau;
+  s->max_internal_dt = h->maxdt;
+  s->update_seconds = h->update_seconds;
+  s->probes = h->probes;
+  s->kernel = h->variant;
+  return PE_OK;
+}
+
+extern "C" void pe_world8d_destroy(pe_world8d *h) {
+  if (!h) return;
+  hipSetDevice(h->device);
+  if (h->stream) hipStreamSynchronize(h->stream);
+  for (void *p : {(void *)h->a, (void *)h->b, (void *)h->arrival, (void *)h->tables,
+                  (void *)h->count, (void *)h->probe_index, (void *)h->probe_weight,
+                  (void *)h->probe_out})
+    if (p) hipFree(p);
+  if (h->t0) hipEventDestroy(h->t0);
+  if (h->t1) hipEventDestroy(h->t1);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+}

@@ -182,6 +182,48 @@ PE_SL_ORDERED = 2
 PE_SL_GRAPH = 4
 PE_SL_GLOBAL_GATHER = 8
 
+class World8DConfig(ctypes.Structure):
+    _fields_ = [
+        ("positions", ctypes.c_int32),
+        ("headings", ctypes.c_int32),
+        ("p_speeds", ctypes.c_int32),
+        ("e_speeds", ctypes.c_int32),
+        ("order", ctypes.c_int32),
+        ("linear_speed_boundary", ctypes.c_int32),
+        ("flags", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("extent", ctypes.c_double),
+        ("arena", ctypes.c_double),
+        ("radius", ctypes.c_double),
+        ("p_min", ctypes.c_double),
+        ("p_max", ctypes.c_double),
+        ("e_min", ctypes.c_double),
+        ("e_max", ctypes.c_double),
+        ("p_accel", ctypes.c_double),
+        ("e_accel", ctypes.c_double),
+        ("p_curvature", ctypes.c_double),
+        ("e_curvature", ctypes.c_double),
+        ("cfl", ctypes.c_double),
+    ]
+
+
+class World8DStats(ctypes.Structure):
+    _fields_ = [
+        ("states", ctypes.c_int64),
+        ("internal_steps", ctypes.c_int64),
+        ("time", ctypes.c_double),
+        ("max_internal_dt", ctypes.c_double),
+        ("update_seconds", ctypes.c_double),
+        ("probes", ctypes.c_int64),
+        ("kernel", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+PE_WORLD8D_GENERIC = 1
+PE_WORLD8D_WIDE_SLAB = 2
+
+
 P = ctypes.c_void_p
 I32 = ctypes.c_int32
 I64 = ctypes.c_int64
@@ -237,6 +279,15 @@ SIGNATURES = {
     "pe_mlp_train": (ctypes.c_int, [P, I32, I32, ctypes.c_uint32, P]),
     "pe_mlp_gemm_check": (ctypes.c_int, [P, P, I32, I32, I32, I32, P]),
     "pe_mlp_destroy": (None, [P]),
+    "pe_world8d_create": (ctypes.c_int, [ctypes.POINTER(World8DConfig), P]),
+    "pe_world8d_set_field": (ctypes.c_int, [P, P, P, F64]),
+    "pe_world8d_advance_to": (ctypes.c_int, [P, F64, I64]),
+    "pe_world8d_read": (ctypes.c_int, [P, P, P]),
+    "pe_world8d_count_nonfinite": (ctypes.c_int, [P, P]),
+    "pe_world8d_set_probes": (ctypes.c_int, [P, P, P, I64]),
+    "pe_world8d_probe": (ctypes.c_int, [P, P]),
+    "pe_world8d_get_stats": (ctypes.c_int, [P, ctypes.POINTER(World8DStats)]),
+    "pe_world8d_destroy": (None, [P]),
 }
 
 
@@ -247,6 +298,8 @@ STRUCTS = {
     "pe_sl_game": SLGame,
     "pe_sl_stats": SLStats,
     "pe_mlp_config": MLPConfig,
+    "pe_world8d_config": World8DConfig,
+    "pe_world8d_stats": World8DStats,
 }
 
 
@@ -499,6 +552,77 @@ class History:
     def close(self):
         if self._h:
             lib().pe_history_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class World8D:
+    """Owning wrapper of a pe_world8d handle: the dense 8D square game,
+    fields resident on the device."""
+
+    def __init__(self, cfg: World8DConfig):
+        self._h = ctypes.c_void_p()
+        check(lib().pe_world8d_create(ctypes.byref(cfg), ctypes.byref(self._h)))
+        self.cfg = cfg
+        n, h = cfg.positions, cfg.headings
+        self.n = n ** 4 * h * h * cfg.p_speeds * cfg.e_speeds
+        self.probe_count = 0
+
+    def set_field(self, field, arrival=None, time=0.0):
+        f = np.ascontiguousarray(field, np.float32).reshape(-1)
+        a = None if arrival is None else np.ascontiguousarray(arrival, np.float32).reshape(-1)
+        if f.size != self.n or (a is not None and a.size != self.n):
+            raise ValueError("field size mismatch")
+        check(lib().pe_world8d_set_field(self._h, ptr(f), ptr(a), float(time)))
+
+    def advance_to(self, goal: float, max_steps: int = 0):
+        check(lib().pe_world8d_advance_to(self._h, float(goal), int(max_steps)))
+
+    def read(self, field=True, arrival=True):
+        f = np.empty(self.n, np.float32) if field is True else field
+        a = np.empty(self.n, np.float32) if arrival is True else arrival
+        f = None if f is False else f
+        a = None if a is False else a
+        check(lib().pe_world8d_read(self._h, ptr(f), ptr(a)))
+        return f, a
+
+    def count_nonfinite(self) -> int:
+        c = ctypes.c_int64()
+        check(lib().pe_world8d_count_nonfinite(self._h, ctypes.byref(c)))
+        return c.value
+
+    def set_probes(self, indices, weights):
+        ii = np.ascontiguousarray(indices, np.uint64).reshape(-1, 256)
+        ww = np.ascontiguousarray(weights, np.float64).reshape(-1, 256)
+        if ii.shape != ww.shape:
+            raise ValueError("probe indices and weights must match")
+        check(lib().pe_world8d_set_probes(self._h, ptr(ii), ptr(ww), len(ii)))
+        self.probe_count = len(ii)
+
+    def probe(self):
+        out = np.empty(self.probe_count, np.float64)
+        check(lib().pe_world8d_probe(self._h, ptr(out)))
+        return out
+
+    def stats(self) -> dict:
+        s = World8DStats()
+        check(lib().pe_world8d_get_stats(self._h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in World8DStats._fields_}
+
+    def close(self):
+        if self._h:
+            lib().pe_world8d_destroy(self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):

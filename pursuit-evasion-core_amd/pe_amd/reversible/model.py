@@ -106,6 +106,20 @@ def _acceleration(v, a, limits):
     return out
 
 
+def world_rhs(states, p_controls, e_controls, config):
+    """Dynamics of the joint state [xP, yP, thP, vP, xE, yE, thE, vE] under
+    commands [acceleration, curvature] (reversible/model.py:194-206)."""
+    s = np.asarray(states, float)
+    p = np.asarray(p_controls, float)
+    e = np.asarray(e_controls, float)
+    parts = []
+    for base, u, limits in ((0, p, config.pursuer), (4, e, config.evader)):
+        heading, v = s[..., base + 2], s[..., base + 3]
+        parts += [v * np.cos(heading), v * np.sin(heading), v * u[..., 1],
+                  _acceleration(v, u[..., 0], limits)]
+    return np.stack(np.broadcast_arrays(*parts), axis=-1)
+
+
 def relative_rhs(states, p_controls, e_controls, config):
     """Relative dynamics of [x, y, psi, vP, vE] (reversible/model.py:209-220)."""
     r = np.asarray(states, float)

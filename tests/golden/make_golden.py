@@ -15,10 +15,10 @@ compiled unmodified by oracle/ref.mk) on small dense 8D grids with the
 default ReversibleConfig (arena 3 m, extent 3.5 m), RK2 with the linear
 speed boundary and RK1 with the legacy signed one; and stores the output
 files (fields, final, arrival, timing numbers) as compressed .npz fixtures
-next to this script.  Needs /root/reference, so it
-only runs in the build container:
+next to this script (the first world case also with 24 probe stencils).
+Needs /root/reference, so it only runs in the build container:
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [file-name prefix]
 """
 
 from __future__ import annotations
@@ -97,7 +97,8 @@ WORLD = dict(p_min=-1.0, p_max=2.0, e_min=-0.5, e_max=1.0, p_accel=2.0, e_accel=
 WORLD_CASES = {
     "world8d_5x4x3x3_order2_linear": dict(positions=5, headings=4, p_speeds=3, e_speeds=3,
                                           horizon=0.4, output_dt=0.1, order=2, cfl=0.8,
-                                          linear_speed_boundary=1, history=1),
+                                          linear_speed_boundary=1, history=1,
+                                          probes=24),
     "world8d_5x4x3x3_order1_legacy": dict(positions=5, headings=4, p_speeds=3, e_speeds=3,
                                           horizon=0.4, output_dt=0.1, order=1, cfl=0.8,
                                           linear_speed_boundary=0, history=1),
@@ -107,13 +108,48 @@ WORLD_CASES = {
 }
 
 
+def probe_states(count: int) -> np.ndarray:
+    """Probe states: the two of tests/test_reversible_world_dp.py:127-129,
+    then seeded uniform states over the numerical domain (5D speeds inside
+    the default limits)."""
+    rng = np.random.default_rng(2024)
+    fixed = np.array([[-1, 0, 0, 0.5, 0, 1, np.pi, 0.25], [0, 0, 0, 0.5, 0.1, 0.1, 0, 0.25]])
+    lo = np.array([-3.5, -3.5, -4.0, -1.0, -3.5, -3.5, -4.0, -0.5])
+    hi = np.array([3.5, 3.5, 4.0, 2.0, 3.5, 3.5, 4.0, 1.0])
+    return np.concatenate([fixed, rng.uniform(lo, hi, size=(count - 2, 8))])
+
+
 def run_world(binary: Path, case: dict, out: Path) -> dict:
+    case = dict(case)
+    probes = case.pop("probes", 0)
     flags = dict(WORLD)
     flags.update(case)
     argv = [str(binary)]
     for k, v in flags.items():
         argv += ["--" + k.replace("_", "-"), repr(v)]
     argv += ["--threads", "4", "--output", str(out)]
+    extra = {}
+    if probes:
+        # Stencils from the package's own interpolation_stencil (inputs of
+        # the executable; its summation is what these vectors pin).
+        sys.path.insert(0, str(ROOT / "pursuit-evasion-core_amd"))
+        from pe_amd.reversible.model import ReversibleConfig
+        from pe_amd.reversible.world_dp import PERIODIC_AXES, WorldGrid
+        from pe_amd.reversible.world_table import interpolation_stencil
+
+        grid = WorldGrid(case["positions"], case["headings"], case["p_speeds"],
+                         case["e_speeds"], WORLD["extent"])
+        states = probe_states(probes)
+        ii, ww, valid, _ = interpolation_stencil(states, grid.axes(ReversibleConfig()),
+                                                 PERIODIC_AXES)
+        assert valid.all()
+        ii.astype(np.uint64).tofile(out.parent / "probe_indices.u64")
+        ww.astype(np.float64).tofile(out.parent / "probe_weights.f64")
+        argv += ["--probe-count", str(probes),
+                 "--probe-indices", str(out.parent / "probe_indices.u64"),
+                 "--probe-weights", str(out.parent / "probe_weights.f64")]
+        extra = dict(probe_states=states, probe_index=ii.astype(np.uint64),
+                     probe_weight=ww.astype(np.float64))
     subprocess.run(argv, check=True, stdout=subprocess.DEVNULL)
     n = case["positions"] ** 4 * case["headings"] ** 2 * case["p_speeds"] * case["e_speeds"]
     timing = json.loads((out / "native_timing.json").read_text())
@@ -126,14 +162,22 @@ def run_world(binary: Path, case: dict, out: Path) -> dict:
     )
     if case["history"]:
         data["fields"] = np.fromfile(out / "fields.f32", np.float32).reshape(-1, n)
+    if probes:
+        data.update(extra)
+        data["probe_fields"] = np.fromfile(out / "probe_fields.f64",
+                                           np.float64).reshape(-1, probes)
     return data
 
 
 def main() -> int:
+    """Writes every fixture, or those whose file name starts with argv[1]."""
+    only = sys.argv[1] if len(sys.argv) > 1 else ""
     subprocess.run(["make", "-f", str(ROOT / "oracle" / "ref.mk")], check=True,
                    stdout=subprocess.DEVNULL)
     binary = ROOT / "oracle" / "_ref" / "relative5d"
     for name, case in CASES.items():
+        if not f"hji5d_{name}".startswith(only):
+            continue
         with tempfile.TemporaryDirectory() as tmp:
             out = Path(tmp) / "out"
             out.mkdir()
@@ -146,6 +190,8 @@ def main() -> int:
         print(name, data["internal_steps"], "steps", data["frames"], "frames")
     world = ROOT / "oracle" / "_ref" / "world8d"
     for name, case in WORLD_CASES.items():
+        if not name.startswith(only):
+            continue
         with tempfile.TemporaryDirectory() as tmp:
             out = Path(tmp) / "out"
             out.mkdir()

@@ -1,0 +1,151 @@
+"""Host logic of the dense 8D square game (pe_amd.reversible.world_dp and
+world_table), no GPU: the stencil, the Hamiltonian, option validation and
+the resource guards (tests/test_reversible_world_dp.py)."""
+
+from itertools import product
+
+import numpy as np
+import pytest
+
+from pe_amd.reversible.model import ReversibleConfig, world_rhs
+from pe_amd.reversible.world_dp import (
+    WorldGrid,
+    global_rate_bound,
+    hamiltonian,
+    output_times,
+    solve,
+)
+from pe_amd.reversible.world_table import interpolation_stencil
+
+
+def test_hamiltonian_matches_enumerated_control_pairs_at_signed_and_boundary_speeds():
+    """min over P, max over E of grad . f over the extreme commands equals the
+    closed form (tests/test_reversible_world_dp.py:18-39)."""
+    config = ReversibleConfig()
+    rng = np.random.default_rng(77)
+    p_actions = np.array(list(product([-2, 2], [-1, 1])))
+    e_actions = np.array(list(product([-1, 1], [-2, 2])))
+    for vp, ve in product([-1, -0.4, 0, 0.7, 2], [-0.5, -0.1, 0, 0.4, 1]):
+        state = np.array([0.1, 0.2, rng.uniform(-np.pi, np.pi), vp, 0.8, -0.4,
+                          rng.uniform(-np.pi, np.pi), ve])
+        gradient = rng.normal(size=8)
+        field = world_rhs(state, p_actions[:, None, :], e_actions[None, :, :], config)
+        exact = np.min(np.max(np.sum(field * gradient, axis=-1), axis=1))
+        assert hamiltonian(state, gradient, config) == pytest.approx(exact, abs=1e-12)
+
+
+def test_stencil_periodic_wrap_and_multilinear_exactness():
+    """(tests/test_reversible_world_dp.py:170-190)"""
+    config = ReversibleConfig()
+    grid = WorldGrid(5, 8, 3, 3)
+    s = np.array([[0.2, -0.3, np.pi, 0.1, 0.7, 0.8, -np.pi, 0.2],
+                  [0.2, -0.3, -np.pi, 0.1, 0.7, 0.8, np.pi, 0.2]])
+    ii, ww, valid, shape = interpolation_stencil(s, grid.axes(config), (2, 6))
+    np.testing.assert_array_equal(ii[0], ii[1])
+    np.testing.assert_allclose(ww.sum(1), 1)
+    assert valid.all() and shape == (2,)
+    fields = np.broadcast_arrays(*np.meshgrid(*grid.axes(config), indexing="ij", sparse=True))
+    linear = sum(fields[k] * (k + 1) for k in (0, 1, 3, 4, 5, 7))
+    expected = sum(s[:, k] * (k + 1) for k in (0, 1, 3, 4, 5, 7))
+    np.testing.assert_allclose((linear.reshape(-1)[ii] * ww).sum(1), expected, atol=1e-12)
+
+
+def test_stencil_matches_brute_force_corner_enumeration():
+    """Indices and weights against a per-state loop over
+    itertools.product([0, 1], repeat=8), the reference's corner order."""
+    config = ReversibleConfig()
+    grid = WorldGrid(6, 5, 4, 3)
+    axes = grid.axes(config)
+    rng = np.random.default_rng(5)
+    lo = np.array([-3.5, -3.5, -7, -1, -3.5, -3.5, -7, -0.5])
+    hi = np.array([3.5, 3.5, 7, 2, 3.5, 3.5, 7, 1])
+    states = rng.uniform(lo, hi, size=(20, 8))
+    states[0, 0] = 3.5  # last node exactly
+    states[1, 3] = -1.0
+    ii, ww, valid, _ = interpolation_stencil(states, axes, (2, 6))
+    assert valid.all()
+    shape = grid.shape
+    for s, row_i, row_w in zip(states, ii, ww):
+        cells = []
+        for k, a in enumerate(axes):
+            h, n = a[1] - a[0], len(a)
+            if k in (2, 6):
+                f = ((s[k] - a[0]) % (h * n)) / h
+                i0 = int(np.floor(f)) % n
+                cells.append((i0, (i0 + 1) % n, f - np.floor(f)))
+            else:
+                f = (min(max(s[k], a[0]), a[-1]) - a[0]) / h
+                i0 = min(int(np.floor(f)), n - 2)
+                cells.append((i0, i0 + 1, min(max(f - i0, 0.0), 1.0)))
+        for c, bits in enumerate(product([0, 1], repeat=8)):
+            node = [cells[k][b] for k, b in enumerate(bits)]
+            w = 1.0
+            for k, b in enumerate(bits):
+                w *= cells[k][2] if b else 1 - cells[k][2]
+            assert row_i[c] == np.ravel_multi_index(node, shape)
+            assert row_w[c] == pytest.approx(w, abs=1e-15)
+
+
+def test_stencil_marks_outside_and_nonfinite_states_invalid():
+    config = ReversibleConfig()
+    axes = WorldGrid(5, 4, 3, 3).axes(config)
+    s = np.zeros((3, 8))
+    s[:, 3] = 0.5
+    s[0, 0] = 20
+    s[1, 7] = np.nan
+    s[2, 2] = 100.0  # periodic axes never leave the domain
+    _, _, valid, _ = interpolation_stencil(s, axes, (2, 6))
+    assert valid.tolist() == [False, False, True]
+
+
+def test_output_times():
+    np.testing.assert_allclose(output_times(0.5, 0.25), [0, 0.25, 0.5])
+    np.testing.assert_allclose(output_times(0.3, 0.25), [0, 0.25, 0.3])
+    with pytest.raises(ValueError):
+        output_times(0.0, 0.1)
+
+
+def test_resource_guard_happens_before_allocation(tmp_path):
+    """(tests/test_reversible_world_dp.py:193-201)"""
+    with pytest.raises(MemoryError):
+        solve(ReversibleConfig(), WorldGrid(17, 16, 7, 7), tmp_path / "big", max_core_gb=0.001)
+    assert not (tmp_path / "big").exists()
+
+
+@pytest.mark.parametrize("kwargs, error", [
+    (dict(order=3), ValueError),
+    (dict(cfl=1.5), ValueError),
+    (dict(threads=0), ValueError),
+    (dict(speed_boundary="clamped"), ValueError),
+    (dict(history=False, write_final=False), ValueError),
+    (dict(max_output_gb=0), ValueError),
+])
+def test_invalid_options_raise_before_output(tmp_path, kwargs, error):
+    with pytest.raises(error):
+        solve(ReversibleConfig(), WorldGrid(5, 4, 3, 3), tmp_path / "x", **kwargs)
+    assert not (tmp_path / "x").exists()
+
+
+def test_grid_and_config_validation(tmp_path):
+    for bad in (dict(positions=4), dict(headings=3), dict(p_speeds=True), dict(extent=-1)):
+        with pytest.raises(ValueError):
+            WorldGrid(**bad)
+    with pytest.raises(ValueError):
+        solve(ReversibleConfig(), WorldGrid(5, 4, 3, 3, 2.5), tmp_path / "pad")
+    with pytest.raises(TypeError):
+        solve(object(), WorldGrid(5, 4, 3, 3), tmp_path / "cfg")
+
+
+def test_existing_run_is_preserved(tmp_path):
+    out = tmp_path / "run"
+    out.mkdir()
+    (out / "meta.json").write_text("{}")
+    with pytest.raises(FileExistsError):
+        solve(ReversibleConfig(), WorldGrid(5, 4, 3, 3), out)
+
+
+def test_global_rate_bound_closed_form():
+    """Default grid: dx = 7/8, dh = pi/4, dvp = 0.5, dve = 0.25;
+    2*(2+1)/dx + (2*1 + 1*2)/dh + 2/0.5 + 1/0.25."""
+    bound = global_rate_bound(ReversibleConfig(), WorldGrid())
+    assert bound == pytest.approx(6 / 0.875 + 4 / (np.pi / 4) + 4 + 4, rel=1e-12)
