@@ -351,11 +351,14 @@ typedef struct pe_world8d_config {
   double cfl;                          /* --cfl, in (0, 1] (double)        */
 } pe_world8d_config;
 
-/* pe_world8d_config.flags: one thread per node everywhere (the default
-   stages each pursuer slab of evader states in LDS when it fits in 64 KiB);
-   512-thread slab blocks instead of 256.  Results are identical. */
+/* pe_world8d_config.flags (comparison runs; results are identical).  The
+   default stages each pursuer slab of evader states in LDS, one block of
+   512 (slab <= 36 KiB) or 1024 threads per slab, and uses one thread per
+   node for slabs above 64 KiB.  GENERIC: one thread per node everywhere;
+   SLAB_256 / SLAB_1024: that many threads per slab block. */
 #define PE_WORLD8D_GENERIC 1
-#define PE_WORLD8D_WIDE_SLAB 2
+#define PE_WORLD8D_SLAB_256 2
+#define PE_WORLD8D_SLAB_1024 4
 
 typedef struct pe_world8d_stats {
   int64_t states;           /* grid nodes                                  */
@@ -364,8 +367,10 @@ typedef struct pe_world8d_stats {
   double max_internal_dt;   /* CFL step limit (native_world.cpp:133-135)   */
   double update_seconds;    /* device time of the stepping                 */
   int64_t probes;           /* probe stencils set                          */
-  int32_t kernel;           /* 0: thread per node; 1, 2: LDS slab, 256/512 */
-  int32_t reserved;
+  int32_t kernel;           /* 0: thread per node; else threads per LDS
+                               slab block                                  */
+  int32_t fast_division;    /* 1: the spacings allow the reciprocal division
+                               (pe_world8d_check_division)                 */
 } pe_world8d_stats;
 
 typedef struct pe_world8d pe_world8d;
@@ -396,6 +401,13 @@ int pe_world8d_set_probes(pe_world8d *h, const uint64_t *indices,
 int pe_world8d_probe(pe_world8d *h, double *values);
 int pe_world8d_get_stats(pe_world8d *h, pe_world8d_stats *out);
 void pe_world8d_destroy(pe_world8d *h);
+/* Self-check of the kernels' division a / b = correction steps on RN(1/b)
+ * (used while every field value is +0 or of magnitude in [2^-60, 2^60])
+ * against the IEEE division, for every float a with exponent in [lo, hi)
+ * of either sign and +0: mismatches[k] for divisors[k] in [2^-20, 2^20].
+ * No counterpart in the reference. */
+int pe_world8d_check_division(const float *divisors, int32_t count, int32_t lo,
+                              int32_t hi, int64_t *mismatches);
 
 #ifdef __cplusplus
 }

@@ -17,9 +17,10 @@
 // The initial field (capture or evader wall fault, max with the avoid
 // surface) is computed on the host with the same float operations.
 //
-// Layout of a launch: one thread per node; a block covers 256 consecutive
-// nodes of one (xP, yP, thP, vP) slab, so the pursuer coordinates and
-// everything derived from them are uniform per block.
+// Layout of a launch: one block per (xP, yP, thP, vP) slab, whose inner
+// (xE, yE, thE, vE) values are staged in LDS, so the pursuer coordinates and
+// everything derived from them are uniform per block and the evader-axis
+// neighbours are LDS reads.  Slabs above 64 KiB use one thread per node.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -34,6 +35,8 @@ struct W8 {
   int n[8];              // axis lengths
   long long stride[8];   // element strides
   float h[8];            // spacings
+  float hh[8];           // h * h (float, as the scheme forms it)
+  float rh[8], rhh[8];   // reciprocals of h and hh rounded to nearest
   long long N;           // nodes
   int inner;             // nodes of one pursuer slab: n * n * nh * ne
   int inner_blocks;      // ceil(inner / 256)
@@ -43,76 +46,146 @@ struct W8 {
   // for every r < inner when fast_decode (inner * n[k] < 2^32).
   unsigned magic[8];
   int fast_decode;
+  int fast_division;     // divisors within [2^-20, 2^20]: see w8_div
 };
 
 __device__ __forceinline__ float w8_mn(float a, float b) { return b < a ? b : a; }
 __device__ __forceinline__ float w8_mx(float a, float b) { return a < b ? b : a; }
 
-// Field accessors: global memory by node index, or a slab staged in LDS by
+// a / b, correctly rounded.  FAST: b > 0 is one of the grid spacings or
+// their squares and y = RN(1/b) (host-computed).  q0 = RN(a y) lies within
+// 1.5 ulp of a/b; one correction q1 = RN(q0 + (a - b q0) y) brings it within
+// one ulp; then, with y within half an ulp of 1/b and q1 within one ulp of
+// a/b, the remainder a - b q1 is exact and RN(q1 + (a - b q1) y) is the
+// correctly rounded quotient (Markstein's theorem, P. Markstein, IBM J. Res.
+// Dev. 34(1), 1990).  The theorem needs a normal quotient and an exact
+// remainder: the kernels take this path only while every value of the field
+// they read is +0 or finite with magnitude in [2^-60, 2^60], so that every
+// non-zero numerator (a difference, second difference or extrapolation of
+// such values) lies in [2^-83, 2^64]; a numerator +0 gives +0 as a / b does
+// and -0 never occurs (the flag counts -0 as out of range).  Otherwise the
+// IEEE division is used.
+template <bool FAST>
+__device__ __forceinline__ float w8_div(float a, float b, float y) {
+  if (!FAST) return a / b;
+  const float q0 = a * y;
+  const float q1 = fmaf(fmaf(-q0, b, a), y, q0);
+  return fmaf(fmaf(-q1, b, a), y, q1);
+}
+
+// Whether a value keeps the next stage on the fast division (see w8_div).
+__device__ __forceinline__ bool w8_in_range(float v) {
+  const float a = fabsf(v);
+  return (a >= 0x1p-60f && a <= 0x1p60f) || __float_as_uint(v) == 0u;
+}
+
+// Field accessors: global memory by node index (32-bit byte offsets from the
+// field's base when the field is below 4 GiB), or a slab staged in LDS by
 // index within the slab.  Same values, so the same arithmetic either way.
+template <typename I>
 struct W8Global {
   const float *__restrict__ p;
-  __device__ __forceinline__ float operator()(long long i) const { return p[i]; }
+  __device__ __forceinline__ float operator()(I i) const {
+    if constexpr (sizeof(I) == 4)
+      return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(p) +
+                                              ((unsigned)i << 2));
+    else
+      return p[i];
+  }
 };
 struct W8Lds {
   const float *p;
-  __device__ __forceinline__ float operator()(long long i) const { return p[(int)i]; }
+  __device__ __forceinline__ float operator()(int i) const { return p[i]; }
 };
 
 // Value `off` cells from node j along one axis: periodic wrap, or the signed
 // linear extrapolation of the edge values past a non-periodic end.
-template <typename V>
-__device__ __forceinline__ float w8_at(const V &v, long long j, long long stride, int pos, int n,
-                                       int off, bool periodic) {
+template <typename I, typename V>
+__device__ __forceinline__ float w8_at(const V &v, I j, I stride, int pos, int n, int off,
+                                       bool periodic) {
   int q = pos + off;
   if (periodic) {  // |off| <= 2 < n
     q = q < 0 ? q + n : q >= n ? q - n : q;
-    return v(j + (long long)(q - pos) * stride);
+    return v(j + (I)(q - pos) * stride);
   }
   if (q < 0) {
-    const long long edge = j - (long long)pos * stride;
+    const I edge = j - (I)pos * stride;
     const float a = v(edge), b = v(edge + stride);
     const float sign = (float)((a > 0) - (a < 0));
     return a + (float)(-q) * fabsf(b - a) * sign;
   }
   if (q >= n) {
-    const long long edge = j + (long long)(n - 1 - pos) * stride;
+    const I edge = j + (I)(n - 1 - pos) * stride;
     const float a = v(edge), b = v(edge - stride);
     const float sign = (float)((a > 0) - (a < 0));
     return a + (float)(q - n + 1) * fabsf(a - b) * sign;
   }
-  return v(j + (long long)off * stride);
+  return v(j + (I)off * stride);
 }
 
-// Left and right ENO slopes of the node j (value f) along one axis.
-template <typename V>
-__device__ __forceinline__ void w8_slopes(const V &v, long long j, float f, long long stride,
-                                          int pos, int n, float h, bool periodic, bool linear,
-                                          int order, float &left, float &right) {
+// Left and right ENO slopes of the node j (value f) along axis k.
+template <bool FAST, typename I, typename V>
+__device__ __forceinline__ void w8_slopes(const V &v, I j, float f, int k, int pos,
+                                          const W8 &g, bool periodic, bool linear, float &left,
+                                          float &right) {
+  const I stride = (I)g.stride[k];
+  const int n = g.n[k];
+  const float h = g.h[k], rh = g.rh[k];
   if (linear && pos == 0) {
-    left = right = (v(j + stride) - f) / h;
+    left = right = w8_div<FAST>(v(j + stride) - f, h, rh);
     return;
   }
   if (linear && pos == n - 1) {
-    left = right = (f - v(j - stride)) / h;
+    left = right = w8_div<FAST>(f - v(j - stride), h, rh);
     return;
   }
   const float fm = w8_at(v, j, stride, pos, n, -1, periodic),
               fp = w8_at(v, j, stride, pos, n, 1, periodic);
-  float l = (f - fm) / h, r = (fp - f) / h;
-  if (order == 2) {
-    const float hh = h * h;
-    const float dc = (fp - 2 * f + fm) / hh;
-    const float dl =
-        linear && pos < 2 ? dc : (f - 2 * fm + w8_at(v, j, stride, pos, n, -2, periodic)) / hh;
+  float l = w8_div<FAST>(f - fm, h, rh), r = w8_div<FAST>(fp - f, h, rh);
+  if (g.order == 2) {
+    const float hh = g.hh[k], rhh = g.rhh[k];
+    const float dc = w8_div<FAST>(fp - 2 * f + fm, hh, rhh);
+    const float dl = linear && pos < 2
+                         ? dc
+                         : w8_div<FAST>(f - 2 * fm + w8_at(v, j, stride, pos, n, -2, periodic),
+                                        hh, rhh);
     const float dr = linear && pos + 2 >= n
                          ? dc
-                         : (w8_at(v, j, stride, pos, n, 2, periodic) - 2 * fp + f) / hh;
+                         : w8_div<FAST>(w8_at(v, j, stride, pos, n, 2, periodic) - 2 * fp + f,
+                                        hh, rhh);
     l += .5f * h * (fabsf(dl) <= fabsf(dc) ? dl : dc);
     r -= .5f * h * (fabsf(dc) <= fabsf(dr) ? dc : dr);
   }
   left = l;
   right = r;
+}
+
+// w8_slopes for a speed axis with the linear speed boundary, without
+// branches (its end cases differ from lane to lane): every lane loads
+// in-range neighbours (its own value in place of a missing one), forms the
+// ENO quantities and selects.  The selected values are the expressions of
+// w8_slopes: at pos 0 both slopes are (v[j+s] - f) / h, the uncorrected right
+// slope; at n-1 both are (f - v[j-s]) / h; at pos 1 (n-2) dl (dr) is dc.
+template <bool FAST, typename I, typename V>
+__device__ __forceinline__ void w8_slopes_speed(const V &v, I j, float f, int k, int pos,
+                                                const W8 &g, float &left, float &right) {
+  const I s = (I)g.stride[k];
+  const int n = g.n[k];
+  const float h = g.h[k], rh = g.rh[k];
+  const float fm = pos > 0 ? v(j - s) : f, fp = pos < n - 1 ? v(j + s) : f;
+  const float l0 = w8_div<FAST>(f - fm, h, rh), r0 = w8_div<FAST>(fp - f, h, rh);
+  float l = l0, r = r0;
+  if (g.order == 2) {
+    const float hh = g.hh[k], rhh = g.rhh[k];
+    const float fm2 = pos >= 2 ? v(j - 2 * s) : f, fp2 = pos + 2 < n ? v(j + 2 * s) : f;
+    const float dc = w8_div<FAST>(fp - 2 * f + fm, hh, rhh);
+    const float dl = pos < 2 ? dc : w8_div<FAST>(f - 2 * fm + fm2, hh, rhh);
+    const float dr = pos + 2 >= n ? dc : w8_div<FAST>(fp2 - 2 * fp + f, hh, rhh);
+    l += .5f * h * (fabsf(dl) <= fabsf(dc) ? dl : dc);
+    r -= .5f * h * (fabsf(dc) <= fabsf(dr) ? dc : dr);
+  }
+  left = pos == 0 ? r0 : pos == n - 1 ? l0 : l;
+  right = pos == 0 ? r0 : pos == n - 1 ? l0 : r;
 }
 
 // Pursuer-side quantities of a slab (uniform over its nodes).
@@ -142,17 +215,26 @@ __device__ __forceinline__ W8Slab w8_slab(long long s, const W8 &g, const float 
   return p;
 }
 
+// Per-launch arguments beside the fields.
+struct W8Launch {
+  const float *x, *cs, *sn, *pv, *ev;  // tables
+  float dt;
+  double tau;
+  // Range flags (see w8_div): the launch that wrote src recorded its id in
+  // *src_flag if it wrote a value out of range; this launch records `id` in
+  // *dst_flag if it does.
+  const unsigned long long *src_flag;
+  unsigned long long src_id, *dst_flag, id;
+};
+
 // The update of node j = slab * inner + i (native_world.cpp:261-314): axes
-// 0-3 read `outer`, axes 4-7 read `inner_v` at index `ii` (i, or j when
-// both are the global field).
-template <typename VO, typename VI>
-__device__ __forceinline__ void w8_node(const VO &outer, const VI &inner_v, long long j,
-                                        long long ii, float center, const W8Slab &p, int i,
-                                        float *dst, const float *base,
-                                        float *__restrict__ arrival, const float *__restrict__ cs,
-                                        const float *__restrict__ sn,
-                                        const float *__restrict__ ev, const W8 &g, float dt,
-                                        double tau) {
+// 0-3 read `outer` at j, axes 4-7 read `inner_v` at ii (i, or j when both
+// are the global field).  Returns whether the written value is in range.
+template <bool FAST, typename I, typename II, typename VO, typename VI>
+__device__ __forceinline__ bool w8_node(const VO &outer, const VI &inner_v, I j, II ii,
+                                        float center, const W8Slab &p, int i, float *dst,
+                                        const float *base, float *__restrict__ arrival,
+                                        const W8 &g, const W8Launch &L) {
   int q[8];
   q[0] = p.q[0];
   q[1] = p.q[1];
@@ -175,19 +257,19 @@ __device__ __forceinline__ void w8_node(const VO &outer, const VI &inner_v, long
     q[4] = r / g.n[5];
   }
   const int eh = q[6], iv = q[7];
-  const float ve = ev[iv], we = fabsf(ve) * g.ke, ealo = iv == 0 ? 0 : -g.ae,
+  const float ve = L.ev[iv], we = fabsf(ve) * g.ke, ealo = iv == 0 ? 0 : -g.ae,
               eahi = iv == g.n[7] - 1 ? 0 : g.ae;
-  const float f[8] = {p.fpx, p.fpy, 0, 0, ve * cs[eh], ve * sn[eh], 0, 0};
+  const float f[8] = {p.fpx, p.fpy, 0, 0, ve * L.cs[eh], ve * L.sn[eh], 0, 0};
   float gk[8], H = 0, diss = 0;
 #pragma unroll
   for (int k = 0; k < 8; k++) {
     float lm, rp;
     if (k < 4)
-      w8_slopes(outer, j, center, g.stride[k], q[k], g.n[k], g.h[k], k == 2,
-                g.linear && k == 3, g.order, lm, rp);
+      w8_slopes<FAST>(outer, j, center, k, q[k], g, k == 2, g.linear && k == 3, lm, rp);
+    else if (k == 7 && g.linear)
+      w8_slopes_speed<FAST>(inner_v, ii, center, k, q[k], g, lm, rp);
     else
-      w8_slopes(inner_v, ii, center, g.stride[k], q[k], g.n[k], g.h[k], k == 6,
-                g.linear && k == 7, g.order, lm, rp);
+      w8_slopes<FAST>(inner_v, ii, center, k, q[k], g, k == 6, false, lm, rp);
     gk[k] = .5f * (lm + rp);
     const float alpha = k == 2 ? p.wp : k == 3 ? g.ap : k == 6 ? we : k == 7 ? g.ae : fabsf(f[k]);
     H += f[k] * gk[k];
@@ -195,55 +277,74 @@ __device__ __forceinline__ void w8_node(const VO &outer, const VI &inner_v, long
   }
   H += -p.wp * fabsf(gk[2]) + we * fabsf(gk[6]) + w8_mn(p.palo * gk[3], p.pahi * gk[3]) +
        w8_mx(ealo * gk[7], eahi * gk[7]);
-  float value = center + dt * (H + diss);
+  float value = center + L.dt * (H + diss);
   if (base) value = .5f * (base[j] + value);
   if (g.order == 1 || base) {
     const float previous = base ? base[j] : center;
     value = w8_mx(w8_mn(value, previous), p.avoid);
     if (previous > 0 && value <= 0 && isnan(arrival[j]))
-      arrival[j] = (float)tau + dt * previous / (previous - value);
+      arrival[j] = (float)L.tau + L.dt * previous / (previous - value);
   }
   dst[j] = value;
+  return w8_in_range(value);
+}
+
+// Whether this launch may divide fast: the divisors allow it and the launch
+// that wrote src wrote no value out of range.
+__device__ __forceinline__ bool w8_fast(const W8 &g, const W8Launch &L) {
+  return g.fast_division && *L.src_flag != L.src_id;
 }
 
 // One RK stage src -> dst, one thread per node; base = the previous field in
 // the blend stage (may alias dst: each node reads base[j] before writing
-// dst[j]).  Blocks cover 256 nodes of one slab.
-__global__ __launch_bounds__(256) void world8d_stage(
-    const float *__restrict__ src, float *dst, const float *base, float *__restrict__ arrival,
-    const float *__restrict__ x, const float *__restrict__ cs, const float *__restrict__ sn,
-    const float *__restrict__ pv, const float *__restrict__ ev, W8 g, float dt, double tau) {
+// dst[j]).  Blocks cover 256 nodes of one slab.  I: index type (int when the
+// field is below 4 GiB).
+template <typename I>
+__global__ __launch_bounds__(256) void world8d_stage(const float *__restrict__ src, float *dst,
+                                                     const float *base,
+                                                     float *__restrict__ arrival, W8 g,
+                                                     W8Launch L) {
   const long long b = blockIdx.x;
   const long long slab = b / g.inner_blocks;  // (xP, yP, thP, vP), uniform
   const int i = (int)(b - slab * g.inner_blocks) * 256 + (int)threadIdx.x;
   if (i >= g.inner) return;
-  const W8Slab p = w8_slab(slab, g, x, cs, sn, pv);
-  const long long j = slab * g.inner + i;
-  const W8Global v{src};
-  w8_node(v, v, j, j, src[j], p, i, dst, base, arrival, cs, sn, ev, g, dt, tau);
+  const W8Slab p = w8_slab(slab, g, L.x, L.cs, L.sn, L.pv);
+  const I j = (I)(slab * g.inner + i);
+  const W8Global<I> v{src};
+  const bool ok = w8_fast(g, L)
+                      ? w8_node<true>(v, v, j, j, v(j), p, i, dst, base, arrival, g, L)
+                      : w8_node<false>(v, v, j, j, v(j), p, i, dst, base, arrival, g, L);
+  if (!ok) atomicMax(L.dst_flag, L.id);
 }
 
 // The same stage with one block per slab: the slab's inner (xE, yE, thE, vE)
 // values are staged in LDS, so the evader-axis neighbours are LDS reads and
 // only the pursuer-axis neighbours (block-uniform boundary cases) are global
 // loads.
-template <int THREADS>
-__global__ __launch_bounds__(THREADS) void world8d_stage_slab(
-    const float *__restrict__ src, float *dst, const float *base, float *__restrict__ arrival,
-    const float *__restrict__ x, const float *__restrict__ cs, const float *__restrict__ sn,
-    const float *__restrict__ pv, const float *__restrict__ ev, W8 g, float dt, double tau) {
+template <int THREADS, typename I>
+__global__ __launch_bounds__(THREADS) void world8d_stage_slab(const float *__restrict__ src,
+                                                              float *dst, const float *base,
+                                                              float *__restrict__ arrival, W8 g,
+                                                              W8Launch L) {
   extern __shared__ float slab_values[];
   const long long slab = blockIdx.x;
   const float *__restrict__ row = src + slab * g.inner;
   for (int i = threadIdx.x; i < g.inner; i += THREADS) slab_values[i] = row[i];
   __syncthreads();
-  const W8Slab p = w8_slab(slab, g, x, cs, sn, pv);
-  const W8Global outer{src};
+  const W8Slab p = w8_slab(slab, g, L.x, L.cs, L.sn, L.pv);
+  const W8Global<I> outer{src};
   const W8Lds inner_v{slab_values};
+  const bool fast = w8_fast(g, L);
+  bool ok = true;
 #pragma unroll 1
-  for (int i = threadIdx.x; i < g.inner; i += THREADS)
-    w8_node(outer, inner_v, slab * g.inner + i, i, slab_values[i], p, i, dst, base, arrival,
-            cs, sn, ev, g, dt, tau);
+  for (int i = threadIdx.x; i < g.inner; i += THREADS) {
+    const I j = (I)(slab * g.inner + i);
+    ok &= fast ? w8_node<true>(outer, inner_v, j, i, slab_values[i], p, i, dst, base, arrival,
+                               g, L)
+               : w8_node<false>(outer, inner_v, j, i, slab_values[i], p, i, dst, base,
+                                arrival, g, L);
+  }
+  if (!ok) atomicMax(L.dst_flag, L.id);
 }
 
 // Count of non-finite values (grid-stride; one atomic per non-zero wave).
@@ -271,9 +372,58 @@ __global__ __launch_bounds__(256) void world8d_probe(const float *__restrict__ v
   out[q] = sum;
 }
 
+// Self-check of w8_div<true> against the IEEE division: every float a of
+// either sign with exponent in [lo, hi) (and +0) divided by b.
+__global__ __launch_bounds__(256) void world8d_division_check(float b, float y, int lo, int hi,
+                                                              unsigned long long *mismatches) {
+  const unsigned long long span = (unsigned long long)(hi - lo) << 23;
+  const unsigned long long total = 2 * span + 1;
+  unsigned long long bad = 0;
+  for (unsigned long long t = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+       t < total; t += (unsigned long long)gridDim.x * blockDim.x) {
+    unsigned bits = 0;
+    if (t < 2 * span) {
+      const unsigned long long u = t % span;
+      bits = (unsigned)(((unsigned long long)(lo + 127) << 23) + u) | (t >= span ? 0x80000000u : 0u);
+    }
+    const float a = __uint_as_float(bits);
+    bad += __float_as_uint(w8_div<true>(a, b, y)) != __float_as_uint(a / b);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) bad += __shfl_xor(bad, off, 64);
+  if ((threadIdx.x & 63) == 0 && bad) atomicAdd(mismatches, bad);
+}
+
 }  // namespace pe
 
 using namespace pe;
+
+// RN(1/b) for b > 0: the float nearest to 1/b among the double quotient's
+// float rounding and its neighbours (b y is exact in double).
+static float recip_nearest(float b) {
+  const float y0 = (float)(1.0 / (double)b);
+  float best = y0;
+  double err = std::fabs(1.0 - (double)b * (double)y0);
+  for (float c : {std::nextafter(y0, 0.f), std::nextafter(y0, INFINITY)}) {
+    const double e = std::fabs(1.0 - (double)b * (double)c);
+    if (e < err) {
+      err = e;
+      best = c;
+    }
+  }
+  return best;
+}
+
+// Whether a field keeps the stages on the fast division (w8_in_range).
+static bool field_in_range(const float *v, long long n) {
+  for (long long k = 0; k < n; k++) {
+    const float a = std::fabs(v[k]);
+    uint32_t bits;
+    std::memcpy(&bits, &v[k], 4);
+    if (!((a >= 0x1p-60f && a <= 0x1p60f) || bits == 0u)) return false;
+  }
+  return true;
+}
 
 struct pe_world8d {
   int device = 0;
@@ -286,7 +436,11 @@ struct pe_world8d {
   unsigned long long *probe_index = nullptr;
   double *probe_weight = nullptr, *probe_out = nullptr;
   long long probes = 0;
-  int variant = 0;  // 0: thread per node; 1, 2: LDS slab kernel, 256 / 512 threads
+  int variant = 0;  // 0: thread per node; else threads per LDS slab block
+  // Range flags of the two field buffers and the id of the launch (or
+  // upload) that last wrote each; ids increase.
+  unsigned long long *flags = nullptr, *a_flag = nullptr, *b_flag = nullptr;
+  unsigned long long a_id = 0, b_id = 0, next_id = 1;
   double maxdt = 0, tau = 0, update_seconds = 0;
   long long steps = 0;
   hipEvent_t t0 = nullptr, t1 = nullptr;
@@ -308,9 +462,15 @@ static void world8d_setup(pe_world8d *h) {
   const float spacing[8] = {dx, dx, dh, dvp, dx, dx, dh, dve};
   g.stride[7] = 1;
   for (int k = 6; k >= 0; k--) g.stride[k] = g.stride[k + 1] * shape[k + 1];
+  g.fast_division = 1;
   for (int k = 0; k < 8; k++) {
     g.n[k] = shape[k];
     g.h[k] = spacing[k];
+    g.hh[k] = spacing[k] * spacing[k];
+    g.rh[k] = recip_nearest(g.h[k]);
+    g.rhh[k] = recip_nearest(g.hh[k]);
+    if (!(g.h[k] >= 0x1p-20f && g.h[k] <= 0x1p20f && g.hh[k] >= 0x1p-20f && g.hh[k] <= 0x1p20f))
+      g.fast_division = 0;
   }
   g.N = g.stride[0] * n;
   g.inner = n * n * nh * ne;
@@ -375,6 +535,14 @@ static void world8d_initial(const pe_world8d *h, std::vector<float> &field,
   }
 }
 
+// Range flag of a field just uploaded into h->a.
+static int world8d_uploaded(pe_world8d *h, const float *field) {
+  h->a_id = h->next_id++;
+  const unsigned long long flag = field_in_range(field, h->g.N) ? 0 : h->a_id;
+  PE_HIP(hipMemcpy(h->a_flag, &flag, 8, hipMemcpyHostToDevice));
+  return PE_OK;
+}
+
 extern "C" int pe_world8d_create(const pe_world8d_config *cfg, pe_world8d **out) {
   clear_error();
   if (!cfg || !out) return fail(PE_ERR_INVALID, "null argument");
@@ -386,7 +554,7 @@ extern "C" int pe_world8d_create(const pe_world8d_config *cfg, pe_world8d **out)
       !(c.p_min < c.p_max) || !(c.e_min < c.e_max) || !(c.p_accel > 0) || !(c.e_accel > 0) ||
       !(c.p_curvature > 0) || !(c.e_curvature > 0) || !(c.radius >= 0) ||
       (c.linear_speed_boundary != 0 && c.linear_speed_boundary != 1) ||
-      (c.flags & ~(PE_WORLD8D_GENERIC | PE_WORLD8D_WIDE_SLAB)) != 0)
+      (c.flags & ~(PE_WORLD8D_GENERIC | PE_WORLD8D_SLAB_256 | PE_WORLD8D_SLAB_1024)) != 0)
     return fail(PE_ERR_INVALID, "Invalid arguments");
   const double nodes = std::pow((double)c.positions, 4) * c.headings * c.headings *
                        c.p_speeds * c.e_speeds;
@@ -398,9 +566,15 @@ extern "C" int pe_world8d_create(const pe_world8d_config *cfg, pe_world8d **out)
   h->device = dev;
   h->cfg = c;
   world8d_setup(h);
-  // The slab kernel needs the slab in LDS; 64 KiB keeps two blocks per CU.
-  if (!(c.flags & PE_WORLD8D_GENERIC) && sizeof(float) * (size_t)h->g.inner <= 65536)
-    h->variant = (c.flags & PE_WORLD8D_WIDE_SLAB) ? 2 : 1;
+  // The slab kernel stages a slab in LDS (at most 64 KiB per block).  512
+  // threads per block up to 36 KiB slabs (four or more blocks per CU), 1024
+  // above, where LDS allows two blocks per CU.
+  const size_t slab_bytes = sizeof(float) * (size_t)h->g.inner;
+  if (!(c.flags & PE_WORLD8D_GENERIC) && slab_bytes <= 65536)
+    h->variant = (c.flags & PE_WORLD8D_SLAB_256)    ? 256
+                 : (c.flags & PE_WORLD8D_SLAB_1024) ? 1024
+                 : slab_bytes <= 36864              ? 512
+                                                    : 1024;
   auto body = [&]() -> int {
     PE_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     PE_HIP(hipEventCreate(&h->t0));
@@ -416,11 +590,15 @@ extern "C" int pe_world8d_create(const pe_world8d_config *cfg, pe_world8d **out)
     PE_HIP(hipMalloc(&h->tables, sizeof(float) * tab.size()));
     PE_HIP(hipMemcpy(h->tables, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice));
     PE_HIP(hipMalloc(&h->count, 8));
+    PE_HIP(hipMalloc(&h->flags, 16));
+    PE_HIP(hipMemset(h->flags, 0, 16));
+    h->a_flag = h->flags;
+    h->b_flag = h->flags + 1;
     std::vector<float> field, arrival;
     world8d_initial(h, field, arrival);
     PE_HIP(hipMemcpy(h->a, field.data(), bytes, hipMemcpyHostToDevice));
     PE_HIP(hipMemcpy(h->arrival, arrival.data(), bytes, hipMemcpyHostToDevice));
-    return PE_OK;
+    return world8d_uploaded(h, field.data());
   };
   rc = body();
   if (rc) {
@@ -450,38 +628,68 @@ extern "C" int pe_world8d_set_field(pe_world8d *h, const float *field, const flo
     PE_HIP(hipMemcpy(h->arrival, arr.data(), bytes, hipMemcpyHostToDevice));
   }
   h->tau = time;
+  return world8d_uploaded(h, field);
+}
+
+// One RK stage src -> dst (buffers a or b with their range flags and ids).
+static int world8d_launch(pe_world8d *h, const float *src, float *dst, const float *base,
+                          double dt) {
+  const W8 &g = h->g;
+  W8Launch L;
+  L.x = h->tables;
+  L.cs = L.x + g.n[0];
+  L.sn = L.cs + g.n[2];
+  L.pv = L.sn + g.n[2];
+  L.ev = L.pv + g.n[3];
+  L.dt = (float)dt;
+  L.tau = h->tau;
+  const bool from_a = src == h->a;
+  L.src_flag = from_a ? h->a_flag : h->b_flag;
+  L.src_id = from_a ? h->a_id : h->b_id;
+  L.dst_flag = from_a ? h->b_flag : h->a_flag;
+  L.id = h->next_id++;
+  (from_a ? h->b_id : h->a_id) = L.id;
+  const unsigned slabs = (unsigned)(g.N / g.inner);
+  const size_t lds = sizeof(float) * (size_t)g.inner;
+  // 32-bit byte offsets from the field base while the field is below 4 GiB.
+  const bool small = (unsigned long long)g.N * 4 < (1ull << 32);
+  if (h->variant == 0) {
+    if (small)
+      hipLaunchKernelGGL(world8d_stage<int>, dim3(slabs * g.inner_blocks), dim3(256), 0,
+                         h->stream, src, dst, base, h->arrival, g, L);
+    else
+      hipLaunchKernelGGL(world8d_stage<long long>, dim3(slabs * g.inner_blocks), dim3(256), 0,
+                         h->stream, src, dst, base, h->arrival, g, L);
+    PE_LAUNCHED(world8d_stage);
+    return PE_OK;
+  }
+#define PE_W8_SLAB(T)                                                                     \
+  if (small)                                                                              \
+    hipLaunchKernelGGL((world8d_stage_slab<T, int>), dim3(slabs), dim3(T), lds, h->stream, \
+                       src, dst, base, h->arrival, g, L);                                 \
+  else                                                                                    \
+    hipLaunchKernelGGL((world8d_stage_slab<T, long long>), dim3(slabs), dim3(T), lds,     \
+                       h->stream, src, dst, base, h->arrival, g, L);
+  if (h->variant == 256) {
+    PE_W8_SLAB(256)
+  } else if (h->variant == 512) {
+    PE_W8_SLAB(512)
+  } else {
+    PE_W8_SLAB(1024)
+  }
+#undef PE_W8_SLAB
+  PE_LAUNCHED(world8d_stage_slab);
   return PE_OK;
 }
 
 // One RK1 or RK2 step of size dt from the current time.
-static int world8d_launch(pe_world8d *h, const float *src, float *dst, const float *base,
-                          double dt) {
-  const W8 &g = h->g;
-  const float *x = h->tables, *cs = x + g.n[0], *sn = cs + g.n[2], *pv = sn + g.n[2],
-              *ev = pv + g.n[3];
-  const unsigned slabs = (unsigned)(g.N / g.inner);
-  const size_t lds = sizeof(float) * (size_t)g.inner;
-  if (h->variant == 0) {
-    hipLaunchKernelGGL(world8d_stage, dim3(slabs * g.inner_blocks), dim3(256), 0, h->stream,
-                       src, dst, base, h->arrival, x, cs, sn, pv, ev, g, (float)dt, h->tau);
-    PE_LAUNCHED(world8d_stage);
-  } else if (h->variant == 1) {
-    hipLaunchKernelGGL(world8d_stage_slab<256>, dim3(slabs), dim3(256), lds, h->stream, src,
-                       dst, base, h->arrival, x, cs, sn, pv, ev, g, (float)dt, h->tau);
-    PE_LAUNCHED(world8d_stage_slab);
-  } else {
-    hipLaunchKernelGGL(world8d_stage_slab<512>, dim3(slabs), dim3(512), lds, h->stream, src,
-                       dst, base, h->arrival, x, cs, sn, pv, ev, g, (float)dt, h->tau);
-    PE_LAUNCHED(world8d_stage_slab);
-  }
-  return PE_OK;
-}
-
 static int world8d_step(pe_world8d *h, double dt) {
   int rc;
   if (h->cfg.order == 1) {
     if ((rc = world8d_launch(h, h->a, h->b, nullptr, dt))) return rc;
     std::swap(h->a, h->b);
+    std::swap(h->a_flag, h->b_flag);
+    std::swap(h->a_id, h->b_id);
   } else {
     if ((rc = world8d_launch(h, h->a, h->b, nullptr, dt))) return rc;
     if ((rc = world8d_launch(h, h->b, h->a, h->a, dt))) return rc;
@@ -586,6 +794,33 @@ extern "C" int pe_world8d_read(pe_world8d *h, float *field, float *arrival) {
   return PE_OK;
 }
 
+extern "C" int pe_world8d_check_division(const float *divisors, int32_t count, int32_t lo,
+                                         int32_t hi, int64_t *mismatches) {
+  clear_error();
+  if (!divisors || count < 0 || !mismatches || lo < -126 || hi > 128 || lo >= hi)
+    return fail(PE_ERR_INVALID, "invalid argument");
+  for (int32_t k = 0; k < count; k++)
+    if (!(divisors[k] >= 0x1p-20f && divisors[k] <= 0x1p20f))
+      return fail(PE_ERR_INVALID, "divisor outside [2^-20, 2^20]");
+  int dev = 0;
+  int rc = require_device(&dev);
+  if (rc) return rc;
+  unsigned long long *d = nullptr;
+  PE_HIP(hipMalloc(&d, 8));
+  for (int32_t k = 0; k < count; k++) {
+    unsigned long long bad = 0;
+    if (hipMemset(d, 0, 8) != hipSuccess) break;
+    hipLaunchKernelGGL(world8d_division_check, dim3(4096), dim3(256), 0, 0, divisors[k],
+                       recip_nearest(divisors[k]), (int)lo, (int)hi, d);
+    if (hipMemcpy(&bad, d, 8, hipMemcpyDeviceToHost) != hipSuccess) break;
+    mismatches[k] = (int64_t)bad;
+  }
+  const hipError_t e = hipGetLastError();
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(PE_ERR_HIP, hipGetErrorString(e));
+  return PE_OK;
+}
+
 extern "C" int pe_world8d_get_stats(pe_world8d *h, pe_world8d_stats *s) {
   clear_error();
   if (!h || !s) return fail(PE_ERR_INVALID, "invalid argument");
@@ -597,6 +832,7 @@ extern "C" int pe_world8d_get_stats(pe_world8d *h, pe_world8d_stats *s) {
   s->update_seconds = h->update_seconds;
   s->probes = h->probes;
   s->kernel = h->variant;
+  s->fast_division = h->g.fast_division;
   return PE_OK;
 }
 
@@ -606,7 +842,7 @@ extern "C" void pe_world8d_destroy(pe_world8d *h) {
   if (h->stream) hipStreamSynchronize(h->stream);
   for (void *p : {(void *)h->a, (void *)h->b, (void *)h->arrival, (void *)h->tables,
                   (void *)h->count, (void *)h->probe_index, (void *)h->probe_weight,
-                  (void *)h->probe_out})
+                  (void *)h->probe_out, (void *)h->flags})
     if (p) hipFree(p);
   if (h->t0) hipEventDestroy(h->t0);
   if (h->t1) hipEventDestroy(h->t1);

@@ -216,12 +216,13 @@ class World8DStats(ctypes.Structure):
         ("update_seconds", ctypes.c_double),
         ("probes", ctypes.c_int64),
         ("kernel", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("fast_division", ctypes.c_int32),
     ]
 
 
 PE_WORLD8D_GENERIC = 1
-PE_WORLD8D_WIDE_SLAB = 2
+PE_WORLD8D_SLAB_256 = 2
+PE_WORLD8D_SLAB_1024 = 4
 
 
 P = ctypes.c_void_p
@@ -288,6 +289,7 @@ SIGNATURES = {
     "pe_world8d_probe": (ctypes.c_int, [P, P]),
     "pe_world8d_get_stats": (ctypes.c_int, [P, ctypes.POINTER(World8DStats)]),
     "pe_world8d_destroy": (None, [P]),
+    "pe_world8d_check_division": (ctypes.c_int, [P, I32, I32, I32, P]),
 }
 
 
@@ -636,3 +638,13 @@ class World8D:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def world8d_check_division(divisors, lo=-90, hi=70):
+    """Mismatch count of the world kernels' reciprocal division against the
+    IEEE division over every numerator with exponent in [lo, hi), per
+    divisor (pe_world8d_check_division)."""
+    d = np.ascontiguousarray(divisors, np.float32)
+    out = np.zeros(len(d), np.int64)
+    check(lib().pe_world8d_check_division(ptr(d), len(d), int(lo), int(hi), ptr(out)))
+    return out

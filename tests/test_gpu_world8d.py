@@ -108,8 +108,9 @@ def test_gpu_matches_oracle_on_ragged_grids(gpu, oracle_lib, shape, order, linea
 
 @pytest.mark.parametrize("shape", [(7, 6, 5, 4), (9, 8, 3, 3), (6, 5, 3, 4)])
 def test_kernel_variants_are_bit_identical(gpu, shape):
-    """The LDS-slab kernel (256 and 512 threads) and the thread-per-node
-    kernel give the same bits, RK2 and RK1, both speed boundaries."""
+    """The LDS-slab kernel (256, 512 and 1024 threads per slab) and the
+    thread-per-node kernel give the same bits, RK2 and RK1, both speed
+    boundaries."""
     from pe_amd import _native
 
     print(f"[world8d] variants {shape}", flush=True)
@@ -120,13 +121,64 @@ def test_kernel_variants_are_bit_identical(gpu, shape):
                     p_min=-1.0, p_max=2.0, e_min=-0.5, e_max=1.0, p_accel=2.0, e_accel=1.0,
                     p_curvature=1.0, e_curvature=2.0, cfl=0.8, horizon=0.3, output_dt=0.1)
         runs = {}
-        for flags in (0, _native.PE_WORLD8D_WIDE_SLAB, _native.PE_WORLD8D_GENERIC):
+        for flags in (0, _native.PE_WORLD8D_GENERIC, _native.PE_WORLD8D_SLAB_256,
+                      _native.PE_WORLD8D_SLAB_1024):
             frames, final, arrival, st, _ = run_frames(case, flags=flags)
             runs[st["kernel"]] = (frames, arrival)
-        assert sorted(runs) == [0, 1, 2]
-        for k in (1, 2):
+        assert sorted(runs) == [0, 256, 512, 1024]
+        for k in (256, 512, 1024):
             np.testing.assert_array_equal(bits(runs[k][0]), bits(runs[0][0]))
             np.testing.assert_array_equal(bits(runs[k][1]), bits(runs[0][1]))
+
+
+def test_reciprocal_division_is_exact_for_the_grid_spacings(gpu):
+    """The kernels divide by a spacing h or its square hh through two
+    correction steps on RN(1/h) while the field stays in range; every float
+    numerator with exponent in [-90, 70) of either sign (and +0), which
+    covers the [2^-83, 2^64] the range flag guarantees, gives the IEEE
+    quotient bit for bit, for the spacings of every grid tested here."""
+    from pe_amd import _native
+
+    print("[world8d] exhaustive division check", flush=True)
+    divisors = set()
+    for n, h, np_, ne, extent in ((9, 8, 7, 7, 3.5), (13, 12, 7, 7, 3.5), (5, 4, 3, 3, 3.5),
+                                  (6, 6, 4, 3, 3.5), (7, 6, 5, 4, 3.5), (8, 5, 4, 5, 3.5),
+                                  (9, 8, 3, 3, 3.5), (5, 4, 4, 4, 3.5), (6, 5, 3, 4, 3.5)):
+        dx = np.float32(2) * np.float32(extent) / np.float32(n - 1)
+        dh = np.float32(2 * np.pi / h)
+        dvp = (np.float32(2.0) - np.float32(-1.0)) / np.float32(np_ - 1)
+        dve = (np.float32(1.0) - np.float32(-0.5)) / np.float32(ne - 1)
+        for d in (dx, dh, dvp, dve):
+            divisors.update((float(d), float(np.float32(d) * np.float32(d))))
+    divisors = np.array(sorted(divisors), np.float32)
+    bad = _native.world8d_check_division(divisors, -90, 70)
+    assert bad.tolist() == [0] * len(divisors), dict(zip(divisors.tolist(), bad.tolist()))
+
+
+def test_out_of_range_field_takes_the_ieee_division(gpu, oracle_lib):
+    """A continuation field holding a -0, a 1e-30 and a 1e30 value (outside
+    the fast division's range) still gives the oracle's bits."""
+    from pe_amd import _native
+
+    print("[world8d] out-of-range continuation", flush=True)
+    case = dict(positions=6, headings=4, p_speeds=3, e_speeds=4, order=2,
+                linear_speed_boundary=1, extent=3.5, arena=3.0, radius=0.3, p_min=-1.0,
+                p_max=2.0, e_min=-0.5, e_max=1.0, p_accel=2.0, e_accel=1.0, p_curvature=1.0,
+                e_curvature=2.0, cfl=0.8)
+    ocfg = oracle_cfg(oracle_lib, case)
+    field, arrival = oracle_lib.world8d_init(ocfg)
+    field[[5, 77, 901]] = np.array([-0.0, 1e-30, 1e30], np.float32)
+    arrival[:] = np.nan
+    with _native.World8D(native_cfg(case)) as w:
+        w.set_field(field, arrival, 0.0)
+        dt = w.stats()["max_internal_dt"]
+        w.advance_to(2 * dt)
+        final, arr = w.read()
+    _, o_final, o_arrival, steps, _ = oracle_lib.world8d_solve(
+        ocfg, 2 * dt, 2 * dt, field=field, arrival=arrival, history=False)
+    assert steps == 2
+    np.testing.assert_array_equal(bits(final), bits(o_final))
+    np.testing.assert_array_equal(bits(arr), bits(o_arrival))
 
 
 def test_gpu_default_grid_matches_oracle_and_invariants(gpu, oracle_lib):
