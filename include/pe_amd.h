@@ -262,6 +262,35 @@ int pe_history_query(pe_history *h, int64_t n, int32_t terms, const int64_t *ids
 void pe_history_destroy(pe_history *h);
 
 /* ======================================================================
+ * Batch queries of a saved 5D relative-game history (the reader of a
+ * completed free_dp solve).
+ *
+ * Replaces the host numpy gathers of RelativeTable.stencil/field/value
+ * (reversible/relative_table.py:76-161).  The history fields.f32
+ * [frames, vP, vE, psi, y, x] is copied to the device once; each query is a
+ * state [x, y, psi, vP, vE] (the reader's public order).  Axes are given in
+ * storage order [vP, vE, psi, y, x]: first value, last value, spacing
+ * (axis[1] - axis[0]).
+ * ====================================================================== */
+typedef struct pe_rtable pe_rtable;
+int pe_rtable_create(const float *fields, int32_t frames, const int32_t *shape,
+                     const double *axes_lo, const double *axes_hi,
+                     const double *axes_step, const double *times,
+                     pe_rtable **out);
+/* Signed field at states [n, 5] and budgets [n]: multilinear in space,
+   linear in time between saved frames; NaN outside the grid or the budget
+   range (relative_table.py:118-131). */
+int pe_rtable_field(pe_rtable *h, const double *states, const double *budgets,
+                    int64_t n, double *out);
+/* First crossing of zero of the interpolated history (relative_table.py:
+   133-161); 0 for captured physical states; speed_limits = P min, P max,
+   E min, E max. */
+int pe_rtable_value(pe_rtable *h, const double *states, int64_t n,
+                    double capture_radius, const double *speed_limits,
+                    double *out);
+void pe_rtable_destroy(pe_rtable *h);
+
+/* ======================================================================
  * Learned value surrogate: MLP with BF16 MFMA kernels (FP32 accumulation).
  *
  * No counterpart in the reference for this network; the reference's own

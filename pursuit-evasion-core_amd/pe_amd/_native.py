@@ -290,6 +290,10 @@ SIGNATURES = {
     "pe_world8d_get_stats": (ctypes.c_int, [P, ctypes.POINTER(World8DStats)]),
     "pe_world8d_destroy": (None, [P]),
     "pe_world8d_check_division": (ctypes.c_int, [P, I32, I32, I32, P]),
+    "pe_rtable_create": (ctypes.c_int, [P, I32, P, P, P, P, P, P]),
+    "pe_rtable_field": (ctypes.c_int, [P, P, P, I64, P]),
+    "pe_rtable_value": (ctypes.c_int, [P, P, I64, F64, P, P]),
+    "pe_rtable_destroy": (None, [P]),
 }
 
 
@@ -648,3 +652,48 @@ def world8d_check_division(divisors, lo=-90, hi=70):
     out = np.zeros(len(d), np.int64)
     check(lib().pe_world8d_check_division(ptr(d), len(d), int(lo), int(hi), ptr(out)))
     return out
+
+
+class RelTable:
+    """Owning wrapper of a pe_rtable handle: a 5D field history resident on
+    the device, queried in batches."""
+
+    def __init__(self, fields, axes, times):
+        f = np.ascontiguousarray(fields, np.float32)
+        frames = f.shape[0]
+        shape = np.array([len(a) for a in axes], np.int32)
+        lo = np.array([a[0] for a in axes], np.float64)
+        hi = np.array([a[-1] for a in axes], np.float64)
+        step = np.array([a[1] - a[0] for a in axes], np.float64)
+        t = np.ascontiguousarray(times, np.float64)
+        if f.size != int(np.prod(shape)) * frames or len(t) != frames:
+            raise ValueError("History shape differs from axes and times")
+        self._h = ctypes.c_void_p()
+        check(lib().pe_rtable_create(ptr(f.reshape(-1)), frames, ptr(shape), ptr(lo), ptr(hi),
+                                     ptr(step), ptr(t), ctypes.byref(self._h)))
+
+    def field(self, states, budgets):
+        s = np.ascontiguousarray(states, np.float64).reshape(-1, 5)
+        b = np.ascontiguousarray(budgets, np.float64).reshape(-1)
+        out = np.empty(len(s), np.float64)
+        check(lib().pe_rtable_field(self._h, ptr(s), ptr(b), len(s), ptr(out)))
+        return out
+
+    def value(self, states, capture_radius, speed_limits):
+        s = np.ascontiguousarray(states, np.float64).reshape(-1, 5)
+        lim = np.ascontiguousarray(speed_limits, np.float64)
+        out = np.empty(len(s), np.float64)
+        check(lib().pe_rtable_value(self._h, ptr(s), len(s), float(capture_radius), ptr(lim),
+                                    ptr(out)))
+        return out
+
+    def close(self):
+        if self._h:
+            lib().pe_rtable_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -443,3 +443,46 @@ def _run(cfg, grid, output, horizon, output_dt, start_time, history, write_final
         "max_internal_dt": st["max_internal_dt"],
         "arithmetic": "float32 fields and local arithmetic",
     }
+
+
+def main(argv=None):
+    """Command line of reversible/world_dp.py:429-463 (same flags, plus
+    --speed-boundary)."""
+    import argparse
+
+    parser = argparse.ArgumentParser(description=__doc__)
+    parser.add_argument("output")
+    parser.add_argument("--positions", type=int, default=9)
+    parser.add_argument("--headings", type=int, default=8)
+    parser.add_argument("--speeds", type=int, default=7)
+    parser.add_argument("--extent", type=float, default=3.5)
+    parser.add_argument("--horizon", type=float, default=0.5)
+    parser.add_argument("--output-dt", type=float, default=0.25)
+    parser.add_argument("--threads", type=int, default=8)
+    parser.add_argument("--history", action="store_true")
+    parser.add_argument("--order", type=int, choices=[1, 2], default=2)
+    parser.add_argument("--max-core-gb", type=float, default=12)
+    parser.add_argument("--max-output-gb", type=float, default=30)
+    parser.add_argument("--speed-boundary", choices=["linear", "legacy_signed"],
+                        default="linear")
+    a = parser.parse_args(argv)
+    m = solve(
+        ReversibleConfig(),
+        WorldGrid(a.positions, a.headings, a.speeds, a.speeds, a.extent),
+        a.output,
+        horizon=a.horizon,
+        output_dt=a.output_dt,
+        threads=a.threads,
+        history=a.history,
+        order=a.order,
+        max_core_gb=a.max_core_gb,
+        max_output_gb=a.max_output_gb,
+        speed_boundary=a.speed_boundary,
+    )
+    print(json.dumps({k: m[k] for k in ["status", "states", "update_seconds", "peak_rss_bytes",
+                                        "internal_steps"]}, indent=2))
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

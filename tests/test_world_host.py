@@ -149,3 +149,11 @@ def test_global_rate_bound_closed_form():
     2*(2+1)/dx + (2*1 + 1*2)/dh + 2/0.5 + 1/0.25."""
     bound = global_rate_bound(ReversibleConfig(), WorldGrid())
     assert bound == pytest.approx(6 / 0.875 + 4 / (np.pi / 4) + 4 + 4, rel=1e-12)
+
+
+def test_cli_rejects_invalid_order(tmp_path):
+    from pe_amd.reversible.world_dp import main
+
+    with pytest.raises(SystemExit):
+        main([str(tmp_path / "x"), "--order", "3"])
+    assert not (tmp_path / "x").exists()
