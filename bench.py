@@ -138,6 +138,59 @@ def run_world8d(grid_dims, steps: int, warmup: int, cpu_steps: int = 0):
     return out
 
 
+def run_reader_c1(cfg, n_queries: int = 1_000_000, cpu_queries: int = 20_000):
+    """5D table reader on the C1 history: the 31 frames (0, 0.1, ..., 3.0 s)
+    of the C1 solve resident on the device, then batches of seeded queries
+    [x, y, psi, vP, vE] over the grid (field at seeded budgets; first-crossing
+    value).  Rates include the state upload and the result download."""
+    import numpy as np
+    from pe_amd import _native
+
+    config, grid, _ = c1_config()
+    axes = grid.axes(config)
+    times = np.round(np.arange(31) * 0.1, 10)
+    with _native.HJI5D(cfg) as h:
+        hist = np.empty((31, grid.states), np.float32)
+        h.read(hist[0], False)
+        for k in range(1, 31):
+            h.advance_to(times[k])
+            h.read(hist[k], False)
+    t0 = time.perf_counter()
+    table = _native.RelTable(hist, axes, times)
+    upload = time.perf_counter() - t0
+    rng = np.random.default_rng(7)
+    states = np.column_stack([rng.uniform(-8, 8, n_queries), rng.uniform(-8, 8, n_queries),
+                             rng.uniform(-np.pi, np.pi, n_queries),
+                             rng.uniform(-1, 2, n_queries), rng.uniform(-0.5, 1, n_queries)])
+    budgets = rng.uniform(0, 3, n_queries)
+    p, e = config.pursuer, config.evader
+    limits = [p.speed_min, p.speed_max, e.speed_min, e.speed_max]
+    table.field(states[:1000], budgets[:1000])
+    t0 = time.perf_counter()
+    table.field(states, budgets)
+    field_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    values = table.value(states, config.capture_radius, limits)
+    value_s = time.perf_counter() - t0
+    table.close()
+    out = {"history": "C1, 31 frames x 34,222,500 nodes (4.24 GB)", "queries": n_queries,
+           "upload_s": upload, "field_queries_per_s": n_queries / field_s,
+           "value_queries_per_s": n_queries / value_s,
+           "resolved_fraction": float(np.isfinite(values).mean())}
+    if cpu_queries:
+        # cpu_baseline leg: the reader's numpy formulas (oracle restatement),
+        # one thread, on a bounded sample of the same queries.
+        import rtable_oracle
+
+        t0 = time.perf_counter()
+        rtable_oracle.field_np(hist, times, axes, states[:cpu_queries], budgets[:cpu_queries])
+        cpu_s = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": cpu_queries / cpu_s, "unit": "field queries/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"{cpu_queries} field queries, numpy ({cpu_s:.2f} s)"}
+    return out
+
+
 def run_c2(args):
     """8D contact game C2: sweeps of the compact min-max backup."""
     from pe_amd import _native
@@ -356,6 +409,9 @@ def main() -> int:
         secondary["world8d_default"] = run_world8d(
             (9, 8, 7, 7), 40, 2, 0 if args.no_cpu_baseline else 2)
         secondary["world8d_13x12x7x7"] = run_world8d((13, 12, 7, 7), 20, 2)
+        print("[bench] secondary: 5D table reader", file=sys.stderr, flush=True)
+        secondary["reader_c1"] = run_reader_c1(cfg, cpu_queries=0 if args.no_cpu_baseline
+                                               else 20_000)
         out["secondary"] = secondary
     print(json.dumps(out), flush=True)
     return 0

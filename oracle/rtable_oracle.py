@@ -103,3 +103,51 @@ def value(fields, times, axes, physics, states):
         fraction = a / (a - b) if abs(a - b) > 1e-14 else 0.0
         out.append(times[lo] + min(max(fraction, 0.0), 1.0) * (times[hi] - times[lo]))
     return out
+
+
+def stencil_np(states, axes):
+    """Vectorised stencil of many states (numpy, as the reader computes it,
+    relative_table.py:76-109): indices [m, 32], weights [m, 32], valid [m]."""
+    import numpy as np
+
+    s = np.asarray(states, dtype=float).reshape(-1, 5)[:, [3, 4, 2, 1, 0]]
+    valid = np.isfinite(s).all(1)
+    s = np.where(np.isfinite(s), s, 0)
+    shape = [len(a) for a in axes]
+    strides = np.cumprod([1] + shape[:0:-1])[::-1]
+    bits = (np.arange(32)[:, None] >> (4 - np.arange(5))[None, :]) & 1
+    ii = np.zeros((len(s), 32), dtype=np.int64)
+    ww = np.ones((len(s), 32))
+    for k, axis in enumerate(axes):
+        step, n = axis[1] - axis[0], shape[k]
+        if k == 2:
+            f = ((s[:, k] + np.pi) % (2 * np.pi)) / step
+            j = np.floor(f).astype(np.int64) % n
+            w = f - np.floor(f)
+        else:
+            valid &= (s[:, k] >= axis[0] - 1e-10) & (s[:, k] <= axis[-1] + 1e-10)
+            f = (np.clip(s[:, k], axis[0], axis[-1]) - axis[0]) / step
+            j = np.minimum(np.floor(f).astype(np.int64), n - 2)
+            w = f - j
+        node = j[:, None] + bits[None, :, k]
+        if k == 2:
+            node %= n
+        ii += node * strides[k]
+        ww *= np.where(bits[None, :, k], w[:, None], 1 - w[:, None])
+    return ii, ww, valid
+
+
+def field_np(fields, times, axes, states, budgets):
+    """Vectorised field (relative_table.py:111-131); fields [frames, N]."""
+    import numpy as np
+
+    ii, ww, valid = stencil_np(states, axes)
+    times = np.asarray(times, dtype=float)
+    tau = np.asarray(budgets, dtype=float).reshape(-1)
+    valid &= np.isfinite(tau) & (tau >= 0) & (tau <= times[-1] + 1e-10)
+    tau = np.where(np.isfinite(tau), tau, 0)
+    k = np.clip(np.searchsorted(times, tau, side="right") - 1, 0, len(times) - 2)
+    fraction = (tau - times[k]) / (times[k + 1] - times[k])
+    a = (fields[k[:, None], ii] * ww).sum(1)
+    b = (fields[(k + 1)[:, None], ii] * ww).sum(1)
+    return np.where(valid, a + fraction * (b - a), np.nan)

@@ -108,3 +108,16 @@ def test_constructor_checks(tmp_path):
         RelativeTable(folder, replace(c, capture_radius=0.4))
     with pytest.raises(ValueError):
         table.stencil(np.zeros((2, 4)))
+
+
+def test_vectorised_oracle_equals_loop_oracle(tmp_path):
+    table, _ = synthetic_table(tmp_path)
+    fields, times, axes, _ = oracle_parts(table)
+    rng = np.random.default_rng(4)
+    states = np.column_stack([rng.uniform(-2.2, 2.2, 200), rng.uniform(-2.2, 2.2, 200),
+                              rng.uniform(-9, 9, 200), rng.uniform(-1.1, 2.1, 200),
+                              rng.uniform(-0.6, 1.1, 200)])
+    budgets = rng.uniform(-0.1, 2.1, 200)
+    got = ro.field_np(fields, times, axes, states, budgets)
+    want = ro.field(fields, times, axes, states.tolist(), budgets.tolist())
+    np.testing.assert_allclose(got, want, atol=1e-12, rtol=0, equal_nan=True)
