@@ -430,6 +430,17 @@ int pe_world8d_set_probes(pe_world8d *h, const uint64_t *indices,
 int pe_world8d_probe(pe_world8d *h, double *values);
 int pe_world8d_get_stats(pe_world8d *h, pe_world8d_stats *out);
 void pe_world8d_destroy(pe_world8d *h);
+/* Footprint-map mode of the same game (replaces the CuPy RawKernels
+ * init/advance of reversible/gpu_map_game.py:14-75): clearance grids
+ * [positions, positions, headings] over (x, y, heading) of each car's
+ * footprint against the map; target = min(capture, E clearance), avoid =
+ * -P clearance + 1e-6, heading spacing float 2 pi / headings; the field is
+ * re-initialised at time 0.  Needs the linear speed boundary. */
+int pe_world8d_set_clearance(pe_world8d *h, const float *p_clearance,
+                             const float *e_clearance);
+/* Replace the CFL step limit (the map pilot takes cfl / global_rate_bound,
+ * gpu_map_game.py:239). */
+int pe_world8d_set_step_limit(pe_world8d *h, double max_dt);
 /* Self-check of the kernels' division a / b = correction steps on RN(1/b)
  * (used while every field value is +0 or of magnitude in [2^-60, 2^60])
  * against the IEEE division, for every float a with exponent in [lo, hi)

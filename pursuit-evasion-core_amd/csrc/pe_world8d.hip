@@ -197,7 +197,8 @@ struct W8Slab {
 __device__ __forceinline__ W8Slab w8_slab(long long s, const W8 &g, const float *__restrict__ x,
                                           const float *__restrict__ cs,
                                           const float *__restrict__ sn,
-                                          const float *__restrict__ pv) {
+                                          const float *__restrict__ pv,
+                                          const float *__restrict__ p_clearance) {
   W8Slab p;
   p.q[3] = (int)(s % g.n[3]);
   s /= g.n[3];
@@ -211,13 +212,19 @@ __device__ __forceinline__ W8Slab w8_slab(long long s, const W8 &g, const float 
   p.fpx = vp * cs[p.q[2]];
   p.fpy = vp * sn[p.q[2]];
   p.wp = fabsf(vp) * g.kp;
-  p.avoid = w8_mx(fabsf(x[p.q[0]]), fabsf(x[p.q[1]])) - g.L + 1e-6f;
+  p.avoid = p_clearance
+                ? -p_clearance[((long long)p.q[0] * g.n[1] + p.q[1]) * g.n[2] + p.q[2]] + 1e-6f
+                : w8_mx(fabsf(x[p.q[0]]), fabsf(x[p.q[1]])) - g.L + 1e-6f;
   return p;
 }
 
 // Per-launch arguments beside the fields.
 struct W8Launch {
   const float *x, *cs, *sn, *pv, *ev;  // tables
+  // Footprint-map mode: pursuer clearance [n, n, nh] over (xP, yP, thP); the
+  // avoid value is -clearance + 1e-6 (gpu_map_game.py:72-73).  Null: the
+  // pursuer wall fault of the square.
+  const float *p_clearance;
   float dt;
   double tau;
   // Range flags (see w8_div): the launch that wrote src recorded its id in
@@ -308,7 +315,7 @@ __global__ __launch_bounds__(256) void world8d_stage(const float *__restrict__ s
   const long long slab = b / g.inner_blocks;  // (xP, yP, thP, vP), uniform
   const int i = (int)(b - slab * g.inner_blocks) * 256 + (int)threadIdx.x;
   if (i >= g.inner) return;
-  const W8Slab p = w8_slab(slab, g, L.x, L.cs, L.sn, L.pv);
+  const W8Slab p = w8_slab(slab, g, L.x, L.cs, L.sn, L.pv, L.p_clearance);
   const I j = (I)(slab * g.inner + i);
   const W8Global<I> v{src};
   const bool ok = w8_fast(g, L)
@@ -331,7 +338,7 @@ __global__ __launch_bounds__(THREADS) void world8d_stage_slab(const float *__res
   const float *__restrict__ row = src + slab * g.inner;
   for (int i = threadIdx.x; i < g.inner; i += THREADS) slab_values[i] = row[i];
   __syncthreads();
-  const W8Slab p = w8_slab(slab, g, L.x, L.cs, L.sn, L.pv);
+  const W8Slab p = w8_slab(slab, g, L.x, L.cs, L.sn, L.pv, L.p_clearance);
   const W8Global<I> outer{src};
   const W8Lds inner_v{slab_values};
   const bool fast = w8_fast(g, L);
@@ -432,6 +439,7 @@ struct pe_world8d {
   W8 g{};
   float *a = nullptr, *b = nullptr, *arrival = nullptr;
   float *tables = nullptr;  // x | cs | sn | pv | ev
+  float *p_clearance = nullptr;  // footprint-map mode (pe_world8d_set_clearance)
   unsigned long long *count = nullptr;   // non-finite counter
   unsigned long long *probe_index = nullptr;
   double *probe_weight = nullptr, *probe_out = nullptr;
@@ -641,6 +649,7 @@ static int world8d_launch(pe_world8d *h, const float *src, float *dst, const flo
   L.sn = L.cs + g.n[2];
   L.pv = L.sn + g.n[2];
   L.ev = L.pv + g.n[3];
+  L.p_clearance = h->p_clearance;
   L.dt = (float)dt;
   L.tau = h->tau;
   const bool from_a = src == h->a;
@@ -794,6 +803,68 @@ extern "C" int pe_world8d_read(pe_world8d *h, float *field, float *arrival) {
   return PE_OK;
 }
 
+extern "C" int pe_world8d_set_clearance(pe_world8d *h, const float *p_clearance,
+                                        const float *e_clearance) {
+  clear_error();
+  if (!h || !p_clearance || !e_clearance) return fail(PE_ERR_INVALID, "invalid argument");
+  if (!h->cfg.linear_speed_boundary)
+    return fail(PE_ERR_INVALID, "the footprint-map game uses the linear speed boundary");
+  W8 &g = h->g;
+  const int n = g.n[0], nh = g.n[2];
+  const size_t poses = (size_t)n * n * nh;
+  for (size_t k = 0; k < poses; k++)
+    if (!std::isfinite(p_clearance[k]) || !std::isfinite(e_clearance[k]))
+      return fail(PE_ERR_INVALID, "clearance grids must be finite");
+  PE_HIP(hipSetDevice(h->device));
+  PE_HIP(hipStreamSynchronize(h->stream));
+  // Heading spacing as the map kernel forms it: float 2 pi / nh
+  // (gpu_map_game.py:58).
+  const float dh = 6.2831853071795864769f / nh;
+  for (int k : {2, 6}) {
+    g.h[k] = dh;
+    g.hh[k] = dh * dh;
+    g.rh[k] = recip_nearest(g.h[k]);
+    g.rhh[k] = recip_nearest(g.hh[k]);
+  }
+  if (!h->p_clearance) PE_HIP(hipMalloc(&h->p_clearance, sizeof(float) * poses));
+  PE_HIP(hipMemcpy(h->p_clearance, p_clearance, sizeof(float) * poses, hipMemcpyHostToDevice));
+  // Initial field max(min(capture, e), -p + 1e-6) (gpu_map_game.py:43-49),
+  // capture = hypot((iP - iE) dx, (jP - jE) dx) - radius.
+  const float extent = (float)h->cfg.extent, radius = (float)h->cfg.radius;
+  const float dx = 2 * extent / (n - 1);
+  std::vector<float> field((size_t)g.N), arrival((size_t)g.N);
+  const long long inner = (long long)g.n[6] * g.n[7];
+  for (long long block = 0; block < g.N / inner; block++) {
+    const long long begin = block * inner;
+    int q[8];
+    for (int k = 0; k < 6; k++) q[k] = (int)((begin / g.stride[k]) % g.n[k]);
+    const float cap = std::hypot((float)(q[0] - q[4]) * dx, (float)(q[1] - q[5]) * dx) - radius;
+    const float pc = p_clearance[((size_t)q[0] * n + q[1]) * nh + q[2]];
+    for (int th = 0; th < g.n[6]; th++) {
+      const float ec = e_clearance[((size_t)q[4] * n + q[5]) * nh + th];
+      const float v = std::fmax(std::fmin(cap, ec), -pc + 1e-6f);
+      for (int iv = 0; iv < g.n[7]; iv++) {
+        field[begin + (long long)th * g.n[7] + iv] = v;
+        arrival[begin + (long long)th * g.n[7] + iv] =
+            v <= 0 ? 0.f : std::numeric_limits<float>::quiet_NaN();
+      }
+    }
+  }
+  const size_t bytes = sizeof(float) * (size_t)g.N;
+  PE_HIP(hipMemcpy(h->a, field.data(), bytes, hipMemcpyHostToDevice));
+  PE_HIP(hipMemcpy(h->arrival, arrival.data(), bytes, hipMemcpyHostToDevice));
+  h->tau = 0;
+  return world8d_uploaded(h, field.data());
+}
+
+extern "C" int pe_world8d_set_step_limit(pe_world8d *h, double max_dt) {
+  clear_error();
+  if (!h || !(max_dt > 0) || !std::isfinite(max_dt))
+    return fail(PE_ERR_INVALID, "invalid argument");
+  h->maxdt = max_dt;
+  return PE_OK;
+}
+
 extern "C" int pe_world8d_check_division(const float *divisors, int32_t count, int32_t lo,
                                          int32_t hi, int64_t *mismatches) {
   clear_error();
@@ -842,7 +913,7 @@ extern "C" void pe_world8d_destroy(pe_world8d *h) {
   if (h->stream) hipStreamSynchronize(h->stream);
   for (void *p : {(void *)h->a, (void *)h->b, (void *)h->arrival, (void *)h->tables,
                   (void *)h->count, (void *)h->probe_index, (void *)h->probe_weight,
-                  (void *)h->probe_out, (void *)h->flags})
+                  (void *)h->probe_out, (void *)h->flags, (void *)h->p_clearance})
     if (p) hipFree(p);
   if (h->t0) hipEventDestroy(h->t0);
   if (h->t1) hipEventDestroy(h->t1);

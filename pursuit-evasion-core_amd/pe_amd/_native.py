@@ -290,6 +290,8 @@ SIGNATURES = {
     "pe_world8d_get_stats": (ctypes.c_int, [P, ctypes.POINTER(World8DStats)]),
     "pe_world8d_destroy": (None, [P]),
     "pe_world8d_check_division": (ctypes.c_int, [P, I32, I32, I32, P]),
+    "pe_world8d_set_clearance": (ctypes.c_int, [P, P, P]),
+    "pe_world8d_set_step_limit": (ctypes.c_int, [P, F64]),
     "pe_rtable_create": (ctypes.c_int, [P, I32, P, P, P, P, P, P]),
     "pe_rtable_field": (ctypes.c_int, [P, P, P, I64, P]),
     "pe_rtable_value": (ctypes.c_int, [P, P, I64, F64, P, P]),
@@ -607,6 +609,18 @@ class World8D:
         c = ctypes.c_int64()
         check(lib().pe_world8d_count_nonfinite(self._h, ctypes.byref(c)))
         return c.value
+
+    def set_clearance(self, p_clearance, e_clearance):
+        """Footprint-map mode: clearance grids [n, n, nh] of each car."""
+        pc = np.ascontiguousarray(p_clearance, np.float32).reshape(-1)
+        ec = np.ascontiguousarray(e_clearance, np.float32).reshape(-1)
+        c = self.cfg
+        if pc.size != c.positions ** 2 * c.headings or ec.size != pc.size:
+            raise ValueError("clearance grids must be [positions, positions, headings]")
+        check(lib().pe_world8d_set_clearance(self._h, ptr(pc), ptr(ec)))
+
+    def set_step_limit(self, max_dt):
+        check(lib().pe_world8d_set_step_limit(self._h, float(max_dt)))
 
     def set_probes(self, indices, weights):
         ii = np.ascontiguousarray(indices, np.uint64).reshape(-1, 256)

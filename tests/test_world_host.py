@@ -157,3 +157,63 @@ def test_cli_rejects_invalid_order(tmp_path):
     with pytest.raises(SystemExit):
         main([str(tmp_path / "x"), "--order", "3"])
     assert not (tmp_path / "x").exists()
+
+
+def test_footprint_heading_and_player_size_change_collision():
+    """(tests/test_footprint_map.py:10-18)"""
+    from pe_amd.reversible.footprint_map import FootprintMap
+
+    m = FootprintMap(2.5)
+    assert m.clearance([2.2, 0, 0], "pursuer") < 0
+    assert m.clearance([2.2, 0, np.pi / 2], "pursuer") > 0
+    assert m.clearance([2.2, 0, 0], "evader") > 0
+    corridor = FootprintMap(2.5, ((-2.5, 2.5, 0.3, 2.5), (-2.5, 2.5, -2.5, -0.3)))
+    assert corridor.clearance([0, 0, 0], "pursuer") > 0
+    assert corridor.clearance([0, 0, np.pi / 2], "pursuer") < 0
+    assert corridor.clearance([0, 0, np.pi / 2], "evader") > 0
+
+
+def test_joint_quarter_rotation_preserves_geometry_and_event_order():
+    """(tests/test_footprint_map.py:21-40)"""
+    from pe_amd.reversible.footprint_map import FootprintMap
+
+    m = FootprintMap(2.5, ((-0.7, -0.4, -1, 0.5), (0.6, 1.1, 0.8, 1.7)))
+    r = FootprintMap(2.5, tuple((-b[3], -b[2], b[0], b[1]) for b in m.obstacles))
+    rng = np.random.default_rng(5)
+    s = rng.uniform(-2, 2, (100, 4))
+    s[:, 2] = rng.uniform(-np.pi, np.pi, 100)
+    rotated = s.copy()
+    rotated[:, 0], rotated[:, 1] = -s[:, 1], s[:, 0]
+    rotated[:, 2] += np.pi / 2
+    for role in ("pursuer", "evader"):
+        np.testing.assert_allclose(m.clearance(s, role), r.clearance(rotated, role), atol=2e-15)
+    assert m.event(np.array([2.2, 0, 0, 0, 2.25, 0, 0, 0]), 0.5) == "pursuer_collision"
+    assert m.event(np.array([0, 0, 0, 0, 0.4, 0, 0, 0]), 0.5) == "capture"
+
+
+def test_footprint_clearance_closed_forms():
+    """An unrotated body against the room: clearance = half_width - |x| -
+    length/2; a box straight ahead: the gap along x."""
+    from pe_amd.reversible.footprint_map import FootprintMap
+
+    m = FootprintMap(2.5, ((1.0, 1.5, -0.2, 0.2),))
+    assert m.clearance([-1.0, 0, 0], "pursuer") == pytest.approx(2.5 - 1.0 - 0.4)
+    assert m.clearance([0.2, 0, 0], "pursuer") == pytest.approx(1.0 - 0.2 - 0.4)
+    assert m.clearance([0.2, 0, np.pi], "evader") == pytest.approx(1.0 - 0.2 - 0.2)
+
+
+def test_map_game_guards_before_allocation(tmp_path):
+    """(tests/test_footprint_map.py:106-118, 142-158)"""
+    from dataclasses import replace
+
+    from pe_amd.reversible.footprint_map import FootprintMap
+    from pe_amd.reversible.gpu_map_game import solve
+
+    with pytest.raises(MemoryError):
+        solve(replace(ReversibleConfig(), point_vehicles=False), WorldGrid(21, 16, 7, 7, 3.5),
+              FootprintMap(3), tmp_path / "huge", max_working_gb=0.001)
+    with pytest.raises(ValueError, match="point_vehicles=False"):
+        solve(ReversibleConfig(), WorldGrid(), FootprintMap(), tmp_path / "unused")
+    with pytest.raises(TypeError):
+        solve(replace(ReversibleConfig(), point_vehicles=False), WorldGrid(), object(),
+              tmp_path / "unused2")
