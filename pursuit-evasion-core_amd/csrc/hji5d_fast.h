@@ -20,7 +20,13 @@
 // xy tile with its halo rows in LDS (eight x/y neighbours from LDS, about two
 // global loads per node for the tile) was bit-identical and much slower
 // (228 / 202 us): the block barrier per vP node stops the waves from running
-// ahead of one another, which is what hides the load latency here.
+// ahead of one another, which is what hides the load latency here.  Staging
+// a block's whole vP range of a few y rows once, behind a single barrier,
+// was also bit-identical and slower (233 / 201 us with 512 threads, 234 / 207
+// with 256).  A load-only kernel with these 17 loads per node takes 129 us on
+// C1, and dwordx2 loads of the same bytes (half the instructions) 158 us:
+// the load path is bound by bytes, and the ~130 VALU per node add ~113 us
+// of issue time, so the stage sits near both floors.
 //
 // Same scheme as hji5d_stage<float, false, *> node by node
 // (reversible/native_relative.cpp:38-67 slopes, :124-172 update), with the
