@@ -31,12 +31,21 @@
 
 namespace pe {
 
+// Axes in spacing classes: positions (0, 1, 4, 5), headings (2, 6), P speed
+// (3), E speed (7).  Per-class constants keep the kernels' uniform values
+// (and their scalar registers) few.
+__host__ __device__ constexpr int w8_cls(int k) {
+  return (k == 2 || k == 6) ? 1 : k == 3 ? 2 : k == 7 ? 3 : 0;
+}
+
 struct W8 {
   int n[8];              // axis lengths
   long long stride[8];   // element strides
-  float h[8];            // spacings
-  float hh[8];           // h * h (float, as the scheme forms it)
-  float rh[8], rhh[8];   // reciprocals of h and hh rounded to nearest
+  int s32[8];            // the same as 32-bit values (fields below 2^31 nodes)
+  // per spacing class: h, h * h (float, as the scheme forms it), their
+  // reciprocals rounded to nearest and the remainders (r + rl = 1/h to about
+  // 2^-48)
+  float h[4], hh[4], rh[4], rhh[4], rhl[4], rhhl[4];
   long long N;           // nodes
   int inner;             // nodes of one pursuer slab: n * n * nh * ne
   int inner_blocks;      // ceil(inner / 256)
@@ -53,24 +62,24 @@ __device__ __forceinline__ float w8_mn(float a, float b) { return b < a ? b : a;
 __device__ __forceinline__ float w8_mx(float a, float b) { return a < b ? b : a; }
 
 // a / b, correctly rounded.  FAST: b > 0 is one of the grid spacings or
-// their squares and y = RN(1/b) (host-computed).  q0 = RN(a y) lies within
-// 1.5 ulp of a/b; one correction q1 = RN(q0 + (a - b q0) y) brings it within
-// one ulp; then, with y within half an ulp of 1/b and q1 within one ulp of
-// a/b, the remainder a - b q1 is exact and RN(q1 + (a - b q1) y) is the
-// correctly rounded quotient (Markstein's theorem, P. Markstein, IBM J. Res.
-// Dev. 34(1), 1990).  The theorem needs a normal quotient and an exact
-// remainder: the kernels take this path only while every value of the field
-// they read is +0 or finite with magnitude in [2^-60, 2^60], so that every
-// non-zero numerator (a difference, second difference or extrapolation of
-// such values) lies in [2^-83, 2^64]; a numerator +0 gives +0 as a / b does
-// and -0 never occurs (the flag counts -0 as out of range).  Otherwise the
-// IEEE division is used.
+// their squares, y = RN(1/b) and y + ylo = 1/b to about 2^-48 relative
+// (both host-computed).  q0 = RN(a y + RN(a ylo)) lies within half an ulp
+// of a/b plus a term of relative size 2^-47, so within one ulp; with y
+// within half an ulp of 1/b and q0 within one ulp of a/b, the remainder
+// a - b q0 is exact and RN(q0 + (a - b q0) y) is the correctly rounded
+// quotient (Markstein's theorem, P. Markstein, IBM J. Res. Dev. 34(1),
+// 1990).  The theorem needs a normal quotient and an exact remainder: the
+// kernels take this path only while every value of the field they read is
+// +0 or finite with magnitude in [2^-60, 2^60], so that every non-zero
+// numerator (a difference, second difference or extrapolation of such
+// values) lies in [2^-83, 2^64]; a numerator +0 gives +0 as a / b does and
+// -0 never occurs (the flag counts -0 as out of range).  Otherwise the IEEE
+// division is used.
 template <bool FAST>
-__device__ __forceinline__ float w8_div(float a, float b, float y) {
+__device__ __forceinline__ float w8_div(float a, float b, float y, float ylo) {
   if (!FAST) return a / b;
-  const float q0 = a * y;
-  const float q1 = fmaf(fmaf(-q0, b, a), y, q0);
-  return fmaf(fmaf(-q1, b, a), y, q1);
+  const float q0 = fmaf(a, y, a * ylo);
+  return fmaf(fmaf(-q0, b, a), y, q0);
 }
 
 // Whether a value keeps the next stage on the fast division (see w8_div).
@@ -128,31 +137,32 @@ template <bool FAST, typename I, typename V>
 __device__ __forceinline__ void w8_slopes(const V &v, I j, float f, int k, int pos,
                                           const W8 &g, bool periodic, bool linear, float &left,
                                           float &right) {
-  const I stride = (I)g.stride[k];
+  const I stride = sizeof(I) == 4 ? (I)g.s32[k] : (I)g.stride[k];
   const int n = g.n[k];
-  const float h = g.h[k], rh = g.rh[k];
+  const int c = w8_cls(k);
+  const float h = g.h[c], rh = g.rh[c], rhl = g.rhl[c];
   if (linear && pos == 0) {
-    left = right = w8_div<FAST>(v(j + stride) - f, h, rh);
+    left = right = w8_div<FAST>(v(j + stride) - f, h, rh, rhl);
     return;
   }
   if (linear && pos == n - 1) {
-    left = right = w8_div<FAST>(f - v(j - stride), h, rh);
+    left = right = w8_div<FAST>(f - v(j - stride), h, rh, rhl);
     return;
   }
   const float fm = w8_at(v, j, stride, pos, n, -1, periodic),
               fp = w8_at(v, j, stride, pos, n, 1, periodic);
-  float l = w8_div<FAST>(f - fm, h, rh), r = w8_div<FAST>(fp - f, h, rh);
+  float l = w8_div<FAST>(f - fm, h, rh, rhl), r = w8_div<FAST>(fp - f, h, rh, rhl);
   if (g.order == 2) {
-    const float hh = g.hh[k], rhh = g.rhh[k];
-    const float dc = w8_div<FAST>(fp - 2 * f + fm, hh, rhh);
+    const float hh = g.hh[c], rhh = g.rhh[c], rhhl = g.rhhl[c];
+    const float dc = w8_div<FAST>(fp - 2 * f + fm, hh, rhh, rhhl);
     const float dl = linear && pos < 2
                          ? dc
                          : w8_div<FAST>(f - 2 * fm + w8_at(v, j, stride, pos, n, -2, periodic),
-                                        hh, rhh);
+                                        hh, rhh, rhhl);
     const float dr = linear && pos + 2 >= n
                          ? dc
                          : w8_div<FAST>(w8_at(v, j, stride, pos, n, 2, periodic) - 2 * fp + f,
-                                        hh, rhh);
+                                        hh, rhh, rhhl);
     l += .5f * h * (fabsf(dl) <= fabsf(dc) ? dl : dc);
     r -= .5f * h * (fabsf(dc) <= fabsf(dr) ? dc : dr);
   }
@@ -169,18 +179,19 @@ __device__ __forceinline__ void w8_slopes(const V &v, I j, float f, int k, int p
 template <bool FAST, typename I, typename V>
 __device__ __forceinline__ void w8_slopes_speed(const V &v, I j, float f, int k, int pos,
                                                 const W8 &g, float &left, float &right) {
-  const I s = (I)g.stride[k];
+  const I s = sizeof(I) == 4 ? (I)g.s32[k] : (I)g.stride[k];
   const int n = g.n[k];
-  const float h = g.h[k], rh = g.rh[k];
+  const int c = w8_cls(k);
+  const float h = g.h[c], rh = g.rh[c], rhl = g.rhl[c];
   const float fm = pos > 0 ? v(j - s) : f, fp = pos < n - 1 ? v(j + s) : f;
-  const float l0 = w8_div<FAST>(f - fm, h, rh), r0 = w8_div<FAST>(fp - f, h, rh);
+  const float l0 = w8_div<FAST>(f - fm, h, rh, rhl), r0 = w8_div<FAST>(fp - f, h, rh, rhl);
   float l = l0, r = r0;
   if (g.order == 2) {
-    const float hh = g.hh[k], rhh = g.rhh[k];
+    const float hh = g.hh[c], rhh = g.rhh[c], rhhl = g.rhhl[c];
     const float fm2 = pos >= 2 ? v(j - 2 * s) : f, fp2 = pos + 2 < n ? v(j + 2 * s) : f;
-    const float dc = w8_div<FAST>(fp - 2 * f + fm, hh, rhh);
-    const float dl = pos < 2 ? dc : w8_div<FAST>(f - 2 * fm + fm2, hh, rhh);
-    const float dr = pos + 2 >= n ? dc : w8_div<FAST>(fp2 - 2 * fp + f, hh, rhh);
+    const float dc = w8_div<FAST>(fp - 2 * f + fm, hh, rhh, rhhl);
+    const float dl = pos < 2 ? dc : w8_div<FAST>(f - 2 * fm + fm2, hh, rhh, rhhl);
+    const float dr = pos + 2 >= n ? dc : w8_div<FAST>(fp2 - 2 * fp + f, hh, rhh, rhhl);
     l += .5f * h * (fabsf(dl) <= fabsf(dc) ? dl : dc);
     r -= .5f * h * (fabsf(dc) <= fabsf(dr) ? dc : dr);
   }
@@ -381,7 +392,8 @@ __global__ __launch_bounds__(256) void world8d_probe(const float *__restrict__ v
 
 // Self-check of w8_div<true> against the IEEE division: every float a of
 // either sign with exponent in [lo, hi) (and +0) divided by b.
-__global__ __launch_bounds__(256) void world8d_division_check(float b, float y, int lo, int hi,
+__global__ __launch_bounds__(256) void world8d_division_check(float b, float y, float ylo, int lo,
+                                                              int hi,
                                                               unsigned long long *mismatches) {
   const unsigned long long span = (unsigned long long)(hi - lo) << 23;
   const unsigned long long total = 2 * span + 1;
@@ -394,7 +406,7 @@ __global__ __launch_bounds__(256) void world8d_division_check(float b, float y, 
       bits = (unsigned)(((unsigned long long)(lo + 127) << 23) + u) | (t >= span ? 0x80000000u : 0u);
     }
     const float a = __uint_as_float(bits);
-    bad += __float_as_uint(w8_div<true>(a, b, y)) != __float_as_uint(a / b);
+    bad += __float_as_uint(w8_div<true>(a, b, y, ylo)) != __float_as_uint(a / b);
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) bad += __shfl_xor(bad, off, 64);
@@ -419,6 +431,28 @@ static float recip_nearest(float b) {
     }
   }
   return best;
+}
+
+// 1/b - y rounded to float (y = RN(1/b)): 1 - b y is exact in double, so
+// the remainder carries 1/b to about 2^-48 relative.
+static float recip_remainder(float b, float y) {
+  return (float)((1.0 - (double)b * (double)y) / (double)b);
+}
+
+// Per-class spacing constants (class c takes the spacing of its first axis).
+static void world8d_set_spacings(W8 &g, const float spacing[8]) {
+  g.fast_division = 1;
+  for (int k : {0, 2, 3, 7}) {
+    const int c = w8_cls(k);
+    g.h[c] = spacing[k];
+    g.hh[c] = spacing[k] * spacing[k];
+    g.rh[c] = recip_nearest(g.h[c]);
+    g.rhh[c] = recip_nearest(g.hh[c]);
+    g.rhl[c] = recip_remainder(g.h[c], g.rh[c]);
+    g.rhhl[c] = recip_remainder(g.hh[c], g.rhh[c]);
+    if (!(g.h[c] >= 0x1p-20f && g.h[c] <= 0x1p20f && g.hh[c] >= 0x1p-20f && g.hh[c] <= 0x1p20f))
+      g.fast_division = 0;
+  }
 }
 
 // Whether a field keeps the stages on the fast division (w8_in_range).
@@ -470,16 +504,11 @@ static void world8d_setup(pe_world8d *h) {
   const float spacing[8] = {dx, dx, dh, dvp, dx, dx, dh, dve};
   g.stride[7] = 1;
   for (int k = 6; k >= 0; k--) g.stride[k] = g.stride[k + 1] * shape[k + 1];
-  g.fast_division = 1;
   for (int k = 0; k < 8; k++) {
     g.n[k] = shape[k];
-    g.h[k] = spacing[k];
-    g.hh[k] = spacing[k] * spacing[k];
-    g.rh[k] = recip_nearest(g.h[k]);
-    g.rhh[k] = recip_nearest(g.hh[k]);
-    if (!(g.h[k] >= 0x1p-20f && g.h[k] <= 0x1p20f && g.hh[k] >= 0x1p-20f && g.hh[k] <= 0x1p20f))
-      g.fast_division = 0;
+    g.s32[k] = (int)std::min<long long>(g.stride[k], INT32_MAX);
   }
+  world8d_set_spacings(g, spacing);
   g.N = g.stride[0] * n;
   g.inner = n * n * nh * ne;
   g.inner_blocks = (g.inner + 255) / 256;
@@ -820,12 +849,8 @@ extern "C" int pe_world8d_set_clearance(pe_world8d *h, const float *p_clearance,
   // Heading spacing as the map kernel forms it: float 2 pi / nh
   // (gpu_map_game.py:58).
   const float dh = 6.2831853071795864769f / nh;
-  for (int k : {2, 6}) {
-    g.h[k] = dh;
-    g.hh[k] = dh * dh;
-    g.rh[k] = recip_nearest(g.h[k]);
-    g.rhh[k] = recip_nearest(g.hh[k]);
-  }
+  const float spacing[8] = {g.h[0], g.h[0], dh, g.h[2], g.h[0], g.h[0], dh, g.h[3]};
+  world8d_set_spacings(g, spacing);
   if (!h->p_clearance) PE_HIP(hipMalloc(&h->p_clearance, sizeof(float) * poses));
   PE_HIP(hipMemcpy(h->p_clearance, p_clearance, sizeof(float) * poses, hipMemcpyHostToDevice));
   // Initial field max(min(capture, e), -p + 1e-6) (gpu_map_game.py:43-49),
@@ -881,8 +906,9 @@ extern "C" int pe_world8d_check_division(const float *divisors, int32_t count, i
   for (int32_t k = 0; k < count; k++) {
     unsigned long long bad = 0;
     if (hipMemset(d, 0, 8) != hipSuccess) break;
-    hipLaunchKernelGGL(world8d_division_check, dim3(4096), dim3(256), 0, 0, divisors[k],
-                       recip_nearest(divisors[k]), (int)lo, (int)hi, d);
+    const float y = recip_nearest(divisors[k]);
+    hipLaunchKernelGGL(world8d_division_check, dim3(4096), dim3(256), 0, 0, divisors[k], y,
+                       recip_remainder(divisors[k], y), (int)lo, (int)hi, d);
     if (hipMemcpy(&bad, d, 8, hipMemcpyDeviceToHost) != hipSuccess) break;
     mismatches[k] = (int64_t)bad;
   }
