@@ -382,9 +382,9 @@ typedef struct pe_world8d_config {
 
 /* pe_world8d_config.flags (comparison runs; results are identical).  The
    default stages each pursuer slab of evader states in LDS, one block of
-   512 (slab <= 36 KiB) or 1024 threads per slab, and uses one thread per
-   node for slabs above 64 KiB.  GENERIC: one thread per node everywhere;
-   SLAB_256 / SLAB_1024: that many threads per slab block. */
+   512 threads per slab (or per range of xE rows of a slab above 36 KiB).
+   GENERIC: one thread per node; SLAB_256 / SLAB_1024: that many threads
+   per slab block. */
 #define PE_WORLD8D_GENERIC 1
 #define PE_WORLD8D_SLAB_256 2
 #define PE_WORLD8D_SLAB_1024 4

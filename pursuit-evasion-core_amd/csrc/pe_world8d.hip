@@ -17,10 +17,11 @@
 // The initial field (capture or evader wall fault, max with the avoid
 // surface) is computed on the host with the same float operations.
 //
-// Layout of a launch: one block per (xP, yP, thP, vP) slab, whose inner
-// (xE, yE, thE, vE) values are staged in LDS, so the pursuer coordinates and
-// everything derived from them are uniform per block and the evader-axis
-// neighbours are LDS reads.  Slabs above 64 KiB use one thread per node.
+// Layout of a launch: one block per (xP, yP, thP, vP) slab, or per range of
+// xE rows of it when the slab is above 36 KiB, whose inner (xE, yE, thE, vE)
+// values (plus two xE rows on either side) are staged in LDS, so the pursuer
+// coordinates and everything derived from them are uniform per block and
+// the evader-axis neighbours are LDS reads.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -56,6 +57,8 @@ struct W8 {
   unsigned magic[8];
   int fast_decode;
   int fast_division;     // divisors within [2^-20, 2^20]: see w8_div
+  int part_rows;         // xE rows per block of the LDS kernel
+  int slab_parts;        // blocks per slab: ceil(n / part_rows)
 };
 
 __device__ __forceinline__ float w8_mn(float a, float b) { return b < a ? b : a; }
@@ -103,7 +106,7 @@ struct W8Global {
   }
 };
 struct W8Lds {
-  const float *p;
+  const float *p;  // LDS values by inner index (offset by the staged rows)
   __device__ __forceinline__ float operator()(int i) const { return p[i]; }
 };
 
@@ -335,32 +338,43 @@ __global__ __launch_bounds__(256) void world8d_stage(const float *__restrict__ s
   if (!ok) atomicMax(L.dst_flag, L.id);
 }
 
-// The same stage with one block per slab: the slab's inner (xE, yE, thE, vE)
-// values are staged in LDS, so the evader-axis neighbours are LDS reads and
-// only the pursuer-axis neighbours (block-uniform boundary cases) are global
-// loads.
-template <int THREADS, typename I>
+// The same stage with one block per part of a slab: the block's evader
+// states (a range of xE rows of the slab) and the two xE rows on either side
+// are staged in LDS, so the evader-axis neighbours are LDS reads and only
+// the pursuer-axis neighbours (block-uniform boundary cases) are global
+// loads.  A node's xE neighbours, and the rows 0-1 / n-2, n-1 its xE
+// extrapolation reads, lie in the staged rows.
+template <int THREADS, typename I, bool SPLIT>
 __global__ __launch_bounds__(THREADS) void world8d_stage_slab(const float *__restrict__ src,
                                                               float *dst, const float *base,
                                                               float *__restrict__ arrival, W8 g,
                                                               W8Launch L) {
   extern __shared__ float slab_values[];
-  const long long slab = blockIdx.x;
-  const float *__restrict__ row = src + slab * g.inner;
-  for (int i = threadIdx.x; i < g.inner; i += THREADS) slab_values[i] = row[i];
+  const long long slab = SPLIT ? blockIdx.x / g.slab_parts : blockIdx.x;
+  const int part = SPLIT ? (int)(blockIdx.x - slab * g.slab_parts) : 0;
+  const int row = (int)g.stride[4];  // evader states of one xE row
+  const int x0 = SPLIT ? part * g.part_rows : 0;
+  const int x1 = SPLIT ? min(g.n[4], x0 + g.part_rows) : g.n[4];
+  const int w0 = SPLIT ? max(0, x0 - 2) : 0, w1 = SPLIT ? min(g.n[4], x1 + 2) : g.n[4];
+  const int lo = w0 * row, hi = w1 * row;  // staged inner indices
+  const float *__restrict__ vals = src + slab * g.inner;
+  for (int i = lo + (int)threadIdx.x; i < hi; i += THREADS) slab_values[i - lo] = vals[i];
   __syncthreads();
   const W8Slab p = w8_slab(slab, g, L.x, L.cs, L.sn, L.pv, L.p_clearance);
   const W8Global<I> outer{src};
-  const W8Lds inner_v{slab_values};
+  // Indexed by inner index: the first staged value is inner index lo.
+  const float *staged = slab_values - lo;
+  const W8Lds inner_v{staged};
   const bool fast = w8_fast(g, L);
   bool ok = true;
+  const int end = x1 * row;
 #pragma unroll 1
-  for (int i = threadIdx.x; i < g.inner; i += THREADS) {
+  for (int i = x0 * row + (int)threadIdx.x; i < end; i += THREADS) {
     const I j = (I)(slab * g.inner + i);
-    ok &= fast ? w8_node<true>(outer, inner_v, j, i, slab_values[i], p, i, dst, base, arrival,
-                               g, L)
-               : w8_node<false>(outer, inner_v, j, i, slab_values[i], p, i, dst, base,
-                                arrival, g, L);
+    ok &= fast ? w8_node<true>(outer, inner_v, j, i, staged[i], p, i, dst, base, arrival, g,
+                               L)
+               : w8_node<false>(outer, inner_v, j, i, staged[i], p, i, dst, base, arrival, g,
+                                L);
   }
   if (!ok) atomicMax(L.dst_flag, L.id);
 }
@@ -479,6 +493,7 @@ struct pe_world8d {
   double *probe_weight = nullptr, *probe_out = nullptr;
   long long probes = 0;
   int variant = 0;  // 0: thread per node; else threads per LDS slab block
+  size_t lds_bytes = 0;  // LDS of one slab block
   // Range flags of the two field buffers and the id of the launch (or
   // upload) that last wrote each; ids increase.
   unsigned long long *flags = nullptr, *a_flag = nullptr, *b_flag = nullptr;
@@ -603,15 +618,28 @@ extern "C" int pe_world8d_create(const pe_world8d_config *cfg, pe_world8d **out)
   h->device = dev;
   h->cfg = c;
   world8d_setup(h);
-  // The slab kernel stages a slab in LDS (at most 64 KiB per block).  512
-  // threads per block up to 36 KiB slabs (four or more blocks per CU), 1024
-  // above, where LDS allows two blocks per CU.
-  const size_t slab_bytes = sizeof(float) * (size_t)h->g.inner;
-  if (!(c.flags & PE_WORLD8D_GENERIC) && slab_bytes <= 65536)
-    h->variant = (c.flags & PE_WORLD8D_SLAB_256)    ? 256
-                 : (c.flags & PE_WORLD8D_SLAB_1024) ? 1024
-                 : slab_bytes <= 36864              ? 512
-                                                    : 1024;
+  // The LDS kernel stages a range of xE rows of a slab plus two rows on
+  // either side: whole slabs up to 36 KiB, else the most rows whose region
+  // fits 36 KiB (at least four blocks per CU).  512 threads per block; one
+  // thread per node when not even one row fits 64 KiB.
+  {
+    W8 &g = h->g;
+    const int n = g.n[4];
+    const size_t row_bytes = sizeof(float) * (size_t)g.stride[4];
+    int rows = n;
+    if (sizeof(float) * (size_t)g.inner > 36864)
+      rows = std::max(1, (int)(36864 / row_bytes) - 4);
+    // balance the parts: the fewest parts of at most `rows` rows, evened out
+    const int parts = (n + rows - 1) / rows;
+    g.part_rows = (n + parts - 1) / parts;
+    g.slab_parts = (n + g.part_rows - 1) / g.part_rows;
+    const int staged = std::min(n, g.part_rows + 4);
+    h->lds_bytes = row_bytes * (size_t)staged;
+    if (!(c.flags & PE_WORLD8D_GENERIC) && h->lds_bytes <= 65536)
+      h->variant = (c.flags & PE_WORLD8D_SLAB_256)    ? 256
+                   : (c.flags & PE_WORLD8D_SLAB_1024) ? 1024
+                                                      : 512;
+  }
   auto body = [&]() -> int {
     PE_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     PE_HIP(hipEventCreate(&h->t0));
@@ -688,7 +716,8 @@ static int world8d_launch(pe_world8d *h, const float *src, float *dst, const flo
   L.id = h->next_id++;
   (from_a ? h->b_id : h->a_id) = L.id;
   const unsigned slabs = (unsigned)(g.N / g.inner);
-  const size_t lds = sizeof(float) * (size_t)g.inner;
+  const size_t lds = h->lds_bytes;
+  const unsigned parts = slabs * (unsigned)g.slab_parts;
   // 32-bit byte offsets from the field base while the field is below 4 GiB.
   const bool small = (unsigned long long)g.N * 4 < (1ull << 32);
   if (h->variant == 0) {
@@ -701,13 +730,19 @@ static int world8d_launch(pe_world8d *h, const float *src, float *dst, const flo
     PE_LAUNCHED(world8d_stage);
     return PE_OK;
   }
-#define PE_W8_SLAB(T)                                                                     \
-  if (small)                                                                              \
-    hipLaunchKernelGGL((world8d_stage_slab<T, int>), dim3(slabs), dim3(T), lds, h->stream, \
-                       src, dst, base, h->arrival, g, L);                                 \
-  else                                                                                    \
-    hipLaunchKernelGGL((world8d_stage_slab<T, long long>), dim3(slabs), dim3(T), lds,     \
+#define PE_W8_SLAB_S(T, S)                                                                 \
+  if (small)                                                                               \
+    hipLaunchKernelGGL((world8d_stage_slab<T, int, S>), dim3(parts), dim3(T), lds,         \
+                       h->stream, src, dst, base, h->arrival, g, L);                       \
+  else                                                                                     \
+    hipLaunchKernelGGL((world8d_stage_slab<T, long long, S>), dim3(parts), dim3(T), lds,   \
                        h->stream, src, dst, base, h->arrival, g, L);
+#define PE_W8_SLAB(T)              \
+  if (g.slab_parts > 1) {          \
+    PE_W8_SLAB_S(T, true)          \
+  } else {                         \
+    PE_W8_SLAB_S(T, false)         \
+  }
   if (h->variant == 256) {
     PE_W8_SLAB(256)
   } else if (h->variant == 512) {
@@ -716,6 +751,7 @@ static int world8d_launch(pe_world8d *h, const float *src, float *dst, const flo
     PE_W8_SLAB(1024)
   }
 #undef PE_W8_SLAB
+#undef PE_W8_SLAB_S
   PE_LAUNCHED(world8d_stage_slab);
   return PE_OK;
 }

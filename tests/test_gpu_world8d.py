@@ -131,6 +131,30 @@ def test_kernel_variants_are_bit_identical(gpu, shape):
             np.testing.assert_array_equal(bits(runs[k][1]), bits(runs[0][1]))
 
 
+@pytest.mark.parametrize("shape", [(13, 12, 3, 7), (11, 16, 3, 5)])
+def test_split_slabs_are_bit_identical(gpu, shape):
+    """Slabs above 36 KiB are staged in parts of xE rows (4 and 2 parts
+    here): two RK2 steps give the bits of the thread-per-node kernel."""
+    from pe_amd import _native
+
+    print(f"[world8d] split slabs {shape}", flush=True)
+    n, h, np_, ne = shape
+    case = dict(positions=n, headings=h, p_speeds=np_, e_speeds=ne, order=2,
+                linear_speed_boundary=1, extent=3.5, arena=3.0, radius=0.3, p_min=-1.0,
+                p_max=2.0, e_min=-0.5, e_max=1.0, p_accel=2.0, e_accel=1.0, p_curvature=1.0,
+                e_curvature=2.0, cfl=0.8)
+    out = {}
+    for flags in (0, _native.PE_WORLD8D_GENERIC):
+        cfg = native_cfg(case)
+        cfg.flags = flags
+        with _native.World8D(cfg) as w:
+            w.advance_to(2 * w.stats()["max_internal_dt"])
+            out[w.stats()["kernel"]] = w.read()
+    assert sorted(out) == [0, 512]
+    for k in (0, 1):
+        np.testing.assert_array_equal(bits(out[512][k]), bits(out[0][k]))
+
+
 def test_reciprocal_division_is_exact_for_the_grid_spacings(gpu):
     """The kernels divide by a spacing h or its square hh through two
     correction steps on RN(1/h) while the field stays in range; every float
