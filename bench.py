@@ -213,6 +213,8 @@ def run_c2(args):
         prof = s.stats()
         s.sweep(min(args.steps, 20), (flags & ~_native.PE_SL_GRAPH) | _native.PE_SL_GLOBAL_GATHER)
         prof_g = s.stats()
+        s.sweep(min(args.steps, 20), (flags & ~_native.PE_SL_GRAPH) | _native.PE_SL_WIDE_IDS)
+        prof_w = s.stats()
         s.set_profiling(False)
         out = {
             "workload": "C2 8D contact game, 11x11 positions (+-3 m) x 12 headings x 5 speeds "
@@ -227,6 +229,8 @@ def run_c2(args):
             "e_side_ms": prof["e_ms"] / max(1, prof["e_launches"]),
             "e_side_global_gather_ms": (prof_g["e_ms"] - prof["e_ms"]) /
                                        max(1, prof_g["e_launches"] - prof["e_launches"]),
+            "e_side_unpacked_tables_ms": (prof_w["e_ms"] - prof_g["e_ms"]) /
+                                   max(1, prof_w["e_launches"] - prof_g["e_launches"]),
             "residual_after": s.stats()["last_residual"],
         }
         if args.to_residual:

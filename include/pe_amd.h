@@ -193,6 +193,10 @@ typedef struct pe_sl_game {
 #define PE_SL_GRAPH 4      /* replay captured sweeps with a hipGraph        */
 #define PE_SL_GLOBAL_GATHER 8 /* E side gathers G from global memory instead
                                  of LDS-staged rows (comparison runs)      */
+#define PE_SL_WIDE_IDS 16     /* E side reads the separate 32-bit id and
+                                 weight tables instead of the packed
+                                 96-byte records of 16-bit ids + weights
+                                 (comparison runs)                         */
 
 typedef struct pe_sl_stats {
   int64_t sweeps;

@@ -51,6 +51,7 @@ struct pe_sl {
   long long pairs = 0, g_size = 0, active = 0;
   // car tables
   int *pids = nullptr, *eids = nullptr, *ppos = nullptr, *epos = nullptr;
+  void *erec = nullptr;  // E stencils as interleaved 16-bit ids + weights (every e.n < 65,536)
   float *pw = nullptr, *ew = nullptr;
   uint8_t *pallowed = nullptr, *eallowed = nullptr, *inside = nullptr,
           *near = nullptr, *order = nullptr;

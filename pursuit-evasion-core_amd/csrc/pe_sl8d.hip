@@ -342,6 +342,14 @@ extern "C" int pe_sl_create_ex(const pe_sl_game *games, int32_t n_maps, int32_t 
       PE_HIP(hipMalloc((void **)dst, sizeof(float) * std::max<size_t>(clear.size(), 1)));
       PE_HIP(hipMemcpy(*dst, clear.data(), sizeof(float) * clear.size(), hipMemcpyHostToDevice));
     }
+    if (h->ne_max < 65536) {
+      const long long n = erow * (long long)na;  // records
+      PE_HIP(hipMalloc(&h->erec, sizeof(ERec) * std::max(n, 1ll)));
+      hipLaunchKernelGGL(sl_pack_erec, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0,
+                         h->eids, h->ew, (ERec *)h->erec, n);
+      PE_LAUNCHED(sl_pack_erec);
+      PE_HIP(hipDeviceSynchronize());
+    }
     PE_HIP(hipMalloc((void **)&h->inside, std::max(toff, 1ll)));
     PE_HIP(hipMalloc((void **)&h->near, std::max(toff, 1ll)));
     long long at = 0;
@@ -564,7 +572,7 @@ extern "C" void pe_sl_destroy(pe_sl *h) {
   if (h->stream) hipStreamSynchronize(h->stream);
   if (h->graph) hipGraphExecDestroy(h->graph);
   for (hipEvent_t e : h->events) hipEventDestroy(e);
-  for (void *p : {(void *)h->d_maps, (void *)h->pids, (void *)h->eids,
+  for (void *p : {(void *)h->d_maps, (void *)h->pids, (void *)h->eids, (void *)h->erec,
                   (void *)h->ppos, (void *)h->epos, (void *)h->pw, (void *)h->ew,
                   (void *)h->pallowed, (void *)h->eallowed, (void *)h->inside,
                   (void *)h->near, (void *)h->order, (void *)h->pstates,

@@ -5,6 +5,8 @@
 // a multiply and an add (oracle/sl8d.c follows local_game/grid_game.py).
 #pragma once
 
+#include <type_traits>
+
 #include "pe_internal.h"
 
 namespace pe {
@@ -85,10 +87,50 @@ __device__ __forceinline__ int nth_command(int i, int first) {
 // row G_u(ip, .).  Early exit: a P command whose running max over E
 // commands reaches the running min cannot lower it (grid_game.py:409-421).
 // Updates the stored command order of the pair.
-template <bool EARLY, bool ORDERED, typename T, typename Row>
+// E stencil ids of one (state, command): 16 values of type ID (int, or
+// 16-bit when every map has fewer than 65,536 E states), in vertex order.
+template <typename ID>
+struct EIds;
+template <>
+struct EIds<int> {
+  int4 v[4];
+  __device__ __forceinline__ void load(const int *p) {
+    const int4 *q = reinterpret_cast<const int4 *>(p);
+#pragma unroll
+    for (int c = 0; c < 4; c++) v[c] = q[c];
+  }
+  __device__ __forceinline__ int operator[](int k) const {
+    const int4 &a = v[k >> 2];
+    const int r = k & 3;
+    return r == 0 ? a.x : r == 1 ? a.y : r == 2 ? a.z : a.w;
+  }
+};
+template <>
+struct EIds<unsigned short> {
+  uint4 v[2];
+  __device__ __forceinline__ void load(const unsigned short *p) {
+    const uint4 *q = reinterpret_cast<const uint4 *>(p);
+    v[0] = q[0];
+    v[1] = q[1];
+  }
+  __device__ __forceinline__ int operator[](int k) const {
+    const uint4 &a = v[k >> 3];
+    const int r = (k >> 1) & 3;
+    const unsigned w = r == 0 ? a.x : r == 1 ? a.y : r == 2 ? a.z : a.w;
+    return (int)((k & 1) ? (w >> 16) : (w & 0xFFFFu));
+  }
+};
+
+// Interleaved 16-bit ids and weights of one (state, command): 96 bytes.
+struct ERec {
+  uint4 ids[2];
+  float4 w[4];
+};
+
+template <bool EARLY, bool ORDERED, typename T, typename Row, typename ID>
 __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T start,
                                              long long pair, Row row,
-                                             const int *__restrict__ eids,
+                                             const ID *__restrict__ eids,
                                              const float *__restrict__ ew,
                                              const uint8_t *__restrict__ eallowed,
                                              const uint8_t *__restrict__ pallowed,
@@ -114,17 +156,30 @@ __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T s
       const int b = ORDERED ? nth_command(j, bfirst) : j;
       const long long cmd = (long long)erow * m.na + b;
       if (!eallowed[cmd]) continue;
-      const int4 *id4 = reinterpret_cast<const int4 *>(eids + cmd * kStencil);
-      const float4 *w4 = reinterpret_cast<const float4 *>(ew + cmd * kStencil);
+      EIds<unsigned short> id16;
+      EIds<int> id32;
+      const float4 *w4;
+      if constexpr (std::is_same<ID, ERec>::value) {
+        const ERec *rec = reinterpret_cast<const ERec *>(eids) + cmd;
+        id16.v[0] = rec->ids[0];
+        id16.v[1] = rec->ids[1];
+        w4 = rec->w;
+      } else {
+        id32.load(reinterpret_cast<const int *>(eids) + cmd * kStencil);
+        w4 = reinterpret_cast<const float4 *>(ew + cmd * kStencil);
+      }
+      auto id = [&](int k) {
+        if constexpr (std::is_same<ID, int>::value) return id32[k];
+        else return id16[k];
+      };
       T q = 0;
 #pragma unroll
       for (int c = 0; c < kStencil / 4; c++) {
-        const int4 id = id4[c];
         const float4 w = w4[c];
-        q = madd(w.x, Gu[id.x], q);
-        q = madd(w.y, Gu[id.y], q);
-        q = madd(w.z, Gu[id.z], q);
-        q = madd(w.w, Gu[id.w], q);
+        q = madd(w.x, Gu[id(4 * c)], q);
+        q = madd(w.y, Gu[id(4 * c + 1)], q);
+        q = madd(w.z, Gu[id(4 * c + 2)], q);
+        q = madd(w.w, Gu[id(4 * c + 3)], q);
       }
       if (q > worst) {
         worst = q;
@@ -161,11 +216,11 @@ __device__ __forceinline__ void block_residual(T change, unsigned long long *slo
 }
 
 // E side with G gathered from global memory: one thread per pair.
-template <typename T, bool EARLY, bool ORDERED>
+template <typename T, bool EARLY, bool ORDERED, typename ID>
 __global__ __launch_bounds__(256) void sl_e_side(
     const SlMap *__restrict__ maps, const T *__restrict__ V,
     T *__restrict__ Vn, const T *__restrict__ G,
-    const int *__restrict__ eids, const float *__restrict__ ew,
+    const ID *__restrict__ eids, const float *__restrict__ ew,
     const uint8_t *__restrict__ eallowed, const uint8_t *__restrict__ pallowed,
     const int *__restrict__ ppos, const int *__restrict__ epos,
     const uint8_t *__restrict__ inside, uint8_t *__restrict__ order,
@@ -199,11 +254,11 @@ __global__ __launch_bounds__(256) void sl_e_side(
 // pair read LDS instead of scattered cache lines.  Dynamic LDS holds
 // [na][ne] floats; only the rows of allowed P commands are filled, and only
 // those are read.  Same arithmetic as sl_e_side, value for value.
-template <typename T, bool EARLY, bool ORDERED>
+template <typename T, bool EARLY, bool ORDERED, typename ID>
 __global__ __launch_bounds__(1024) void sl_e_side_lds(
     const SlMap *__restrict__ maps, const T *__restrict__ V,
     T *__restrict__ Vn, const T *__restrict__ G,
-    const int *__restrict__ eids, const float *__restrict__ ew,
+    const ID *__restrict__ eids, const float *__restrict__ ew,
     const uint8_t *__restrict__ eallowed, const uint8_t *__restrict__ pallowed,
     const int *__restrict__ ppos, const int *__restrict__ epos,
     const uint8_t *__restrict__ inside, uint8_t *__restrict__ order,
@@ -276,22 +331,53 @@ void enqueue_sweep_t(pe_sl *h, const void *src_, void *dst_, int flags,
   const bool early = flags & PE_SL_EARLY_EXIT, ordered = flags & PE_SL_ORDERED;
   const bool lds = h->lds_bytes > 0 && !(flags & PE_SL_GLOBAL_GATHER);
   const dim3 lds_grid((unsigned)h->np_max, 1, (unsigned)h->B), lds_block(1024);
-#define PE_E_SIDE(E, O)                                                           \
-  if (lds)                                                                        \
-    hipLaunchKernelGGL((sl_e_side_lds<T, E, O>), lds_grid, lds_block, h->lds_bytes, \
-                       h->stream, h->d_maps, src, dst, G, h->eids, h->ew,           \
-                       h->eallowed, h->pallowed, h->ppos, h->epos, h->inside,       \
-                       h->order, slot);                                             \
-  else                                                                              \
-    hipLaunchKernelGGL((sl_e_side<T, E, O>), grid, block, 0, h->stream, h->d_maps,  \
-                       src, dst, G, h->eids, h->ew, h->eallowed, h->pallowed,       \
-                       h->ppos, h->epos, h->inside, h->order, slot)
+  const bool packed = h->erec && !(flags & PE_SL_WIDE_IDS);
+#define PE_E_SIDE_ID(E, O, ID, IDS)                                                    \
+  if (lds)                                                                             \
+    hipLaunchKernelGGL((sl_e_side_lds<T, E, O, ID>), lds_grid, lds_block, h->lds_bytes,  \
+                       h->stream, h->d_maps, src, dst, G, IDS, h->ew, h->eallowed,       \
+                       h->pallowed, h->ppos, h->epos, h->inside, h->order, slot);        \
+  else                                                                                   \
+    hipLaunchKernelGGL((sl_e_side<T, E, O, ID>), grid, block, 0, h->stream, h->d_maps,   \
+                       src, dst, G, IDS, h->ew, h->eallowed, h->pallowed, h->ppos,       \
+                       h->epos, h->inside, h->order, slot)
+#define PE_E_SIDE(E, O)                                            \
+  do {                                                             \
+    if (packed) {                                                  \
+      PE_E_SIDE_ID(E, O, ERec, (const ERec *)h->erec);             \
+    } else {                                                       \
+      PE_E_SIDE_ID(E, O, int, h->eids);                            \
+    }                                                              \
+  } while (0)
   if (early && ordered) PE_E_SIDE(true, true);
   else if (early) PE_E_SIDE(true, false);
   else if (ordered) PE_E_SIDE(false, true);
   else PE_E_SIDE(false, false);
 #undef PE_E_SIDE
+#undef PE_E_SIDE_ID
   if (record) sl_record(h, 1, false);
+}
+
+// Interleaved records of the E stencils: 16-bit ids (every id below 65,536)
+// then the 16 weights, one record per (state, command).
+__global__ __launch_bounds__(256) inline void sl_pack_erec(const int *__restrict__ ids,
+                                                           const float *__restrict__ w,
+                                                           ERec *__restrict__ out,
+                                                           long long n) {
+  const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (r >= n) return;
+  unsigned short id[16];
+  float wt[16];
+  for (int k = 0; k < 16; k++) {
+    id[k] = (unsigned short)ids[r * 16 + k];
+    wt[k] = w[r * 16 + k];
+  }
+  ERec rec;
+  unsigned *u = reinterpret_cast<unsigned *>(&rec.ids[0]);
+  for (int k = 0; k < 8; k++) u[k] = (unsigned)id[2 * k] | ((unsigned)id[2 * k + 1] << 16);
+  float *f = reinterpret_cast<float *>(&rec.w[0]);
+  for (int k = 0; k < 16; k++) f[k] = wt[k];
+  out[r] = rec;
 }
 
 // Lets the LDS E side use `bytes` of dynamic LDS for every flag variant;
@@ -299,10 +385,14 @@ void enqueue_sweep_t(pe_sl *h, const void *src_, void *dst_, int flags,
 template <typename T>
 bool sl_allow_lds(size_t bytes) {
   bool ok = true;
-  for (const void *f : {(const void *)sl_e_side_lds<T, true, true>,
-                        (const void *)sl_e_side_lds<T, true, false>,
-                        (const void *)sl_e_side_lds<T, false, true>,
-                        (const void *)sl_e_side_lds<T, false, false>})
+  for (const void *f : {(const void *)sl_e_side_lds<T, true, true, int>,
+                        (const void *)sl_e_side_lds<T, true, false, int>,
+                        (const void *)sl_e_side_lds<T, false, true, int>,
+                        (const void *)sl_e_side_lds<T, false, false, int>,
+                        (const void *)sl_e_side_lds<T, true, true, ERec>,
+                        (const void *)sl_e_side_lds<T, true, false, ERec>,
+                        (const void *)sl_e_side_lds<T, false, true, ERec>,
+                        (const void *)sl_e_side_lds<T, false, false, ERec>})
     ok = ok && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) ==
                    hipSuccess;
   if (!ok) (void)hipGetLastError();

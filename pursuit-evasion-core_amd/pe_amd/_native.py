@@ -181,6 +181,7 @@ PE_SL_EARLY_EXIT = 1
 PE_SL_ORDERED = 2
 PE_SL_GRAPH = 4
 PE_SL_GLOBAL_GATHER = 8
+PE_SL_WIDE_IDS = 16
 
 class World8DConfig(ctypes.Structure):
     _fields_ = [

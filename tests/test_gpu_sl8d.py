@@ -39,6 +39,7 @@ from pe_amd.reversible.model import VehicleLimits
 pytestmark = pytest.mark.gpu
 EARLY, ORDERED, GRAPH = _native.PE_SL_EARLY_EXIT, _native.PE_SL_ORDERED, _native.PE_SL_GRAPH
 GLOBAL = _native.PE_SL_GLOBAL_GATHER
+WIDE = _native.PE_SL_WIDE_IDS
 CONTROLS = np.array([[a, k] for a in (-3, 0, 3) for k in (-1, 0, 1)], float)
 
 
@@ -123,7 +124,8 @@ def run_variants(tables, sweeps):
     out = {}
     for name, flags in [("plain", 0), ("early", EARLY), ("ordered", EARLY | ORDERED),
                         ("graph", EARLY | ORDERED | GRAPH), ("global", EARLY | ORDERED | GLOBAL),
-                        ("global_plain", GLOBAL)]:
+                        ("global_plain", GLOBAL), ("wide_ids", EARLY | ORDERED | WIDE),
+                        ("wide_ids_global", EARLY | ORDERED | GLOBAL | WIDE)]:
         with GameSolver(tables) as s:
             s.sweep(sweeps, flags)
             out[name] = (s.read(), s.trace()[:, 0])
@@ -137,7 +139,8 @@ def test_early_exit_ordering_and_graph_are_exact(gpu):
     print("[gpu] flag variants", flush=True)
     res = run_variants(tables, 6)
     base, trace = res["plain"]
-    for name in ("early", "ordered", "graph", "global", "global_plain"):
+    for name in ("early", "ordered", "graph", "global", "global_plain", "wide_ids",
+                 "wide_ids_global"):
         np.testing.assert_array_equal(res[name][0], base, err_msg=name)
         np.testing.assert_array_equal(res[name][1], trace, err_msg=name)
 
