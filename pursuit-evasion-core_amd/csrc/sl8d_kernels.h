@@ -82,11 +82,6 @@ __device__ __forceinline__ int nth_command(int i, int first) {
   return i == 0 ? first : (i <= first ? i - 1 : i);
 }
 
-// Min over allowed P commands of the max over allowed E commands of one
-// inside pair (ip, ie), starting from its current value; row(u) is the
-// row G_u(ip, .).  Early exit: a P command whose running max over E
-// commands reaches the running min cannot lower it (grid_game.py:409-421).
-// Updates the stored command order of the pair.
 // E stencil ids of one (state, command): 16 values of type ID (int, or
 // 16-bit when every map has fewer than 65,536 E states), in vertex order.
 template <typename ID>
@@ -108,11 +103,6 @@ struct EIds<int> {
 template <>
 struct EIds<unsigned short> {
   uint4 v[2];
-  __device__ __forceinline__ void load(const unsigned short *p) {
-    const uint4 *q = reinterpret_cast<const uint4 *>(p);
-    v[0] = q[0];
-    v[1] = q[1];
-  }
   __device__ __forceinline__ int operator[](int k) const {
     const uint4 &a = v[k >> 3];
     const int r = (k >> 1) & 3;
@@ -127,14 +117,65 @@ struct ERec {
   float4 w[4];
 };
 
+// One E stencil in registers: from the packed records (ID = ERec) or from
+// the separate 32-bit id and float weight tables (ID = int).
+template <typename ID>
+struct EStencil {
+  EIds<typename std::conditional<std::is_same<ID, ERec>::value, unsigned short, int>::type> id;
+  float4 w[4];
+  __device__ __forceinline__ void load(const ID *__restrict__ eids,
+                                       const float *__restrict__ ew, long long cmd) {
+    if constexpr (std::is_same<ID, ERec>::value) {
+      const ERec *rec = eids + cmd;
+      id.v[0] = rec->ids[0];
+      id.v[1] = rec->ids[1];
+#pragma unroll
+      for (int c = 0; c < 4; c++) w[c] = rec->w[c];
+    } else {
+      id.load(eids + cmd * kStencil);
+      const float4 *w4 = reinterpret_cast<const float4 *>(ew + cmd * kStencil);
+#pragma unroll
+      for (int c = 0; c < 4; c++) w[c] = w4[c];
+    }
+  }
+  // sum_k w_k G[id_k], vertex order (grid_game.py:406-408)
+  template <typename T>
+  __device__ __forceinline__ T eval(const T *Gu) const {
+    T q = 0;
+#pragma unroll
+    for (int c = 0; c < kStencil / 4; c++) {
+      q = madd(w[c].x, Gu[id[4 * c]], q);
+      q = madd(w[c].y, Gu[id[4 * c + 1]], q);
+      q = madd(w[c].z, Gu[id[4 * c + 2]], q);
+      q = madd(w[c].w, Gu[id[4 * c + 3]], q);
+    }
+    return q;
+  }
+};
+
+// Min over allowed P commands of the max over allowed E commands of one
+// inside pair (ip, ie), starting from its current value; row(u) is the
+// row G_u(ip, .).  Early exit: a P command whose running max over E
+// commands reaches the running min cannot lower it (grid_game.py:409-421).
+// Updates the stored command order of the pair.
+//
+// Without ORDERED the commands are visited u-major as in the reference.
+// With ORDERED the stored best P command u* is evaluated first, u-major;
+// then every other allowed P command is carried along one pass over the E
+// commands, so each E stencil is read once for all of them instead of once
+// per P command.  The value and the stored commands are the same: max and
+// min are exact, each (u, b) combination is summed in the same order, ties
+// go to the first command in visiting order, and a command cut against the
+// bound of u* could not have been chosen later (its max already reaches a
+// value at or above the final min).
 template <bool EARLY, bool ORDERED, typename T, typename Row, typename ID>
 __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T start,
-                                             long long pair, Row row,
-                                             const ID *__restrict__ eids,
-                                             const float *__restrict__ ew,
-                                             const uint8_t *__restrict__ eallowed,
-                                             const uint8_t *__restrict__ pallowed,
-                                             uint8_t *__restrict__ order) {
+                                         long long pair, Row row,
+                                         const ID *__restrict__ eids,
+                                         const float *__restrict__ ew,
+                                         const uint8_t *__restrict__ eallowed,
+                                         const uint8_t *__restrict__ pallowed,
+                                         uint8_t *__restrict__ order) {
   T best = start;
   int ufirst = 0, bfirst = 0;
   if (ORDERED) {
@@ -144,43 +185,22 @@ __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T s
       bfirst = o >> 4;
     }
   }
+  const long long cmd0 = (long long)erow * m.na;
+  const uint8_t *pa = pallowed + (long long)prow * m.na;
   int best_u = -1, best_b = -1;
-  for (int i = 0; i < m.na; i++) {
-    const int u = ORDERED ? nth_command(i, ufirst) : i;
-    if (!pallowed[(long long)prow * m.na + u]) continue;
+  for (int i = 0; i < (ORDERED ? 1 : m.na); i++) {
+    const int u = ORDERED ? ufirst : i;
+    if (!pa[u]) continue;
     const T *Gu = row(u);
     T worst = T(-1e30);
     int arg = -1;
     bool cut = false;
     for (int j = 0; j < m.na; j++) {
       const int b = ORDERED ? nth_command(j, bfirst) : j;
-      const long long cmd = (long long)erow * m.na + b;
-      if (!eallowed[cmd]) continue;
-      EIds<unsigned short> id16;
-      EIds<int> id32;
-      const float4 *w4;
-      if constexpr (std::is_same<ID, ERec>::value) {
-        const ERec *rec = reinterpret_cast<const ERec *>(eids) + cmd;
-        id16.v[0] = rec->ids[0];
-        id16.v[1] = rec->ids[1];
-        w4 = rec->w;
-      } else {
-        id32.load(reinterpret_cast<const int *>(eids) + cmd * kStencil);
-        w4 = reinterpret_cast<const float4 *>(ew + cmd * kStencil);
-      }
-      auto id = [&](int k) {
-        if constexpr (std::is_same<ID, int>::value) return id32[k];
-        else return id16[k];
-      };
-      T q = 0;
-#pragma unroll
-      for (int c = 0; c < kStencil / 4; c++) {
-        const float4 w = w4[c];
-        q = madd(w.x, Gu[id(4 * c)], q);
-        q = madd(w.y, Gu[id(4 * c + 1)], q);
-        q = madd(w.z, Gu[id(4 * c + 2)], q);
-        q = madd(w.w, Gu[id(4 * c + 3)], q);
-      }
+      if (!eallowed[cmd0 + b]) continue;
+      EStencil<ID> st;
+      st.load(eids, ew, cmd0 + b);
+      const T q = st.eval(Gu);
       if (q > worst) {
         worst = q;
         arg = b;
@@ -196,7 +216,42 @@ __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T s
       best_b = arg;
     }
   }
-  if (ORDERED && best_u >= 0) order[pair] = (uint8_t)(best_u | (best_b << 4));
+  if constexpr (ORDERED) {
+    unsigned alive = 0;
+#pragma unroll
+    for (int u = 0; u < kMaxActions; u++)
+      if (u < m.na && u != ufirst && pa[u]) alive |= 1u << u;
+    T worst[kMaxActions];
+    unsigned long long args = 0;  // 4 bits per P command
+#pragma unroll
+    for (int u = 0; u < kMaxActions; u++) worst[u] = T(-1e30);
+    for (int j = 0; j < m.na && alive; j++) {
+      const int b = nth_command(j, bfirst);
+      if (!eallowed[cmd0 + b]) continue;
+      EStencil<ID> st;
+      st.load(eids, ew, cmd0 + b);
+#pragma unroll
+      for (int u = 0; u < kMaxActions; u++) {
+        if (!((alive >> u) & 1u)) continue;
+        const T q = st.eval(row(u));
+        if (q > worst[u]) {
+          worst[u] = q;
+          args = (args & ~(15ull << (4 * u))) | ((unsigned long long)b << (4 * u));
+        }
+        if (EARLY && worst[u] >= best) alive &= ~(1u << u);
+      }
+    }
+    // Survivors in visiting order (ascending u after u*).
+#pragma unroll
+    for (int u = 0; u < kMaxActions; u++) {
+      if (((alive >> u) & 1u) && worst[u] < best) {
+        best = worst[u];
+        best_u = u;
+        best_b = (int)((args >> (4 * u)) & 15);
+      }
+    }
+    if (best_u >= 0) order[pair] = (uint8_t)(best_u | (best_b << 4));
+  }
   return best;
 }
 
