@@ -20,6 +20,7 @@ struct SlMap {
   long long g_off;      // offset in G
   long long tab_off;    // offset in the inside / near tables
   int p_row, e_row;     // first state row in the concatenated car tables
+  int ipos_off;         // first entry of this map's P positions in the inside lists
   float exit_value, pl, pw, el, ew;
 };
 
@@ -52,6 +53,9 @@ struct pe_sl {
   // car tables
   int *pids = nullptr, *eids = nullptr, *ppos = nullptr, *epos = nullptr;
   void *erec = nullptr;  // E stencils as interleaved 16-bit ids + weights (every e.n < 65,536)
+  // E indices of the inside pairs of each (map, P position), ascending:
+  // ilist[ioff[k] .. ioff[k + 1]) for k = SlMap::ipos_off + position.
+  int *ilist = nullptr, *ioff = nullptr;
   float *pw = nullptr, *ew = nullptr;
   uint8_t *pallowed = nullptr, *eallowed = nullptr, *inside = nullptr,
           *near = nullptr, *order = nullptr;

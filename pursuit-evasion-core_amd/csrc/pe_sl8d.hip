@@ -247,6 +247,7 @@ extern "C" int pe_sl_create_ex(const pe_sl_game *games, int32_t n_maps, int32_t 
   h->B = n_maps;
   h->na = na;
   long long prow = 0, erow = 0, voff = 0, goff = 0, toff = 0;
+  std::vector<int> ilist, ioff;
   for (int b = 0; b < n_maps; b++) {
     const pe_sl_game &g = games[b];
     SlMap m{};
@@ -259,6 +260,7 @@ extern "C" int pe_sl_create_ex(const pe_sl_game *games, int32_t n_maps, int32_t 
     m.tab_off = toff;
     m.p_row = (int)prow;
     m.e_row = (int)erow;
+    m.ipos_off = (int)ioff.size();
     m.exit_value = g.exit_value;
     m.pl = g.p_length;
     m.pw = g.p_width;
@@ -272,12 +274,19 @@ extern "C" int pe_sl_create_ex(const pe_sl_game *games, int32_t n_maps, int32_t 
     erow += m.ne;
     h->np_max = std::max(h->np_max, m.np);
     h->ne_max = std::max(h->ne_max, m.ne);
-    for (int i = 0; i < m.np; i++)
+    std::vector<int> count(g.p.positions, 0);
+    for (int pp = 0; pp < g.p.positions; pp++) {
+      ioff.push_back((int)std::min<size_t>(ilist.size(), INT32_MAX));
       for (int j = 0; j < m.ne; j++)
-        h->active += g.inside[(long long)g.p.position[i] * g.e.positions +
-                              g.e.position[j]] != 0;
+        if (g.inside[(long long)pp * g.e.positions + g.e.position[j]]) {
+          ilist.push_back(j);
+          count[pp]++;
+        }
+    }
+    for (int i = 0; i < m.np; i++) h->active += count[g.p.position[i]];
   }
-  if (prow > INT32_MAX || erow > INT32_MAX) {
+  ioff.push_back((int)std::min<size_t>(ilist.size(), INT32_MAX));
+  if (prow > INT32_MAX || erow > INT32_MAX || ilist.size() >= (size_t)INT32_MAX) {
     delete h;
     return fail(PE_ERR_INVALID, "too many states");
   }
@@ -350,6 +359,10 @@ extern "C" int pe_sl_create_ex(const pe_sl_game *games, int32_t n_maps, int32_t 
       PE_LAUNCHED(sl_pack_erec);
       PE_HIP(hipDeviceSynchronize());
     }
+    PE_HIP(hipMalloc((void **)&h->ilist, sizeof(int) * std::max<size_t>(ilist.size(), 1)));
+    PE_HIP(hipMalloc((void **)&h->ioff, sizeof(int) * ioff.size()));
+    PE_HIP(hipMemcpy(h->ilist, ilist.data(), sizeof(int) * ilist.size(), hipMemcpyHostToDevice));
+    PE_HIP(hipMemcpy(h->ioff, ioff.data(), sizeof(int) * ioff.size(), hipMemcpyHostToDevice));
     PE_HIP(hipMalloc((void **)&h->inside, std::max(toff, 1ll)));
     PE_HIP(hipMalloc((void **)&h->near, std::max(toff, 1ll)));
     long long at = 0;
@@ -410,6 +423,9 @@ static int initialize_from(pe_sl *h, const S *v0) {
       PE_HIP(hipMemcpy(h->va, v.data(), sizeof(float) * h->pairs, hipMemcpyHostToDevice));
     }
   }
+  // Both value buffers hold the same field: the LDS E side writes inside
+  // pairs only, so the others must already agree in the buffer it writes.
+  PE_HIP(hipMemcpyAsync(h->vb, h->va, h->esize * h->pairs, hipMemcpyDeviceToDevice, h->stream));
   PE_HIP(hipStreamSynchronize(h->stream));
   h->trace.clear();
   h->sweeps = 0;
@@ -573,6 +589,7 @@ extern "C" void pe_sl_destroy(pe_sl *h) {
   if (h->graph) hipGraphExecDestroy(h->graph);
   for (hipEvent_t e : h->events) hipEventDestroy(e);
   for (void *p : {(void *)h->d_maps, (void *)h->pids, (void *)h->eids, (void *)h->erec,
+                  (void *)h->ilist, (void *)h->ioff,
                   (void *)h->ppos, (void *)h->epos, (void *)h->pw, (void *)h->ew,
                   (void *)h->pallowed, (void *)h->eallowed, (void *)h->inside,
                   (void *)h->near, (void *)h->order, (void *)h->pstates,

@@ -121,6 +121,10 @@ extern "C" int pe_sl_warm_start(pe_sl *sl, pe_mlp *mlp, float delta) {
     }
   }
   PE_HIP(hipStreamSynchronize(mlp->stream));
+  // The sweep's second buffer must agree on the pairs it does not write.
+  PE_HIP(hipMemcpyAsync(sl->vb, sl->va, sizeof(float) * sl->pairs, hipMemcpyDeviceToDevice,
+                        sl->stream));
+  PE_HIP(hipStreamSynchronize(sl->stream));
   sl->trace.clear();
   sl->sweeps = 0;
   sl->last = 0;

@@ -305,18 +305,21 @@ __global__ __launch_bounds__(256) void sl_e_side(
 }
 
 // E side with the G rows of one P state staged in LDS: one block per
-// (P state, map) walks the whole E row, so the 16-point E gathers of every
-// pair read LDS instead of scattered cache lines.  Dynamic LDS holds
-// [na][ne] floats; only the rows of allowed P commands are filled, and only
-// those are read.  Same arithmetic as sl_e_side, value for value.
+// (P state, map) walks the inside pairs of that P state, so the 16-point E
+// gathers of every pair read LDS instead of scattered cache lines.  Dynamic
+// LDS holds [na][ne] floats; only the rows of allowed P commands are filled,
+// and only those are read.  The pairs come from the list of inside E states
+// of the P state's position, so every lane has a pair to update; the pairs
+// outside the exit radius are not written (both value buffers already hold
+// the same exit value there).  Same arithmetic as sl_e_side, value for value.
 template <typename T, bool EARLY, bool ORDERED, typename ID>
 __global__ __launch_bounds__(1024) void sl_e_side_lds(
     const SlMap *__restrict__ maps, const T *__restrict__ V,
     T *__restrict__ Vn, const T *__restrict__ G,
     const ID *__restrict__ eids, const float *__restrict__ ew,
     const uint8_t *__restrict__ eallowed, const uint8_t *__restrict__ pallowed,
-    const int *__restrict__ ppos, const int *__restrict__ epos,
-    const uint8_t *__restrict__ inside, uint8_t *__restrict__ order,
+    const int *__restrict__ ppos, const int *__restrict__ ilist,
+    const int *__restrict__ ioff, uint8_t *__restrict__ order,
     unsigned long long *__restrict__ slot) {
   extern __shared__ __align__(16) unsigned char tile_bytes[];
   T *tile = reinterpret_cast<T *>(tile_bytes);
@@ -335,17 +338,16 @@ __global__ __launch_bounds__(1024) void sl_e_side_lds(
   __syncthreads();
   T change = 0;
   if (active) {
-    const int pp = ppos[prow];
-    for (int ie = threadIdx.x; ie < m.ne; ie += blockDim.x) {
+    const int k0 = ioff[m.ipos_off + ppos[prow]];
+    const int count = ioff[m.ipos_off + ppos[prow] + 1] - k0;
+    auto row = [&](int u) { return tile + u * m.ne; };
+    for (int k = threadIdx.x; k < count; k += blockDim.x) {
+      const int ie = ilist[k0 + k];
       const int erow = m.e_row + ie;
       const long long pair = m.v_off + (long long)ip * m.ne + ie;
       const T start = V[pair];
-      T best = start;
-      if (inside[m.tab_off + (long long)pp * m.npos_e + epos[erow]]) {
-        auto row = [&](int u) { return tile + u * m.ne; };
-        best = pair_backup<EARLY, ORDERED>(m, prow, erow, start, pair, row, eids, ew,
-                                           eallowed, pallowed, order);
-      }
+      const T best = pair_backup<EARLY, ORDERED>(m, prow, erow, start, pair, row, eids, ew,
+                                                 eallowed, pallowed, order);
       Vn[pair] = best;
       const T d = best > start ? best - start : start - best;
       change = change > d ? change : d;
@@ -391,7 +393,7 @@ void enqueue_sweep_t(pe_sl *h, const void *src_, void *dst_, int flags,
   if (lds)                                                                             \
     hipLaunchKernelGGL((sl_e_side_lds<T, E, O, ID>), lds_grid, lds_block, h->lds_bytes,  \
                        h->stream, h->d_maps, src, dst, G, IDS, h->ew, h->eallowed,       \
-                       h->pallowed, h->ppos, h->epos, h->inside, h->order, slot);        \
+                       h->pallowed, h->ppos, h->ilist, h->ioff, h->order, slot);         \
   else                                                                                   \
     hipLaunchKernelGGL((sl_e_side<T, E, O, ID>), grid, block, 0, h->stream, h->d_maps,   \
                        src, dst, G, IDS, h->ew, h->eallowed, h->pallowed, h->ppos,       \
