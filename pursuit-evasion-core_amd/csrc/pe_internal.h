@@ -56,6 +56,7 @@ struct pe_sl {
   // E indices of the inside pairs of each (map, P position), ascending:
   // ilist[ioff[k] .. ioff[k + 1]) for k = SlMap::ipos_off + position.
   int *ilist = nullptr, *ioff = nullptr;
+  int *nlist = nullptr, *noff = nullptr;  // the same for the near pairs
   float *pw = nullptr, *ew = nullptr;
   uint8_t *pallowed = nullptr, *eallowed = nullptr, *inside = nullptr,
           *near = nullptr, *order = nullptr;

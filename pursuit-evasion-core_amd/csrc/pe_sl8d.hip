@@ -12,8 +12,9 @@
 //
 //   sl_p_side  G_u(ip, e) for all u at once, one thread per (ip, e), e
 //              fastest so the 16 V rows it gathers are read coalesced; the
-//              P tables of (ip, u) are wave-uniform (scalar loads).  Entries
-//              no inside pair can reach ("near" table) are skipped.
+//              P tables of (ip, u) are wave-uniform (scalar loads).  Only
+//              entries an inside pair can reach ("near" table) are computed:
+//              the threads walk a list of the near E states of ip's position.
 //   sl_e_side  one thread per pair (ip, ie), ie fastest: the min over P
 //              commands of the max over E commands of the 16-point E
 //              interpolation of G_u, with the reference's exact early exit
@@ -23,9 +24,10 @@
 //              also max-reduces |V' - V| per map (the residual).
 //   sl_e_side_lds  the same per pair, but one 1024-thread block per P state
 //              first copies that state's G_u rows into LDS ([na][ne] floats,
-//              94 KB at 2,612 E states), then walks the whole E row, so the
-//              16-point E gathers hit LDS instead of one cache line per lane.
-//              Used whenever the rows fit; bit-identical to sl_e_side.
+//              94 KB at 2,612 E states), then walks the list of inside E
+//              states of the P state's position, so the 16-point E gathers
+//              hit LDS instead of one cache line per lane.  Used whenever
+//              the rows fit; bit-identical to sl_e_side.
 //
 // G holds [maps][u][ip][e] float32 (9 x the field) and is zero-filled once:
 // entries skipped by sl_p_side are never read with a non-zero weight, and
@@ -247,7 +249,7 @@ extern "C" int pe_sl_create_ex(const pe_sl_game *games, int32_t n_maps, int32_t 
   h->B = n_maps;
   h->na = na;
   long long prow = 0, erow = 0, voff = 0, goff = 0, toff = 0;
-  std::vector<int> ilist, ioff;
+  std::vector<int> ilist, ioff, nlist, noff;
   for (int b = 0; b < n_maps; b++) {
     const pe_sl_game &g = games[b];
     SlMap m{};
@@ -277,16 +279,22 @@ extern "C" int pe_sl_create_ex(const pe_sl_game *games, int32_t n_maps, int32_t 
     std::vector<int> count(g.p.positions, 0);
     for (int pp = 0; pp < g.p.positions; pp++) {
       ioff.push_back((int)std::min<size_t>(ilist.size(), INT32_MAX));
-      for (int j = 0; j < m.ne; j++)
-        if (g.inside[(long long)pp * g.e.positions + g.e.position[j]]) {
+      noff.push_back((int)std::min<size_t>(nlist.size(), INT32_MAX));
+      for (int j = 0; j < m.ne; j++) {
+        const long long t = (long long)pp * g.e.positions + g.e.position[j];
+        if (g.inside[t]) {
           ilist.push_back(j);
           count[pp]++;
         }
+        if (g.near[t]) nlist.push_back(j);
+      }
     }
     for (int i = 0; i < m.np; i++) h->active += count[g.p.position[i]];
   }
   ioff.push_back((int)std::min<size_t>(ilist.size(), INT32_MAX));
-  if (prow > INT32_MAX || erow > INT32_MAX || ilist.size() >= (size_t)INT32_MAX) {
+  noff.push_back((int)std::min<size_t>(nlist.size(), INT32_MAX));
+  if (prow > INT32_MAX || erow > INT32_MAX || ilist.size() >= (size_t)INT32_MAX ||
+      nlist.size() >= (size_t)INT32_MAX) {
     delete h;
     return fail(PE_ERR_INVALID, "too many states");
   }
@@ -363,6 +371,10 @@ extern "C" int pe_sl_create_ex(const pe_sl_game *games, int32_t n_maps, int32_t 
     PE_HIP(hipMalloc((void **)&h->ioff, sizeof(int) * ioff.size()));
     PE_HIP(hipMemcpy(h->ilist, ilist.data(), sizeof(int) * ilist.size(), hipMemcpyHostToDevice));
     PE_HIP(hipMemcpy(h->ioff, ioff.data(), sizeof(int) * ioff.size(), hipMemcpyHostToDevice));
+    PE_HIP(hipMalloc((void **)&h->nlist, sizeof(int) * std::max<size_t>(nlist.size(), 1)));
+    PE_HIP(hipMalloc((void **)&h->noff, sizeof(int) * noff.size()));
+    PE_HIP(hipMemcpy(h->nlist, nlist.data(), sizeof(int) * nlist.size(), hipMemcpyHostToDevice));
+    PE_HIP(hipMemcpy(h->noff, noff.data(), sizeof(int) * noff.size(), hipMemcpyHostToDevice));
     PE_HIP(hipMalloc((void **)&h->inside, std::max(toff, 1ll)));
     PE_HIP(hipMalloc((void **)&h->near, std::max(toff, 1ll)));
     long long at = 0;
@@ -589,7 +601,7 @@ extern "C" void pe_sl_destroy(pe_sl *h) {
   if (h->graph) hipGraphExecDestroy(h->graph);
   for (hipEvent_t e : h->events) hipEventDestroy(e);
   for (void *p : {(void *)h->d_maps, (void *)h->pids, (void *)h->eids, (void *)h->erec,
-                  (void *)h->ilist, (void *)h->ioff,
+                  (void *)h->ilist, (void *)h->ioff, (void *)h->nlist, (void *)h->noff,
                   (void *)h->ppos, (void *)h->epos, (void *)h->pw, (void *)h->ew,
                   (void *)h->pallowed, (void *)h->eallowed, (void *)h->inside,
                   (void *)h->near, (void *)h->order, (void *)h->pstates,

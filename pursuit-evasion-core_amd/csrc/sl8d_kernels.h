@@ -49,21 +49,25 @@ __global__ __launch_bounds__(256) void sl_initialize(
   }
 }
 
+// P side: G_u(ip, e) for the near E states e of each P state ip, taken
+// from the list of near E states of ip's position (ascending, so the V rows
+// are still read nearly coalesced) and one thread per list entry; entries
+// of G that no inside pair can reach are not computed.
 template <typename T>
 __global__ __launch_bounds__(256) void sl_p_side(
     const SlMap *__restrict__ maps, const T *__restrict__ V,
     T *__restrict__ G, const int *__restrict__ pids,
     const float *__restrict__ pw, const uint8_t *__restrict__ pallowed,
-    const int *__restrict__ ppos, const int *__restrict__ epos,
-    const uint8_t *__restrict__ near) {
+    const int *__restrict__ ppos, const int *__restrict__ nlist,
+    const int *__restrict__ noff) {
   const SlMap m = maps[blockIdx.z];
-  const int e = blockIdx.x * 256 + threadIdx.x;
-  if (e >= m.ne) return;
+  const int k = blockIdx.x * 256 + threadIdx.x;
   const T *__restrict__ Vm = V + m.v_off;
-  const int ep = epos[m.e_row + e];
   for (int ip = blockIdx.y; ip < m.np; ip += gridDim.y) {
     const int row = m.p_row + ip;
-    if (!near[m.tab_off + (long long)ppos[row] * m.npos_e + ep]) continue;
+    const int k0 = noff[m.ipos_off + ppos[row]];
+    if (k >= noff[m.ipos_off + ppos[row] + 1] - k0) continue;
+    const int e = nlist[k0 + k];
     for (int u = 0; u < m.na; u++) {
       const long long cmd = (long long)row * m.na + u;
       if (!pallowed[cmd]) continue;
@@ -71,7 +75,7 @@ __global__ __launch_bounds__(256) void sl_p_side(
       const float *w = pw + cmd * kStencil;
       T acc = 0;
 #pragma unroll
-      for (int k = 0; k < kStencil; k++) acc = madd(w[k], Vm[(long long)id[k] * m.ne + e], acc);
+      for (int q = 0; q < kStencil; q++) acc = madd(w[q], Vm[(long long)id[q] * m.ne + e], acc);
       G[m.g_off + ((long long)u * m.np + ip) * m.ne + e] = acc;
     }
   }
@@ -382,7 +386,7 @@ void enqueue_sweep_t(pe_sl *h, const void *src_, void *dst_, int flags,
   const dim3 grid = pair_grid(h), block(256);
   if (record) sl_record(h, 0, true);
   hipLaunchKernelGGL(sl_p_side<T>, grid, block, 0, h->stream, h->d_maps, src, G,
-                     h->pids, h->pw, h->pallowed, h->ppos, h->epos, h->near);
+                     h->pids, h->pw, h->pallowed, h->ppos, h->nlist, h->noff);
   if (record) sl_record(h, 0, false);
   if (record) sl_record(h, 1, true);
   const bool early = flags & PE_SL_EARLY_EXIT, ordered = flags & PE_SL_ORDERED;
