@@ -57,6 +57,9 @@ struct pe_sl {
   // ilist[ioff[k] .. ioff[k + 1]) for k = SlMap::ipos_off + position.
   int *ilist = nullptr, *ioff = nullptr;
   int *nlist = nullptr, *noff = nullptr;  // the same for the near pairs
+  // P stencils compacted (non-zero weights first, in vertex order) and the
+  // count of non-zero weights per (state, command); pids / pw hold them.
+  uint8_t *pcount = nullptr;
   float *pw = nullptr, *ew = nullptr;
   uint8_t *pallowed = nullptr, *eallowed = nullptr, *inside = nullptr,
           *near = nullptr, *order = nullptr;

@@ -359,6 +359,37 @@ extern "C" int pe_sl_create_ex(const pe_sl_game *games, int32_t n_maps, int32_t 
       PE_HIP(hipMalloc((void **)dst, sizeof(float) * std::max<size_t>(clear.size(), 1)));
       PE_HIP(hipMemcpy(*dst, clear.data(), sizeof(float) * clear.size(), hipMemcpyHostToDevice));
     }
+    {
+      // Compacted P stencils (see sl_p_side).
+      const size_t n = (size_t)prow * na;
+      std::vector<int> ids(n * kStencil);
+      std::vector<float> wts(n * kStencil);
+      std::vector<uint8_t> cnt(n);
+      size_t at = 0;
+      for (int b = 0; b < n_maps; b++) {
+        const pe_sl_car &c = games[b].p;
+        for (long long r = 0; r < (long long)c.n * na; r++, at++) {
+          const int *si = c.successors + r * kStencil;
+          const float *sw = c.weights + r * kStencil;
+          int k = 0;
+          for (int q = 0; q < kStencil; q++)
+            if (sw[q] != 0.f) {
+              ids[at * kStencil + k] = si[q];
+              wts[at * kStencil + k] = sw[q];
+              k++;
+            }
+          cnt[at] = (uint8_t)k;
+          for (int q = k; q < kStencil; q++) {  // same row as the last term
+            ids[at * kStencil + q] = k ? ids[at * kStencil + k - 1] : si[0];
+            wts[at * kStencil + q] = 0.f;
+          }
+        }
+      }
+      PE_HIP(hipMemcpy(h->pids, ids.data(), sizeof(int) * ids.size(), hipMemcpyHostToDevice));
+      PE_HIP(hipMemcpy(h->pw, wts.data(), sizeof(float) * wts.size(), hipMemcpyHostToDevice));
+      PE_HIP(hipMalloc((void **)&h->pcount, std::max<size_t>(n, 1)));
+      PE_HIP(hipMemcpy(h->pcount, cnt.data(), n, hipMemcpyHostToDevice));
+    }
     if (h->ne_max < 65536) {
       const long long n = erow * (long long)na;  // records
       PE_HIP(hipMalloc(&h->erec, sizeof(ERec) * std::max(n, 1ll)));
@@ -602,6 +633,7 @@ extern "C" void pe_sl_destroy(pe_sl *h) {
   for (hipEvent_t e : h->events) hipEventDestroy(e);
   for (void *p : {(void *)h->d_maps, (void *)h->pids, (void *)h->eids, (void *)h->erec,
                   (void *)h->ilist, (void *)h->ioff, (void *)h->nlist, (void *)h->noff,
+                  (void *)h->pcount,
                   (void *)h->ppos, (void *)h->epos, (void *)h->pw, (void *)h->ew,
                   (void *)h->pallowed, (void *)h->eallowed, (void *)h->inside,
                   (void *)h->near, (void *)h->order, (void *)h->pstates,

@@ -49,15 +49,35 @@ __global__ __launch_bounds__(256) void sl_initialize(
   }
 }
 
+// First N terms of a compacted P stencil, in order, all loads issued
+// together.
+template <int N, typename T>
+__device__ __forceinline__ T p_sum(const int *__restrict__ id, const float *__restrict__ w,
+                                   const T *__restrict__ Vm, int ne, int e) {
+  T v[N];
+#pragma unroll
+  for (int q = 0; q < N; q++) v[q] = Vm[(long long)id[q] * ne + e];
+  T acc = 0;
+#pragma unroll
+  for (int q = 0; q < N; q++) acc = madd(w[q], v[q], acc);
+  return acc;
+}
+
 // P side: G_u(ip, e) for the near E states e of each P state ip, taken
 // from the list of near E states of ip's position (ascending, so the V rows
 // are still read nearly coalesced) and one thread per list entry; entries
-// of G that no inside pair can reach are not computed.
+// of G that no inside pair can reach are not computed.  The P stencils are
+// compacted: the vertices of non-zero weight first, in vertex order, then
+// zero-weight padding, with their count per (state, command).  About half
+// the weights are zero (illegal vertices, or a coordinate exactly on a grid
+// node), so 4, 8 or 16 terms are summed; adding 0 * V to a sum that starts
+// at +0 leaves it unchanged, so the result is the full 16-term sum.
 template <typename T>
 __global__ __launch_bounds__(256) void sl_p_side(
     const SlMap *__restrict__ maps, const T *__restrict__ V,
     T *__restrict__ G, const int *__restrict__ pids,
-    const float *__restrict__ pw, const uint8_t *__restrict__ pallowed,
+    const float *__restrict__ pw, const uint8_t *__restrict__ pcount,
+    const uint8_t *__restrict__ pallowed,
     const int *__restrict__ ppos, const int *__restrict__ nlist,
     const int *__restrict__ noff) {
   const SlMap m = maps[blockIdx.z];
@@ -73,9 +93,10 @@ __global__ __launch_bounds__(256) void sl_p_side(
       if (!pallowed[cmd]) continue;
       const int *id = pids + cmd * kStencil;
       const float *w = pw + cmd * kStencil;
-      T acc = 0;
-#pragma unroll
-      for (int q = 0; q < kStencil; q++) acc = madd(w[q], Vm[(long long)id[q] * m.ne + e], acc);
+      const int n = pcount[cmd];
+      const T acc = n <= 4   ? p_sum<4>(id, w, Vm, m.ne, e)
+                    : n <= 8 ? p_sum<8>(id, w, Vm, m.ne, e)
+                             : p_sum<kStencil>(id, w, Vm, m.ne, e);
       G[m.g_off + ((long long)u * m.np + ip) * m.ne + e] = acc;
     }
   }
@@ -386,7 +407,7 @@ void enqueue_sweep_t(pe_sl *h, const void *src_, void *dst_, int flags,
   const dim3 grid = pair_grid(h), block(256);
   if (record) sl_record(h, 0, true);
   hipLaunchKernelGGL(sl_p_side<T>, grid, block, 0, h->stream, h->d_maps, src, G,
-                     h->pids, h->pw, h->pallowed, h->ppos, h->nlist, h->noff);
+                     h->pids, h->pw, h->pcount, h->pallowed, h->ppos, h->nlist, h->noff);
   if (record) sl_record(h, 0, false);
   if (record) sl_record(h, 1, true);
   const bool early = flags & PE_SL_EARLY_EXIT, ordered = flags & PE_SL_ORDERED;
