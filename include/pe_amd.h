@@ -193,6 +193,13 @@ typedef struct pe_sl_game {
 #define PE_SL_GRAPH 4      /* replay captured sweeps with a hipGraph        */
 #define PE_SL_GLOBAL_GATHER 8 /* E side gathers G from global memory instead
                                  of LDS-staged rows (comparison runs)      */
+#define PE_SL_ONE_PASS 32     /* with PE_SL_ORDERED: force the one-pass E
+                                 side (one read of each E stencil for all P
+                                 commands); default when the G rows leave
+                                 room for one LDS block per CU            */
+#define PE_SL_U_MAJOR 64      /* with PE_SL_ORDERED: force the u-major E
+                                 side (fewer registers); default when two
+                                 LDS blocks fit one CU                     */
 #define PE_SL_WIDE_IDS 16     /* E side reads the separate 32-bit id and
                                  weight tables instead of the packed
                                  96-byte records of 16-bit ids + weights

@@ -314,8 +314,16 @@ extern "C" int pe_sl_create_ex(const pe_sl_game *games, int32_t n_maps, int32_t 
       const size_t bytes = h->esize * (size_t)na * (size_t)h->ne_max;
       // If the runtime refuses the LDS size, the global-gather E side is used.
       if (bytes + 256 <= (size_t)limit &&
-          (h->f64 ? sl_allow_lds_f64(bytes) : sl_allow_lds<float>(bytes)))
+          (h->f64 ? sl_allow_lds_f64(bytes) : sl_allow_lds<float>(bytes))) {
         h->lds_bytes = bytes;
+        int per_cu = 0;
+        if (hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                                  h->device) != hipSuccess) {
+          (void)hipGetLastError();
+          per_cu = limit;
+        }
+        h->lds_blocks_per_cu = (int)(std::max(per_cu, limit) / (bytes + 256));
+      }
     }
     PE_HIP(hipEventCreate(&h->t0));
     PE_HIP(hipEventCreate(&h->t1));
@@ -482,9 +490,16 @@ extern "C" int pe_sl_initialize_f64(pe_sl *h, const double *v0) {
   return initialize_from(h, v0);
 }
 
+static bool valid_flags(int32_t flags) {
+  const int32_t known = PE_SL_EARLY_EXIT | PE_SL_ORDERED | PE_SL_GRAPH | PE_SL_GLOBAL_GATHER |
+                        PE_SL_WIDE_IDS | PE_SL_ONE_PASS | PE_SL_U_MAJOR;
+  return !(flags & ~known) && !((flags & PE_SL_ONE_PASS) && (flags & PE_SL_U_MAJOR));
+}
+
 extern "C" int pe_sl_sweep(pe_sl *h, int32_t sweeps, int32_t flags) {
   clear_error();
   if (!h || sweeps < 0) return fail(PE_ERR_INVALID, "invalid argument");
+  if (!valid_flags(flags)) return fail(PE_ERR_INVALID, "unknown or conflicting sweep flags");
   PE_HIP(hipSetDevice(h->device));
   return do_sweeps(h, sweeps, flags);
 }
@@ -495,6 +510,7 @@ extern "C" int pe_sl_run(pe_sl *h, double tol, int32_t max_sweeps,
   clear_error();
   if (!h || !(tol >= 0) || max_sweeps < 0 || check_every < 1)
     return fail(PE_ERR_INVALID, "invalid argument");
+  if (!valid_flags(flags)) return fail(PE_ERR_INVALID, "unknown or conflicting sweep flags");
   PE_HIP(hipSetDevice(h->device));
   int done = 0, ok = 0;
   while (done < max_sweeps) {

@@ -184,16 +184,22 @@ struct EStencil {
 // commands reaches the running min cannot lower it (grid_game.py:409-421).
 // Updates the stored command order of the pair.
 //
-// Without ORDERED the commands are visited u-major as in the reference.
-// With ORDERED the stored best P command u* is evaluated first, u-major;
-// then every other allowed P command is carried along one pass over the E
-// commands, so each E stencil is read once for all of them instead of once
-// per P command.  The value and the stored commands are the same: max and
-// min are exact, each (u, b) combination is summed in the same order, ties
-// go to the first command in visiting order, and a command cut against the
+// ORDER selects the visiting order:
+//   kPlainOrder   commands in index order, u-major, as in the reference;
+//   kUMajor       the stored best commands (u*, b*) first, then the others
+//                 in index order, u-major;
+//   kOnePass      u* first, u-major; then every other allowed P command is
+//                 carried along one pass over the E commands, so each E
+//                 stencil is read once for all of them instead of once per P
+//                 command (more registers: ~96 VGPRs against ~48).
+// The value and the stored commands are the same in all three: max and min
+// are exact, each (u, b) combination is summed in the same order, ties go
+// to the first command in visiting order, and a command cut against the
 // bound of u* could not have been chosen later (its max already reaches a
 // value at or above the final min).
-template <bool EARLY, bool ORDERED, typename T, typename Row, typename ID>
+constexpr int kPlainOrder = 0, kUMajor = 1, kOnePass = 2;
+
+template <bool EARLY, int ORDER, typename T, typename Row, typename ID>
 __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T start,
                                          long long pair, Row row,
                                          const ID *__restrict__ eids,
@@ -201,6 +207,7 @@ __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T s
                                          const uint8_t *__restrict__ eallowed,
                                          const uint8_t *__restrict__ pallowed,
                                          uint8_t *__restrict__ order) {
+  constexpr bool ORDERED = ORDER != kPlainOrder;
   T best = start;
   int ufirst = 0, bfirst = 0;
   if (ORDERED) {
@@ -213,8 +220,8 @@ __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T s
   const long long cmd0 = (long long)erow * m.na;
   const uint8_t *pa = pallowed + (long long)prow * m.na;
   int best_u = -1, best_b = -1;
-  for (int i = 0; i < (ORDERED ? 1 : m.na); i++) {
-    const int u = ORDERED ? ufirst : i;
+  for (int i = 0; i < (ORDER == kOnePass ? 1 : m.na); i++) {
+    const int u = ORDERED ? nth_command(i, ufirst) : i;
     if (!pa[u]) continue;
     const T *Gu = row(u);
     T worst = T(-1e30);
@@ -241,7 +248,7 @@ __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T s
       best_b = arg;
     }
   }
-  if constexpr (ORDERED) {
+  if constexpr (ORDER == kOnePass) {
     unsigned alive = 0;
 #pragma unroll
     for (int u = 0; u < kMaxActions; u++)
@@ -275,8 +282,8 @@ __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T s
         best_b = (int)((args >> (4 * u)) & 15);
       }
     }
-    if (best_u >= 0) order[pair] = (uint8_t)(best_u | (best_b << 4));
   }
+  if (ORDERED && best_u >= 0) order[pair] = (uint8_t)(best_u | (best_b << 4));
   return best;
 }
 
@@ -296,7 +303,7 @@ __device__ __forceinline__ void block_residual(T change, unsigned long long *slo
 }
 
 // E side with G gathered from global memory: one thread per pair.
-template <typename T, bool EARLY, bool ORDERED, typename ID>
+template <typename T, bool EARLY, int ORDER, typename ID>
 __global__ __launch_bounds__(256) void sl_e_side(
     const SlMap *__restrict__ maps, const T *__restrict__ V,
     T *__restrict__ Vn, const T *__restrict__ G,
@@ -318,7 +325,7 @@ __global__ __launch_bounds__(256) void sl_e_side(
       const int prow = m.p_row + ip;
       if (inside[m.tab_off + (long long)ppos[prow] * m.npos_e + ep]) {
         auto row = [&](int u) { return G + m.g_off + ((long long)u * m.np + ip) * m.ne; };
-        best = pair_backup<EARLY, ORDERED>(m, prow, erow, start, pair, row, eids, ew,
+        best = pair_backup<EARLY, ORDER>(m, prow, erow, start, pair, row, eids, ew,
                                            eallowed, pallowed, order);
       }
       Vn[pair] = best;
@@ -337,7 +344,7 @@ __global__ __launch_bounds__(256) void sl_e_side(
 // of the P state's position, so every lane has a pair to update; the pairs
 // outside the exit radius are not written (both value buffers already hold
 // the same exit value there).  Same arithmetic as sl_e_side, value for value.
-template <typename T, bool EARLY, bool ORDERED, typename ID>
+template <typename T, bool EARLY, int ORDER, typename ID>
 __global__ __launch_bounds__(1024) void sl_e_side_lds(
     const SlMap *__restrict__ maps, const T *__restrict__ V,
     T *__restrict__ Vn, const T *__restrict__ G,
@@ -371,7 +378,7 @@ __global__ __launch_bounds__(1024) void sl_e_side_lds(
       const int erow = m.e_row + ie;
       const long long pair = m.v_off + (long long)ip * m.ne + ie;
       const T start = V[pair];
-      const T best = pair_backup<EARLY, ORDERED>(m, prow, erow, start, pair, row, eids, ew,
+      const T best = pair_backup<EARLY, ORDER>(m, prow, erow, start, pair, row, eids, ew,
                                                  eallowed, pallowed, order);
       Vn[pair] = best;
       const T d = best > start ? best - start : start - best;
@@ -414,6 +421,10 @@ void enqueue_sweep_t(pe_sl *h, const void *src_, void *dst_, int flags,
   const bool lds = h->lds_bytes > 0 && !(flags & PE_SL_GLOBAL_GATHER);
   const dim3 lds_grid((unsigned)h->np_max, 1, (unsigned)h->B), lds_block(1024);
   const bool packed = h->erec && !(flags & PE_SL_WIDE_IDS);
+  // One pass over the E commands when its registers cost no occupancy: the
+  // G rows leave room for one 1024-thread block per CU either way.
+  const bool one_pass = (flags & PE_SL_ONE_PASS) ||
+                        (!(flags & PE_SL_U_MAJOR) && !(lds && h->lds_blocks_per_cu >= 2));
 #define PE_E_SIDE_ID(E, O, ID, IDS)                                                    \
   if (lds)                                                                             \
     hipLaunchKernelGGL((sl_e_side_lds<T, E, O, ID>), lds_grid, lds_block, h->lds_bytes,  \
@@ -431,10 +442,12 @@ void enqueue_sweep_t(pe_sl *h, const void *src_, void *dst_, int flags,
       PE_E_SIDE_ID(E, O, int, h->eids);                            \
     }                                                              \
   } while (0)
-  if (early && ordered) PE_E_SIDE(true, true);
-  else if (early) PE_E_SIDE(true, false);
-  else if (ordered) PE_E_SIDE(false, true);
-  else PE_E_SIDE(false, false);
+  if (early && ordered && one_pass) PE_E_SIDE(true, kOnePass);
+  else if (early && ordered) PE_E_SIDE(true, kUMajor);
+  else if (early) PE_E_SIDE(true, kPlainOrder);
+  else if (ordered && one_pass) PE_E_SIDE(false, kOnePass);
+  else if (ordered) PE_E_SIDE(false, kUMajor);
+  else PE_E_SIDE(false, kPlainOrder);
 #undef PE_E_SIDE
 #undef PE_E_SIDE_ID
   if (record) sl_record(h, 1, false);
@@ -467,14 +480,18 @@ __global__ __launch_bounds__(256) inline void sl_pack_erec(const int *__restrict
 template <typename T>
 bool sl_allow_lds(size_t bytes) {
   bool ok = true;
-  for (const void *f : {(const void *)sl_e_side_lds<T, true, true, int>,
-                        (const void *)sl_e_side_lds<T, true, false, int>,
-                        (const void *)sl_e_side_lds<T, false, true, int>,
-                        (const void *)sl_e_side_lds<T, false, false, int>,
-                        (const void *)sl_e_side_lds<T, true, true, ERec>,
-                        (const void *)sl_e_side_lds<T, true, false, ERec>,
-                        (const void *)sl_e_side_lds<T, false, true, ERec>,
-                        (const void *)sl_e_side_lds<T, false, false, ERec>})
+  for (const void *f : {(const void *)sl_e_side_lds<T, true, kOnePass, int>,
+                        (const void *)sl_e_side_lds<T, true, kUMajor, int>,
+                        (const void *)sl_e_side_lds<T, true, kPlainOrder, int>,
+                        (const void *)sl_e_side_lds<T, false, kOnePass, int>,
+                        (const void *)sl_e_side_lds<T, false, kUMajor, int>,
+                        (const void *)sl_e_side_lds<T, false, kPlainOrder, int>,
+                        (const void *)sl_e_side_lds<T, true, kOnePass, ERec>,
+                        (const void *)sl_e_side_lds<T, true, kUMajor, ERec>,
+                        (const void *)sl_e_side_lds<T, true, kPlainOrder, ERec>,
+                        (const void *)sl_e_side_lds<T, false, kOnePass, ERec>,
+                        (const void *)sl_e_side_lds<T, false, kUMajor, ERec>,
+                        (const void *)sl_e_side_lds<T, false, kPlainOrder, ERec>})
     ok = ok && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) ==
                    hipSuccess;
   if (!ok) (void)hipGetLastError();

@@ -182,6 +182,8 @@ PE_SL_ORDERED = 2
 PE_SL_GRAPH = 4
 PE_SL_GLOBAL_GATHER = 8
 PE_SL_WIDE_IDS = 16
+PE_SL_ONE_PASS = 32
+PE_SL_U_MAJOR = 64
 
 class World8DConfig(ctypes.Structure):
     _fields_ = [

@@ -40,6 +40,7 @@ pytestmark = pytest.mark.gpu
 EARLY, ORDERED, GRAPH = _native.PE_SL_EARLY_EXIT, _native.PE_SL_ORDERED, _native.PE_SL_GRAPH
 GLOBAL = _native.PE_SL_GLOBAL_GATHER
 WIDE = _native.PE_SL_WIDE_IDS
+ONE_PASS, U_MAJOR = _native.PE_SL_ONE_PASS, _native.PE_SL_U_MAJOR
 CONTROLS = np.array([[a, k] for a in (-3, 0, 3) for k in (-1, 0, 1)], float)
 
 
@@ -125,7 +126,12 @@ def run_variants(tables, sweeps):
     for name, flags in [("plain", 0), ("early", EARLY), ("ordered", EARLY | ORDERED),
                         ("graph", EARLY | ORDERED | GRAPH), ("global", EARLY | ORDERED | GLOBAL),
                         ("global_plain", GLOBAL), ("wide_ids", EARLY | ORDERED | WIDE),
-                        ("wide_ids_global", EARLY | ORDERED | GLOBAL | WIDE)]:
+                        ("wide_ids_global", EARLY | ORDERED | GLOBAL | WIDE),
+                        ("one_pass", EARLY | ORDERED | ONE_PASS),
+                        ("u_major", EARLY | ORDERED | U_MAJOR),
+                        ("one_pass_full", ORDERED | ONE_PASS),
+                        ("u_major_global", EARLY | ORDERED | U_MAJOR | GLOBAL),
+                        ("one_pass_global_wide", EARLY | ORDERED | ONE_PASS | GLOBAL | WIDE)]:
         with GameSolver(tables) as s:
             s.sweep(sweeps, flags)
             out[name] = (s.read(), s.trace()[:, 0])
@@ -140,7 +146,8 @@ def test_early_exit_ordering_and_graph_are_exact(gpu):
     res = run_variants(tables, 6)
     base, trace = res["plain"]
     for name in ("early", "ordered", "graph", "global", "global_plain", "wide_ids",
-                 "wide_ids_global"):
+                 "wide_ids_global", "one_pass", "u_major", "one_pass_full", "u_major_global",
+                 "one_pass_global_wide"):
         np.testing.assert_array_equal(res[name][0], base, err_msg=name)
         np.testing.assert_array_equal(res[name][1], trace, err_msg=name)
 
@@ -172,6 +179,30 @@ def test_small_map_sweeps_match_oracle(gpu, oracle_lib, small_game):
     assert d.max() <= 1e-5
     flip = np.flatnonzero((got.ravel() <= 0) != (V.ravel() <= 0))
     assert (np.abs(V.ravel()[flip]) <= 1e-5).all()
+
+
+def test_small_map_visiting_orders_are_exact(gpu, small_game):
+    """On a real map (structured zero weights, walls, a stored command order
+    that changes as values settle), the one-pass and u-major E sides and
+    the plain index order give the same values over 300 sweeps."""
+    tables = small_game[4]
+    out = {}
+    for name, flags in [("plain", 0), ("one_pass", EARLY | ORDERED | ONE_PASS | GRAPH),
+                        ("u_major", EARLY | ORDERED | U_MAJOR | GRAPH)]:
+        with GameSolver(tables) as s:
+            s.sweep(300, flags)
+            out[name] = (s.read(), s.trace()[:, 0])
+    for name in ("one_pass", "u_major"):
+        np.testing.assert_array_equal(out[name][0], out["plain"][0], err_msg=name)
+        np.testing.assert_array_equal(out[name][1], out["plain"][1], err_msg=name)
+
+
+def test_conflicting_sweep_flags_are_rejected(gpu, small_game):
+    with GameSolver(small_game[4]) as s:
+        with pytest.raises(ValueError):
+            s.sweep(1, EARLY | ORDERED | ONE_PASS | U_MAJOR)
+        with pytest.raises(ValueError):
+            s.sweep(1, 1 << 20)
 
 
 def test_small_map_converged_matches_fp64_oracle(gpu, oracle_lib, small_game):
