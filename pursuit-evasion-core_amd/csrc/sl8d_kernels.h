@@ -163,6 +163,24 @@ struct EStencil {
       for (int c = 0; c < 4; c++) w[c] = w4[c];
     }
   }
+  // Two stencils over the same row at once (two independent chains).
+  template <typename T>
+  __device__ __forceinline__ static void eval2(const EStencil &a, const EStencil &b,
+                                               const T *Gu, T &qa, T &qb) {
+    qa = 0;
+    qb = 0;
+#pragma unroll
+    for (int c = 0; c < kStencil / 4; c++) {
+      qa = madd(a.w[c].x, Gu[a.id[4 * c]], qa);
+      qb = madd(b.w[c].x, Gu[b.id[4 * c]], qb);
+      qa = madd(a.w[c].y, Gu[a.id[4 * c + 1]], qa);
+      qb = madd(b.w[c].y, Gu[b.id[4 * c + 1]], qb);
+      qa = madd(a.w[c].z, Gu[a.id[4 * c + 2]], qa);
+      qb = madd(b.w[c].z, Gu[b.id[4 * c + 2]], qb);
+      qa = madd(a.w[c].w, Gu[a.id[4 * c + 3]], qa);
+      qb = madd(b.w[c].w, Gu[b.id[4 * c + 3]], qb);
+    }
+  }
   // sum_k w_k G[id_k], vertex order (grid_game.py:406-408)
   template <typename T>
   __device__ __forceinline__ T eval(const T *Gu) const {
@@ -220,7 +238,51 @@ __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T s
   const long long cmd0 = (long long)erow * m.na;
   const uint8_t *pa = pallowed + (long long)prow * m.na;
   int best_u = -1, best_b = -1;
-  for (int i = 0; i < (ORDER == kOnePass ? 1 : m.na); i++) {
+  if constexpr (ORDER == kOnePass) {
+    // u* over its E commands two at a time: two stencil reads in flight and
+    // two independent sums; the second result is dropped when the first
+    // already cuts, so the visited sequence is the sequential one.
+    if (pa[ufirst]) {
+      const T *Gu = row(ufirst);
+      unsigned ea = 0;  // allowed E commands, bit j = j-th in visiting order
+      for (int j = 0; j < m.na; j++)
+        if (eallowed[cmd0 + nth_command(j, bfirst)]) ea |= 1u << j;
+      T worst = T(-1e30);
+      int arg = -1;
+      bool cut = false;
+      while (ea && !cut) {
+        const int b0 = nth_command(__builtin_ctz(ea), bfirst);
+        ea &= ea - 1;
+        const bool two = ea != 0;
+        const int b1 = two ? nth_command(__builtin_ctz(ea), bfirst) : b0;
+        if (two) ea &= ea - 1;
+        EStencil<ID> s0, s1;
+        s0.load(eids, ew, cmd0 + b0);
+        s1.load(eids, ew, cmd0 + b1);
+        T q0, q1;
+        EStencil<ID>::eval2(s0, s1, Gu, q0, q1);
+        if (q0 > worst) {
+          worst = q0;
+          arg = b0;
+        }
+        if (EARLY && worst >= best) {
+          cut = true;
+        } else if (two) {
+          if (q1 > worst) {
+            worst = q1;
+            arg = b1;
+          }
+          if (EARLY && worst >= best) cut = true;
+        }
+      }
+      if (!cut && worst < best) {
+        best = worst;
+        best_u = ufirst;
+        best_b = arg;
+      }
+    }
+  }
+  for (int i = 0; i < (ORDER == kOnePass ? 0 : m.na); i++) {
     const int u = ORDERED ? nth_command(i, ufirst) : i;
     if (!pa[u]) continue;
     const T *Gu = row(u);
