@@ -406,8 +406,10 @@ __global__ __launch_bounds__(256) void sl_e_side(
 // of the P state's position, so every lane has a pair to update; the pairs
 // outside the exit radius are not written (both value buffers already hold
 // the same exit value there).  Same arithmetic as sl_e_side, value for value.
+constexpr int kLdsThreads = 1024;  // block size of sl_e_side_lds
+
 template <typename T, bool EARLY, int ORDER, typename ID>
-__global__ __launch_bounds__(1024) void sl_e_side_lds(
+__global__ __launch_bounds__(kLdsThreads) void sl_e_side_lds(
     const SlMap *__restrict__ maps, const T *__restrict__ V,
     T *__restrict__ Vn, const T *__restrict__ G,
     const ID *__restrict__ eids, const float *__restrict__ ew,
@@ -422,11 +424,30 @@ __global__ __launch_bounds__(1024) void sl_e_side_lds(
   const bool active = ip < m.np;
   const int prow = m.p_row + ip;
   if (active) {
-    for (int u = 0; u < m.na; u++) {
-      if (!pallowed[(long long)prow * m.na + u]) continue;
-      const T *__restrict__ src = G + m.g_off + ((long long)u * m.np + ip) * m.ne;
-      T *dst = tile + u * m.ne;
-      for (int e = threadIdx.x; e < m.ne; e += blockDim.x) dst[e] = src[e];
+    // All na rows (rows of disallowed commands are read but never used),
+    // flattened as tile[u * ne + e], eight loads in flight per thread.
+    const int total = m.na * m.ne;
+    const T *__restrict__ src = G + m.g_off + (long long)ip * m.ne;
+    const long long ustride = (long long)m.np * m.ne;
+    constexpr int kBatch = 8;
+    int u = threadIdx.x / m.ne, e = threadIdx.x - u * m.ne;  // of t0
+    for (int t0 = threadIdx.x; t0 < total; t0 += kBatch * kLdsThreads) {
+      T v[kBatch];
+      int uu = u, ee = e;
+#pragma unroll
+      for (int b = 0; b < kBatch; b++) {
+        if (t0 + b * kLdsThreads < total) v[b] = src[uu * ustride + ee];
+        ee += kLdsThreads;
+        while (ee >= m.ne) {
+          ee -= m.ne;
+          uu++;
+        }
+      }
+#pragma unroll
+      for (int b = 0; b < kBatch; b++)
+        if (t0 + b * kLdsThreads < total) tile[t0 + b * kLdsThreads] = v[b];
+      u = uu;
+      e = ee;
     }
   }
   __syncthreads();
@@ -481,7 +502,7 @@ void enqueue_sweep_t(pe_sl *h, const void *src_, void *dst_, int flags,
   if (record) sl_record(h, 1, true);
   const bool early = flags & PE_SL_EARLY_EXIT, ordered = flags & PE_SL_ORDERED;
   const bool lds = h->lds_bytes > 0 && !(flags & PE_SL_GLOBAL_GATHER);
-  const dim3 lds_grid((unsigned)h->np_max, 1, (unsigned)h->B), lds_block(1024);
+  const dim3 lds_grid((unsigned)h->np_max, 1, (unsigned)h->B), lds_block(kLdsThreads);
   const bool packed = h->erec && !(flags & PE_SL_WIDE_IDS);
   // One pass over the E commands when its registers cost no occupancy: the
   // G rows leave room for one 1024-thread block per CU either way.
