@@ -63,17 +63,23 @@ __device__ __forceinline__ T p_sum(const int *__restrict__ id, const float *__re
   return acc;
 }
 
-// P side: G_u(ip, e) for the near E states e of each P state ip, taken
-// from the list of near E states of ip's position (ascending, so the V rows
-// are still read nearly coalesced) and one thread per list entry; entries
-// of G that no inside pair can reach are not computed.  The P stencils are
+// P side: G_u(ip, e) for the near E states e of each P state ip: blocks
+// (ip, chunk) walk the list of near E states of ip's position (ascending,
+// so the V rows are still read nearly coalesced) in gridDim.y strided
+// chunks, loading the P stencils once per block; entries of G that no
+// inside pair can reach are not computed.  Few chunks when there are many
+// P states (each block loops longer on its stencils), more when there are
+// few (enough blocks to fill the GPU).  The P stencils are
 // compacted: the vertices of non-zero weight first, in vertex order, then
 // zero-weight padding, with their count per (state, command).  About half
 // the weights are zero (illegal vertices, or a coordinate exactly on a grid
 // node), so 4, 8 or 16 terms are summed; adding 0 * V to a sum that starts
 // at +0 leaves it unchanged, so the result is the full 16-term sum.
+constexpr int kPThreads = 256;             // block size of sl_p_side
+constexpr long long kPSideBlocks = 16384;  // P-side blocks to aim for
+
 template <typename T>
-__global__ __launch_bounds__(256) void sl_p_side(
+__global__ __launch_bounds__(kPThreads) void sl_p_side(
     const SlMap *__restrict__ maps, const T *__restrict__ V,
     T *__restrict__ G, const int *__restrict__ pids,
     const float *__restrict__ pw, const uint8_t *__restrict__ pcount,
@@ -81,23 +87,24 @@ __global__ __launch_bounds__(256) void sl_p_side(
     const int *__restrict__ ppos, const int *__restrict__ nlist,
     const int *__restrict__ noff) {
   const SlMap m = maps[blockIdx.z];
-  const int k = blockIdx.x * 256 + threadIdx.x;
+  const int ip = blockIdx.x;  // blockIdx.y of gridDim.y strided chunks of the list
+  if (ip >= m.np) return;
   const T *__restrict__ Vm = V + m.v_off;
-  for (int ip = blockIdx.y; ip < m.np; ip += gridDim.y) {
-    const int row = m.p_row + ip;
-    const int k0 = noff[m.ipos_off + ppos[row]];
-    if (k >= noff[m.ipos_off + ppos[row] + 1] - k0) continue;
-    const int e = nlist[k0 + k];
-    for (int u = 0; u < m.na; u++) {
-      const long long cmd = (long long)row * m.na + u;
-      if (!pallowed[cmd]) continue;
-      const int *id = pids + cmd * kStencil;
-      const float *w = pw + cmd * kStencil;
-      const int n = pcount[cmd];
-      const T acc = n <= 4   ? p_sum<4>(id, w, Vm, m.ne, e)
-                    : n <= 8 ? p_sum<8>(id, w, Vm, m.ne, e)
-                             : p_sum<kStencil>(id, w, Vm, m.ne, e);
-      G[m.g_off + ((long long)u * m.np + ip) * m.ne + e] = acc;
+  const int row = m.p_row + ip;
+  const int k0 = noff[m.ipos_off + ppos[row]];
+  const int count = noff[m.ipos_off + ppos[row] + 1] - k0;
+  for (int u = 0; u < m.na; u++) {
+    const long long cmd = (long long)row * m.na + u;
+    if (!pallowed[cmd]) continue;
+    const int *id = pids + cmd * kStencil;
+    const float *w = pw + cmd * kStencil;
+    const int n = pcount[cmd];
+    T *__restrict__ Gu = G + m.g_off + ((long long)u * m.np + ip) * m.ne;
+    for (int k = blockIdx.y * kPThreads + threadIdx.x; k < count; k += gridDim.y * kPThreads) {
+      const int e = nlist[k0 + k];
+      Gu[e] = n <= 4   ? p_sum<4>(id, w, Vm, m.ne, e)
+              : n <= 8 ? p_sum<8>(id, w, Vm, m.ne, e)
+                       : p_sum<kStencil>(id, w, Vm, m.ne, e);
     }
   }
 }
@@ -496,7 +503,12 @@ void enqueue_sweep_t(pe_sl *h, const void *src_, void *dst_, int flags,
   T *G = reinterpret_cast<T *>(h->G);
   const dim3 grid = pair_grid(h), block(256);
   if (record) sl_record(h, 0, true);
-  hipLaunchKernelGGL(sl_p_side<T>, grid, block, 0, h->stream, h->d_maps, src, G,
+  const long long p_blocks = (long long)h->np_max * h->B;
+  const int chunks = (int)std::max(
+      1ll, std::min((long long)(h->ne_max + kPThreads - 1) / kPThreads,
+                    (kPSideBlocks + p_blocks - 1) / p_blocks));
+  hipLaunchKernelGGL(sl_p_side<T>, dim3((unsigned)h->np_max, (unsigned)chunks, (unsigned)h->B),
+                     dim3(kPThreads), 0, h->stream, h->d_maps, src, G,
                      h->pids, h->pw, h->pcount, h->pallowed, h->ppos, h->nlist, h->noff);
   if (record) sl_record(h, 0, false);
   if (record) sl_record(h, 1, true);
