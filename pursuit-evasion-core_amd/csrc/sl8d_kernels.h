@@ -5,6 +5,7 @@
 // a multiply and an add (oracle/sl8d.c follows local_game/grid_game.py).
 #pragma once
 
+#include <algorithm>
 #include <type_traits>
 
 #include "pe_internal.h"
@@ -505,8 +506,8 @@ void enqueue_sweep_t(pe_sl *h, const void *src_, void *dst_, int flags,
   if (record) sl_record(h, 0, true);
   const long long p_blocks = (long long)h->np_max * h->B;
   const int chunks = (int)std::max(
-      1ll, std::min((long long)(h->ne_max + kPThreads - 1) / kPThreads,
-                    (kPSideBlocks + p_blocks - 1) / p_blocks));
+      1ll, std::min({(long long)(h->ne_max + kPThreads - 1) / kPThreads,
+                     (kPSideBlocks + p_blocks - 1) / p_blocks, 65535ll}));
   hipLaunchKernelGGL(sl_p_side<T>, dim3((unsigned)h->np_max, (unsigned)chunks, (unsigned)h->B),
                      dim3(kPThreads), 0, h->stream, h->d_maps, src, G,
                      h->pids, h->pw, h->pcount, h->pallowed, h->ppos, h->nlist, h->noff);
