@@ -191,6 +191,28 @@ def run_reader_c1(cfg, n_queries: int = 1_000_000, cpu_queries: int = 20_000):
     return out
 
 
+def c2_cpu_baseline(p, e, tables, v0, sweeps: int):
+    """cpu_baseline leg of the C2 line: the oracle's FP64 OpenMP Jacobi
+    sweep (the restatement of the reference's backup) timed for a few sweeps
+    of the same map from the same initial field."""
+    import numpy as np
+    import oracle
+    from pe_amd.local_game.grid_game import dense_inside
+
+    threads = min(os.cpu_count() or 1, 16)
+    inside = dense_inside(p, e, tables["exit_value"])
+    V = np.asarray(v0, np.float64)
+    t0 = time.perf_counter()
+    for _ in range(sweeps):
+        V, _ = oracle.sl_sweep(V, p, e, inside, threads=threads)
+    seconds = time.perf_counter() - t0
+    legal = len(p.states) * len(e.states)
+    return {"value": legal * sweeps / seconds, "unit": "legal-pair updates/s",
+            "cores": threads, "kind": "port",
+            "sample": f"{sweeps} FP64 Jacobi sweeps of the same map from the payoff "
+                      f"({seconds:.2f} s, OpenMP)"}
+
+
 def run_c2(args):
     """8D contact game C2: sweeps of the compact min-max backup."""
     from pe_amd import _native
@@ -203,6 +225,7 @@ def run_c2(args):
     flags = _native.PE_SL_EARLY_EXIT | _native.PE_SL_ORDERED | _native.PE_SL_GRAPH
     with GameSolver(tables) as s:
         st0 = s.stats()
+        v0 = s.read()
         if args.warmup:
             s.sweep(args.warmup, flags)
         t1 = time.perf_counter()
@@ -239,6 +262,8 @@ def run_c2(args):
             done, ok = s.run(args.to_residual, args.max_sweeps, 16, flags)
             out.update(to_residual=args.to_residual, sweeps_to_residual=done, converged=ok,
                        seconds_to_residual=time.perf_counter() - t2)
+    if getattr(args, "cpu_sweeps", 0):
+        out["cpu_baseline"] = c2_cpu_baseline(p, e, tables, v0, args.cpu_sweeps)
     return out
 
 
@@ -407,7 +432,7 @@ def main() -> int:
                 "last_residual": st["last_residual"]}
         print("[bench] secondary: 8D contact game (seed 0)", file=sys.stderr, flush=True)
         c2 = argparse.Namespace(seed=0, warmup=10, steps=200, to_residual=1e-4,
-                                max_sweeps=20000)
+                                max_sweeps=20000, cpu_sweeps=0 if args.no_cpu_baseline else 48)
         secondary = {"hji5d_c1_to_residual": res5, "contact_game_c2": run_c2(c2)}
         print("[bench] secondary: dense 8D square game", file=sys.stderr, flush=True)
         secondary["world8d_default"] = run_world8d(
