@@ -238,6 +238,8 @@ def run_c2(args):
         prof_g = s.stats()
         s.sweep(min(args.steps, 20), (flags & ~_native.PE_SL_GRAPH) | _native.PE_SL_WIDE_IDS)
         prof_w = s.stats()
+        s.sweep(min(args.steps, 20), _native.PE_SL_EARLY_EXIT)
+        prof_r = s.stats()
         s.set_profiling(False)
         out = {
             "workload": "C2 8D contact game, 11x11 positions (+-3 m) x 12 headings x 5 speeds "
@@ -254,6 +256,10 @@ def run_c2(args):
                                        max(1, prof_g["e_launches"] - prof["e_launches"]),
             "e_side_unpacked_tables_ms": (prof_w["e_ms"] - prof_g["e_ms"]) /
                                    max(1, prof_w["e_launches"] - prof_g["e_launches"]),
+            # The reference's early exit in index order, without the stored
+            # command order.
+            "e_side_reference_order_ms": (prof_r["e_ms"] - prof_w["e_ms"]) /
+                                         max(1, prof_r["e_launches"] - prof_w["e_launches"]),
             "residual_after": s.stats()["last_residual"],
         }
         if args.to_residual:
@@ -292,6 +298,12 @@ def run_c4(args):
         s.sweep(args.steps, flags)
         wall = time.perf_counter() - t1
         res = s.trace()[-1]
+        s.set_profiling(True)
+        s.sweep(20, flags & ~_native.PE_SL_GRAPH)
+        prof = s.stats()
+        s.sweep(20, _native.PE_SL_EARLY_EXIT)
+        prof_r = s.stats()
+        s.set_profiling(False)
         out = {
             "workload": f"C4 {args.maps} seeded 30-rectangle maps, 9x9 positions x 8 headings x "
                         "4 speeds per car, batched, hipGraph-replayed sweeps",
@@ -301,6 +313,12 @@ def run_c4(args):
             "legal_pair_updates_per_s": st0["pairs"] * args.steps / wall,
             "active_pair_updates_per_s": st0["active_pairs"] * args.steps / wall,
             "max_residual_after": float(res.max()),
+            "p_side_ms": prof["p_ms"] / max(1, prof["p_launches"]),
+            "e_side_ms": prof["e_ms"] / max(1, prof["e_launches"]),
+            # The reference's early exit in index order, without the stored
+            # command order.
+            "e_side_reference_order_ms": (prof_r["e_ms"] - prof["e_ms"]) /
+                                         max(1, prof_r["e_launches"] - prof["e_launches"]),
         }
         if args.to_residual:
             s.initialize()
