@@ -10,11 +10,12 @@
 // and per chunk of 256 P states and stages every chunk through host memory;
 // here one sweep is two launches over all maps of a batch, entirely in HBM:
 //
-//   sl_p_side  G_u(ip, e) for all u at once, one thread per (ip, e), e
-//              fastest so the 16 V rows it gathers are read coalesced; the
-//              P tables of (ip, u) are wave-uniform (scalar loads).  Only
-//              entries an inside pair can reach ("near" table) are computed:
-//              the threads walk a list of the near E states of ip's position.
+//   sl_p_side  G_u(ip, e) for all u, blocks of (ip, strided chunk of the
+//              list of near E states of ip's position): only entries an
+//              inside pair can reach are computed, e ascending so the V rows
+//              are read nearly coalesced; the P tables of (ip, u) are
+//              block-uniform (scalar loads) and compacted to their non-zero
+//              weights (4-, 8- or 16-term sums).
 //   sl_e_side  one thread per pair (ip, ie), ie fastest: the min over P
 //              commands of the max over E commands of the 16-point E
 //              interpolation of G_u, with the reference's exact early exit
@@ -28,6 +29,12 @@
 //              states of the P state's position, so the 16-point E gathers
 //              hit LDS instead of one cache line per lane.  Used whenever
 //              the rows fit; bit-identical to sl_e_side.
+//
+// Both E sides read each (E state, command) stencil as one packed 96-byte
+// record (16-bit ids, then the weights) when every map has fewer than
+// 65,536 E states, and visit the commands in one of two orders (one pass
+// over the E commands for all P commands, or u-major), chosen per handle
+// from the LDS occupancy; all choices give the same values.
 //
 // G holds [maps][u][ip][e] float32 (9 x the field) and is zero-filled once:
 // entries skipped by sl_p_side are never read with a non-zero weight, and
