@@ -197,6 +197,27 @@ def test_small_map_visiting_orders_are_exact(gpu, small_game):
         np.testing.assert_array_equal(out[name][1], out["plain"][1], err_msg=name)
 
 
+def test_given_field_outside_the_radius_is_kept(gpu, small_game):
+    """An initial field whose pairs beyond the exit radius hold another
+    value than the exit value: those pairs keep it through an odd number of
+    sweeps (both value buffers), and the LDS E side (which writes inside
+    pairs only) gives the same field as the global-gather E side (which
+    rewrites every pair)."""
+    config, arena, p, e, tables = small_game
+    inside = dense_inside(p, e, 3.0)
+    with GameSolver(tables) as s:
+        v0 = s.read().copy()
+    v0[~inside] = 7.0
+    out = {}
+    for name, flags in [("lds", EARLY | ORDERED), ("global", EARLY | ORDERED | GLOBAL)]:
+        with GameSolver(tables) as s:
+            s.initialize(v0)
+            s.sweep(3, flags)
+            out[name] = s.read()
+    assert (out["lds"][~inside] == 7.0).all()
+    np.testing.assert_array_equal(out["lds"], out["global"])
+
+
 def test_conflicting_sweep_flags_are_rejected(gpu, small_game):
     with GameSolver(small_game[4]) as s:
         with pytest.raises(ValueError):
