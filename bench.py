@@ -321,16 +321,27 @@ def run_c4(args):
                                          max(1, prof_r["e_launches"] - prof["e_launches"]),
         }
         if args.to_residual:
+            import numpy as np
+
             s.initialize()
+            # Each map stops being swept at the first check (every 16 sweeps)
+            # where its own residual is at most the tolerance, as if solved
+            # alone; --no-freeze sweeps every map until the last converges.
+            freeze = not args.no_freeze
+            print(f"[bench] C4 to residual {args.to_residual} (freeze {freeze})",
+                  file=sys.stderr, flush=True)
             t2 = time.perf_counter()
-            done, ok = 0, False
-            while done < args.max_sweeps and not ok:
-                d, ok = s.run(args.to_residual, min(512, args.max_sweeps - done), 16, flags)
-                done += d
-                print(f"[bench] C4 {done} sweeps, max residual {s.trace()[-1].max():.3e}",
-                      file=sys.stderr, flush=True)
-            out.update(to_residual=args.to_residual, sweeps_to_residual=done, converged=ok,
-                       seconds_to_residual=time.perf_counter() - t2)
+            done, ok = s.run(args.to_residual, args.max_sweeps, 16,
+                             flags | (_native.PE_SL_FREEZE if freeze else 0))
+            seconds = time.perf_counter() - t2
+            checks = s.trace()[15::16]
+            stops = [16 * (int(np.argmax(checks[:, k] <= args.to_residual)) + 1)
+                     for k in range(args.maps)]
+            print(f"[bench] C4 {done} sweeps, {seconds:.1f} s", file=sys.stderr, flush=True)
+            out.update(to_residual=args.to_residual, freeze_converged_maps=freeze,
+                       sweeps_to_residual=done, converged=ok, seconds_to_residual=seconds,
+                       map_sweeps_to_residual={"min": min(stops), "median": int(np.median(stops)),
+                                               "max": max(stops), "total": int(sum(stops))})
     return out
 
 
@@ -349,6 +360,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--to-residual", type=float, default=None)
     ap.add_argument("--max-sweeps", type=int, default=20000)
+    ap.add_argument("--no-freeze", action="store_true",
+                    help="C4: sweep every map until the last one converges")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="after the run, copy this many bytes once (counter calibration)")
     args = ap.parse_args()

@@ -200,6 +200,10 @@ typedef struct pe_sl_game {
 #define PE_SL_U_MAJOR 64      /* with PE_SL_ORDERED: force the u-major E
                                  side (fewer registers); default when two
                                  LDS blocks fit one CU                     */
+#define PE_SL_FREEZE 128      /* pe_sl_run only: a map whose residual is at
+                                 most tol at a check is no longer swept in
+                                 that call (each map stops as if solved
+                                 alone); all maps are swept again after   */
 #define PE_SL_WIDE_IDS 16     /* E side reads the separate 32-bit id and
                                  weight tables instead of the packed
                                  96-byte records of 16-bit ids + weights

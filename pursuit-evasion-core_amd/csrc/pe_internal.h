@@ -21,6 +21,7 @@ struct SlMap {
   long long tab_off;    // offset in the inside / near tables
   int p_row, e_row;     // first state row in the concatenated car tables
   int ipos_off;         // first entry of this map's P positions in the inside lists
+  int active;           // swept (1), or frozen by a converged pe_sl_run (0)
   float exit_value, pl, pw, el, ew;
 };
 

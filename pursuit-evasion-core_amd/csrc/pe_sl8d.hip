@@ -270,6 +270,7 @@ extern "C" int pe_sl_create_ex(const pe_sl_game *games, int32_t n_maps, int32_t 
     m.p_row = (int)prow;
     m.e_row = (int)erow;
     m.ipos_off = (int)ioff.size();
+    m.active = 1;
     m.exit_value = g.exit_value;
     m.pl = g.p_length;
     m.pw = g.p_width;
@@ -497,9 +498,32 @@ extern "C" int pe_sl_initialize_f64(pe_sl *h, const double *v0) {
   return initialize_from(h, v0);
 }
 
+// Freezes every still-swept map whose residual in the last sweep is at most
+// tol: its current field is copied into the other value buffer (so the
+// ping-pong keeps it) and the sweep kernels skip it from then on.
+static int freeze_converged(pe_sl *h, double tol, bool *frozen) {
+  const double *last = h->trace.data() + h->trace.size() - h->B;
+  bool changed = false;
+  for (int b = 0; b < h->B; b++) {
+    SlMap &m = h->maps[b];
+    if (!m.active || !(last[b] <= tol)) continue;
+    m.active = 0;
+    changed = true;
+    const size_t off = h->esize * (size_t)m.v_off, bytes = h->esize * (size_t)m.np * m.ne;
+    PE_HIP(hipMemcpyAsync(reinterpret_cast<char *>(h->vb) + off,
+                          reinterpret_cast<const char *>(h->va) + off, bytes,
+                          hipMemcpyDeviceToDevice, h->stream));
+  }
+  if (!changed) return PE_OK;
+  PE_HIP(hipStreamSynchronize(h->stream));
+  PE_HIP(hipMemcpy(h->d_maps, h->maps.data(), sizeof(SlMap) * h->B, hipMemcpyHostToDevice));
+  *frozen = true;
+  return PE_OK;
+}
+
 static bool valid_flags(int32_t flags) {
   const int32_t known = PE_SL_EARLY_EXIT | PE_SL_ORDERED | PE_SL_GRAPH | PE_SL_GLOBAL_GATHER |
-                        PE_SL_WIDE_IDS | PE_SL_ONE_PASS | PE_SL_U_MAJOR;
+                        PE_SL_WIDE_IDS | PE_SL_ONE_PASS | PE_SL_U_MAJOR | PE_SL_FREEZE;
   return !(flags & ~known) && !((flags & PE_SL_ONE_PASS) && (flags & PE_SL_U_MAJOR));
 }
 
@@ -519,17 +543,27 @@ extern "C" int pe_sl_run(pe_sl *h, double tol, int32_t max_sweeps,
     return fail(PE_ERR_INVALID, "invalid argument");
   if (!valid_flags(flags)) return fail(PE_ERR_INVALID, "unknown or conflicting sweep flags");
   PE_HIP(hipSetDevice(h->device));
-  int done = 0, ok = 0;
+  const bool freeze = flags & PE_SL_FREEZE;
+  flags &= ~PE_SL_FREEZE;
+  int done = 0, ok = 0, rc = PE_OK;
+  bool frozen = false;
   while (done < max_sweeps) {
     const int chunk = std::min(check_every, max_sweeps - done);
-    int rc = do_sweeps(h, chunk, flags);
-    if (rc) return rc;
+    if ((rc = do_sweeps(h, chunk, flags))) break;
     done += chunk;
     if (h->last <= tol) {
       ok = 1;
       break;
     }
+    if (freeze && (rc = freeze_converged(h, tol, &frozen))) break;
   }
+  if (frozen) {  // every map is swept again after this call
+    for (SlMap &m : h->maps) m.active = 1;
+    const hipError_t e = hipMemcpy(h->d_maps, h->maps.data(), sizeof(SlMap) * h->B,
+                                   hipMemcpyHostToDevice);
+    if (!rc && e != hipSuccess) rc = fail(PE_ERR_HIP, hipGetErrorString(e));
+  }
+  if (rc) return rc;
   if (sweeps_done) *sweeps_done = done;
   if (converged) *converged = ok;
   return PE_OK;

@@ -89,7 +89,7 @@ __global__ __launch_bounds__(kPThreads) void sl_p_side(
     const int *__restrict__ noff) {
   const SlMap m = maps[blockIdx.z];
   const int ip = blockIdx.x;  // blockIdx.y of gridDim.y strided chunks of the list
-  if (ip >= m.np) return;
+  if (ip >= m.np || !m.active) return;
   const T *__restrict__ Vm = V + m.v_off;
   const int row = m.p_row + ip;
   const int k0 = noff[m.ipos_off + ppos[row]];
@@ -383,6 +383,7 @@ __global__ __launch_bounds__(256) void sl_e_side(
     const uint8_t *__restrict__ inside, uint8_t *__restrict__ order,
     unsigned long long *__restrict__ slot) {
   const SlMap m = maps[blockIdx.z];
+  if (!m.active) return;  // the whole block: no residual for a frozen map
   const int ie = blockIdx.x * 256 + threadIdx.x;
   T change = 0;
   if (ie < m.ne) {
@@ -428,6 +429,7 @@ __global__ __launch_bounds__(kLdsThreads) void sl_e_side_lds(
   extern __shared__ __align__(16) unsigned char tile_bytes[];
   T *tile = reinterpret_cast<T *>(tile_bytes);
   const SlMap m = maps[blockIdx.z];
+  if (!m.active) return;  // the whole block: no residual for a frozen map
   const int ip = blockIdx.x;  // uniform per block
   const bool active = ip < m.np;
   const int prow = m.p_row + ip;

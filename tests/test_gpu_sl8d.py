@@ -41,6 +41,7 @@ EARLY, ORDERED, GRAPH = _native.PE_SL_EARLY_EXIT, _native.PE_SL_ORDERED, _native
 GLOBAL = _native.PE_SL_GLOBAL_GATHER
 WIDE = _native.PE_SL_WIDE_IDS
 ONE_PASS, U_MAJOR = _native.PE_SL_ONE_PASS, _native.PE_SL_U_MAJOR
+FREEZE = _native.PE_SL_FREEZE
 CONTROLS = np.array([[a, k] for a in (-3, 0, 3) for k in (-1, 0, 1)], float)
 
 
@@ -323,6 +324,41 @@ def test_batched_maps_equal_individual_solves(gpu):
             s1.sweep(10, EARLY | ORDERED)
             np.testing.assert_array_equal(s1.read(), batched[k])
             np.testing.assert_array_equal(s1.trace()[:, 0], traces[:, k])
+
+
+def test_freeze_stops_each_map_as_if_solved_alone(gpu):
+    """pe_sl_run with PE_SL_FREEZE on a batch: each map stops being swept at
+    the first check where its own residual is at most tol, so its field and
+    its stopping sweep equal a run of that map alone; afterwards every map
+    is swept again, from the field it stopped at."""
+    from pe_amd.local_game.reference import benchmark_game
+
+    print("[gpu] freeze converged maps", flush=True)
+    tables = [benchmark_game(1000 + k, positions=7, headings=8, speeds=3, rectangles=12)[4]
+              for k in range(4)]
+    flags = EARLY | ORDERED | GRAPH
+    tol, every = 1e-4, 16
+    alone, stops, after = [], [], []
+    for t in tables:
+        with GameSolver(t) as s1:
+            done, ok = s1.run(tol, 20_000, every, flags)
+            assert ok
+            alone.append(s1.read())
+            stops.append(done)
+            s1.sweep(6, flags)
+            after.append(s1.read())
+    assert len(set(stops)) > 1, "maps should converge at different sweeps"
+    with GameSolver(tables) as s:
+        done, ok = s.run(tol, 20_000, every, flags | FREEZE)
+        assert ok and done == max(stops)
+        trace = s.trace()
+        for k in range(4):
+            np.testing.assert_array_equal(s.read(k), alone[k], err_msg=f"map {k}")
+            # swept up to its own stop, residual zero (not swept) after it
+            assert (trace[:stops[k], k] > 0).any() and (trace[stops[k]:, k] == 0).all()
+        s.sweep(6, flags)
+        for k in range(4):
+            np.testing.assert_array_equal(s.read(k), after[k], err_msg=f"map {k} after")
 
 
 @pytest.mark.parametrize("flags", [0, EARLY | ORDERED, EARLY | ORDERED | GRAPH,
