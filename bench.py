@@ -360,6 +360,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--to-residual", type=float, default=None)
     ap.add_argument("--max-sweeps", type=int, default=20000)
+    ap.add_argument("--no-c4", action="store_true",
+                    help="leave the 128-map C4 line out of the secondary object")
     ap.add_argument("--no-freeze", action="store_true",
                     help="C4: sweep every map until the last one converges")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -465,10 +467,12 @@ def main() -> int:
         c2 = argparse.Namespace(seed=0, warmup=10, steps=200, to_residual=1e-4,
                                 max_sweeps=20000, cpu_sweeps=0 if args.no_cpu_baseline else 48)
         secondary = {"hji5d_c1_to_residual": res5, "contact_game_c2": run_c2(c2)}
-        print("[bench] secondary: 8D contact game, 128 maps (C4)", file=sys.stderr, flush=True)
-        c4 = argparse.Namespace(maps=128, warmup=10, steps=100, to_residual=1e-4,
-                                max_sweeps=20000, no_freeze=False)
-        secondary["contact_game_c4"] = run_c4(c4)
+        if not args.no_c4:
+            print("[bench] secondary: 8D contact game, 128 maps (C4)", file=sys.stderr,
+                  flush=True)
+            c4 = argparse.Namespace(maps=128, warmup=10, steps=100, to_residual=1e-4,
+                                    max_sweeps=20000, no_freeze=False)
+            secondary["contact_game_c4"] = run_c4(c4)
         print("[bench] secondary: dense 8D square game", file=sys.stderr, flush=True)
         secondary["world8d_default"] = run_world8d(
             (9, 8, 7, 7), 40, 2, 0 if args.no_cpu_baseline else 2)
