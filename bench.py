@@ -250,7 +250,7 @@ def run_c2(args):
             "legal_pair_updates_per_s": st0["pairs"] * args.steps / wall,
             "active_pair_updates_per_s": st0["active_pairs"] * args.steps / wall,
             "p_side_ms": prof["p_ms"] / max(1, prof["p_launches"]),
-            "e_side_lds": bool(st0["lds_e_side"]),
+            "e_side_lds_threads": st0["lds_e_side"],
             "e_side_ms": prof["e_ms"] / max(1, prof["e_launches"]),
             "e_side_global_gather_ms": (prof_g["e_ms"] - prof["e_ms"]) /
                                        max(1, prof_g["e_launches"] - prof["e_launches"]),

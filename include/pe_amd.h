@@ -193,13 +193,11 @@ typedef struct pe_sl_game {
 #define PE_SL_GRAPH 4      /* replay captured sweeps with a hipGraph        */
 #define PE_SL_GLOBAL_GATHER 8 /* E side gathers G from global memory instead
                                  of LDS-staged rows (comparison runs)      */
-#define PE_SL_ONE_PASS 32     /* with PE_SL_ORDERED: force the one-pass E
-                                 side (one read of each E stencil for all P
-                                 commands); default when the G rows leave
-                                 room for one LDS block per CU            */
-#define PE_SL_U_MAJOR 64      /* with PE_SL_ORDERED: force the u-major E
-                                 side (fewer registers); default when two
-                                 LDS blocks fit one CU                     */
+#define PE_SL_ONE_PASS 32     /* with PE_SL_ORDERED: the one-pass E side
+                                 (one read of each E stencil for all P
+                                 commands); the default                    */
+#define PE_SL_U_MAJOR 64      /* with PE_SL_ORDERED: the u-major E side
+                                 (fewer registers; comparison runs)        */
 #define PE_SL_FREEZE 128      /* pe_sl_run only: a map whose residual is at
                                  most tol at a check is no longer swept in
                                  that call (each map stops as if solved
@@ -212,7 +210,9 @@ typedef struct pe_sl_game {
 typedef struct pe_sl_stats {
   int64_t sweeps;
   int32_t maps;
-  int32_t lds_e_side;     /* 1: the E side stages G rows in LDS         */
+  int32_t lds_e_side;     /* threads per block of the E side that stages
+                             G rows in LDS (256, 512 or 1024, chosen per
+                             handle); 0: G is gathered from global memory */
   int64_t pairs;          /* stored pairs, all maps                         */
   int64_t active_pairs;   /* pairs inside the exit radius, all maps         */
   double update_seconds;  /* device time of the sweeps                      */

@@ -50,7 +50,7 @@ struct pe_sl {
   bool f64 = false;      // FP64 values (va, vb, G hold doubles)
   size_t esize = 4;      // bytes per value
   size_t lds_bytes = 0;  // G rows of one P state, [na][ne_max] floats (0: no LDS E side)
-  int lds_blocks_per_cu = 0;  // 1024-thread LDS E-side blocks that fit one CU's LDS
+  int lds_threads = 1024;  // threads per block of the LDS E side
   long long pairs = 0, g_size = 0, active = 0;
   // car tables
   int *pids = nullptr, *eids = nullptr, *ppos = nullptr, *epos = nullptr;

@@ -257,6 +257,7 @@ extern "C" int pe_sl_create_ex(const pe_sl_game *games, int32_t n_maps, int32_t 
   h->na = na;
   long long prow = 0, erow = 0, voff = 0, goff = 0, toff = 0;
   std::vector<int> ilist, ioff, nlist, noff;
+  long long lds_slots[3] = {0, 0, 0};
   for (int b = 0; b < n_maps; b++) {
     const pe_sl_game &g = games[b];
     SlMap m{};
@@ -297,7 +298,14 @@ extern "C" int pe_sl_create_ex(const pe_sl_game *games, int32_t n_maps, int32_t 
         if (g.near[t]) nlist.push_back(j);
       }
     }
-    for (int i = 0; i < m.np; i++) h->active += count[g.p.position[i]];
+    for (int i = 0; i < m.np; i++) {
+      const int c = count[g.p.position[i]];
+      h->active += c;
+      for (int k = 0; k < 3; k++) {  // lane slots of E-side blocks of 256 << k threads
+        const long long t = 256ll << k;
+        lds_slots[k] += (c + t - 1) / t * t;
+      }
+    }
   }
   ioff.push_back((int)std::min<size_t>(ilist.size(), INT32_MAX));
   noff.push_back((int)std::min<size_t>(nlist.size(), INT32_MAX));
@@ -324,13 +332,12 @@ extern "C" int pe_sl_create_ex(const pe_sl_game *games, int32_t n_maps, int32_t 
       if (bytes + 256 <= (size_t)limit &&
           (h->f64 ? sl_allow_lds_f64(bytes) : sl_allow_lds<float>(bytes))) {
         h->lds_bytes = bytes;
-        int per_cu = 0;
-        if (hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
-                                  h->device) != hipSuccess) {
-          (void)hipGetLastError();
-          per_cu = limit;
+        if (!h->f64) {
+          double fill[3];
+          for (int k = 0; k < 3; k++)
+            fill[k] = lds_slots[k] ? (double)h->active / (double)lds_slots[k] : 1.0;
+          h->lds_threads = sl_lds_threads(bytes, fill, h->ne_max < 65536);
         }
-        h->lds_blocks_per_cu = (int)(std::max(per_cu, limit) / (bytes + 256));
       }
     }
     PE_HIP(hipEventCreate(&h->t0));
@@ -661,7 +668,7 @@ extern "C" int pe_sl_get_stats(pe_sl *h, pe_sl_stats *s) {
   std::memset(s, 0, sizeof(*s));
   s->sweeps = h->sweeps;
   s->maps = h->B;
-  s->lds_e_side = h->lds_bytes > 0;
+  s->lds_e_side = h->lds_bytes > 0 ? h->lds_threads : 0;
   s->pairs = h->pairs;
   s->active_pairs = h->active;
   s->update_seconds = h->update_seconds;

@@ -6,6 +6,7 @@
 #pragma once
 
 #include <algorithm>
+#include <cmath>
 #include <type_traits>
 
 #include "pe_internal.h"
@@ -415,10 +416,12 @@ __global__ __launch_bounds__(256) void sl_e_side(
 // of the P state's position, so every lane has a pair to update; the pairs
 // outside the exit radius are not written (both value buffers already hold
 // the same exit value there).  Same arithmetic as sl_e_side, value for value.
-constexpr int kLdsThreads = 1024;  // block size of sl_e_side_lds
+// sl_e_side_lds runs with 256, 512 or 1024 threads per block (chosen per
+// handle from the occupancy and the pairs per P state).
+constexpr int kLdsMaxThreads = 1024;
 
 template <typename T, bool EARLY, int ORDER, typename ID>
-__global__ __launch_bounds__(kLdsThreads) void sl_e_side_lds(
+__global__ __launch_bounds__(kLdsMaxThreads) void sl_e_side_lds(
     const SlMap *__restrict__ maps, const T *__restrict__ V,
     T *__restrict__ Vn, const T *__restrict__ G,
     const ID *__restrict__ eids, const float *__restrict__ ew,
@@ -440,14 +443,15 @@ __global__ __launch_bounds__(kLdsThreads) void sl_e_side_lds(
     const T *__restrict__ src = G + m.g_off + (long long)ip * m.ne;
     const long long ustride = (long long)m.np * m.ne;
     constexpr int kBatch = 8;
+    const int nt = (int)blockDim.x;
     int u = threadIdx.x / m.ne, e = threadIdx.x - u * m.ne;  // of t0
-    for (int t0 = threadIdx.x; t0 < total; t0 += kBatch * kLdsThreads) {
+    for (int t0 = threadIdx.x; t0 < total; t0 += kBatch * nt) {
       T v[kBatch];
       int uu = u, ee = e;
 #pragma unroll
       for (int b = 0; b < kBatch; b++) {
-        if (t0 + b * kLdsThreads < total) v[b] = src[uu * ustride + ee];
-        ee += kLdsThreads;
+        if (t0 + b * nt < total) v[b] = src[uu * ustride + ee];
+        ee += nt;
         while (ee >= m.ne) {
           ee -= m.ne;
           uu++;
@@ -455,7 +459,7 @@ __global__ __launch_bounds__(kLdsThreads) void sl_e_side_lds(
       }
 #pragma unroll
       for (int b = 0; b < kBatch; b++)
-        if (t0 + b * kLdsThreads < total) tile[t0 + b * kLdsThreads] = v[b];
+        if (t0 + b * nt < total) tile[t0 + b * nt] = v[b];
       u = uu;
       e = ee;
     }
@@ -517,12 +521,10 @@ void enqueue_sweep_t(pe_sl *h, const void *src_, void *dst_, int flags,
   if (record) sl_record(h, 1, true);
   const bool early = flags & PE_SL_EARLY_EXIT, ordered = flags & PE_SL_ORDERED;
   const bool lds = h->lds_bytes > 0 && !(flags & PE_SL_GLOBAL_GATHER);
-  const dim3 lds_grid((unsigned)h->np_max, 1, (unsigned)h->B), lds_block(kLdsThreads);
+  const dim3 lds_grid((unsigned)h->np_max, 1, (unsigned)h->B), lds_block(h->lds_threads);
   const bool packed = h->erec && !(flags & PE_SL_WIDE_IDS);
-  // One pass over the E commands when its registers cost no occupancy: the
-  // G rows leave room for one 1024-thread block per CU either way.
-  const bool one_pass = (flags & PE_SL_ONE_PASS) ||
-                        (!(flags & PE_SL_U_MAJOR) && !(lds && h->lds_blocks_per_cu >= 2));
+  // One pass over the E commands unless u-major is asked for.
+  const bool one_pass = !(flags & PE_SL_U_MAJOR);
 #define PE_E_SIDE_ID(E, O, ID, IDS)                                                    \
   if (lds)                                                                             \
     hipLaunchKernelGGL((sl_e_side_lds<T, E, O, ID>), lds_grid, lds_block, h->lds_bytes,  \
@@ -594,6 +596,33 @@ bool sl_allow_lds(size_t bytes) {
                    hipSuccess;
   if (!ok) (void)hipGetLastError();
   return ok;
+}
+
+// Threads per block of the LDS E side for FP32 values in the default
+// (one-pass) order: of 1024, 512 and 256, the size that keeps the most
+// lanes busy, counting the blocks one CU holds at this kernel's registers
+// and `bytes` of LDS, times the fraction of lane slots that the inside
+// pairs of the P states fill (fill[k] for blocks of 256 << k threads); ties
+// go to the smaller block.
+inline int sl_lds_threads(size_t bytes, const double fill[3], bool packed) {
+  const void *f = packed ? (const void *)sl_e_side_lds<float, true, kOnePass, ERec>
+                         : (const void *)sl_e_side_lds<float, true, kOnePass, int>;
+  int best = kLdsMaxThreads;
+  double best_score = -1;
+  for (int k = 0; k < 3; k++) {
+    const int t = 256 << k;
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, f, t, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      continue;
+    }
+    const double score = blocks * t * fill[k];
+    if (score > best_score * 1.0001) {
+      best_score = score;
+      best = t;
+    }
+  }
+  return best;
 }
 
 // FP64 instances (pe_sl8d_f64.hip, built without contraction).
