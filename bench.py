@@ -279,16 +279,15 @@ def run_c4(args):
     from a captured hipGraph."""
     from pe_amd import _native
     from pe_amd.local_game.grid_game import GameSolver
-    from pe_amd.local_game.reference import benchmark_game
+    from pe_amd.local_game.reference import benchmark_games
 
     t0 = time.perf_counter()
-    tables = []
-    for k in range(args.maps):
-        tables.append(benchmark_game(1000 + k, positions=9, headings=8, speeds=4)[4])
-        if k % 16 == 15:
-            print(f"[bench] C4 tables {k + 1}/{args.maps} ({time.perf_counter() - t0:.1f} s)",
-                  file=sys.stderr, flush=True)
+    # All maps' geometry in one launch per phase, then per-map host stencils.
+    tables = [g[4] for g in benchmark_games(range(1000, 1000 + args.maps), positions=9,
+                                            headings=8, speeds=4)]
     build_s = time.perf_counter() - t0
+    print(f"[bench] C4 tables for {args.maps} maps ({build_s:.1f} s)", file=sys.stderr,
+          flush=True)
     flags = _native.PE_SL_EARLY_EXIT | _native.PE_SL_ORDERED | _native.PE_SL_GRAPH
     with GameSolver(tables) as s:
         st0 = s.stats()

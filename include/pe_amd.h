@@ -143,6 +143,21 @@ int pe_geom_actions(const double *states, int64_t n, const double *controls,
                     int32_t na, double dt, const double *obstacles, int32_t nc,
                     double half, const double *param, uint8_t *ok,
                     double *next);
+/* pe_geom_backups and pe_geom_actions for the states of many maps in one
+   launch (no counterpart in the reference, which builds one map per call):
+   state i belongs to map map_of[i] in [0, n_maps), whose circles are
+   obstacles[obstacle_offsets[b] .. obstacle_offsets[b + 1]) (offsets
+   ascending from 0, n_maps + 1 entries) and whose room half-width is
+   halves[b].  Results equal the single-map calls bit for bit. */
+int pe_geom_backups_multi(const double *states, int64_t n, const int32_t *map_of,
+                          int32_t n_maps, const double *obstacles,
+                          const int64_t *obstacle_offsets, const double *halves,
+                          const double *param, uint8_t *out);
+int pe_geom_actions_multi(const double *states, int64_t n, const int32_t *map_of,
+                          int32_t n_maps, const double *controls, int32_t na, double dt,
+                          const double *obstacles, const int64_t *obstacle_offsets,
+                          const double *halves, const double *param, uint8_t *ok,
+                          double *next);
 /* configuration_connections, native_geometry.cpp:237-252 */
 int pe_geom_connections(const double *from, const double *to, int64_t n,
                         const double *obstacles, int32_t nc, double half,

@@ -254,6 +254,8 @@ SIGNATURES = {
     "pe_geom_clearance": (ctypes.c_int, [P, I64, P, I32, F64, P, P]),
     "pe_geom_backups": (ctypes.c_int, [P, I64, P, I32, F64, P, P]),
     "pe_geom_actions": (ctypes.c_int, [P, I64, P, I32, F64, P, I32, F64, P, P, P]),
+    "pe_geom_backups_multi": (ctypes.c_int, [P, I64, P, I32, P, P, P, P, P]),
+    "pe_geom_actions_multi": (ctypes.c_int, [P, I64, P, I32, P, I32, F64, P, P, P, P, P, P]),
     "pe_geom_connections": (ctypes.c_int, [P, P, I64, P, I32, F64, P, P]),
     "pe_sl_create": (ctypes.c_int, [ctypes.POINTER(SLGame), I32, P]),
     "pe_sl_create_ex": (ctypes.c_int, [ctypes.POINTER(SLGame), I32, I32, P]),

@@ -183,22 +183,43 @@ def make_grid(arena, limits, role, *, position_step=0.3, headings=8, speeds=None
               positions=None, half_width=None):
     """Legal states, commands {-A,0,A} x {-K,0,K} (acceleration-major) and the
     successor stencils of one car (grid_game.py:158-196)."""
+    return make_grids([arena], limits, role, position_step=position_step, headings=headings,
+                      speeds=speeds, dt=dt, positions=positions, half_width=half_width)[0]
+
+
+def make_grids(arenas, limits, role, *, position_step=0.3, headings=8, speeds=None, dt=0.05,
+               positions=None, half_width=None):
+    """make_grid for several arenas of one room size with one car model: the
+    legality and the action tables of all maps each run as one GPU launch
+    (pe_geom_backups_multi, pe_geom_actions_multi), then each map's
+    stencils are formed on the host.  Equal to make_grid map by map."""
+    from .native import actions_many, backup_safe_many
+
     if speeds is None:
         speeds = [-1, 0, 0.5, 1, 2] if role == "pursuer" else [-0.5, 0, 0.25, 0.5, 1]
-    half = arena.half_width if half_width is None else half_width
+    halves = {a.half_width for a in arenas}
+    if half_width is None and len(halves) != 1:
+        raise ValueError("One room half-width is required for a batch of arenas")
+    half = arenas[0].half_width if half_width is None else half_width
     axes = grid_axes(half, position_step=position_step, positions=positions,
                      headings=headings, speeds=speeds)
     states = np.stack(np.meshgrid(*axes, indexing="ij"), -1).reshape(-1, 4)
-    engine = GeometryEngine(arena, limits, role, control_dt=dt)
-    legal = engine.backup_safe(states)
-    mapping = np.full(len(states), -1, np.int32)
-    mapping[legal] = np.arange(legal.sum(), dtype=np.int32)
-    kept = states[legal]
+    engines = [GeometryEngine(a, limits, role, control_dt=dt) for a in arenas]
+    legal_all = backup_safe_many(engines, [states] * len(arenas))
     a, k = limits.acceleration, limits.max_curvature
     controls = np.array([[u, c] for u in (-a, 0, a) for c in (-k, 0, k)], float)
+    kept_all = [states[legal] for legal in legal_all]
+    acts = actions_many(engines, kept_all, controls, dt)
+    return [_finish_grid(axes, states, legal, kept, controls, allowed, nxt, arena, limits,
+                         role, dt)
+            for legal, kept, (allowed, nxt), arena in zip(legal_all, kept_all, acts, arenas)]
+
+
+def _finish_grid(axes, states, legal, kept, controls, allowed, nxt, arena, limits, role, dt):
+    mapping = np.full(len(states), -1, np.int32)
+    mapping[legal] = np.arange(legal.sum(), dtype=np.int32)
     grid = VehicleGrid(axes, kept, mapping, controls, None, None, None, None)
     grid.bind_geometry(arena, limits, role, dt)
-    allowed, nxt = engine.actions(kept, controls, dt)
     idx, w, valid, _, mass = grid.stencil(nxt)
     grid.successors = idx.reshape(len(kept), len(controls), 16)
     grid.weights = w.reshape(len(kept), len(controls), 16)

@@ -73,3 +73,45 @@ class GeometryEngine:
             _native.ptr(self.obstacles), len(self.obstacles), float(self.arena.half_width),
             _native.ptr(self.params), _native.ptr(ok), _native.ptr(nxt)))
         return ok.astype(bool), nxt
+
+
+def _many(engines, state_lists):
+    """Concatenated states, their map indices, the circles of every map with
+    offsets, and the half-widths, for the multi-map queries (one car model
+    for all maps)."""
+    params = engines[0].params
+    if any(not np.array_equal(e.params, params) for e in engines):
+        raise ValueError("One vehicle model is required for a multi-map query")
+    xs = [GeometryEngine._states(s) for s in state_lists]
+    x = np.ascontiguousarray(np.concatenate(xs) if xs else np.zeros((0, 4)))
+    map_of = np.concatenate([np.full(len(v), b, np.int32) for b, v in enumerate(xs)])
+    obs = np.ascontiguousarray(np.concatenate([e.obstacles for e in engines]), np.float64)
+    off = np.concatenate([[0], np.cumsum([len(e.obstacles) for e in engines])]).astype(np.int64)
+    halves = np.array([e.arena.half_width for e in engines], np.float64)
+    return x, np.ascontiguousarray(map_of), obs, off, halves, params, [len(v) for v in xs]
+
+
+def backup_safe_many(engines, state_lists):
+    """GeometryEngine.backup_safe of several maps in one launch
+    (pe_geom_backups_multi); returns one mask per map."""
+    x, map_of, obs, off, halves, params, sizes = _many(engines, state_lists)
+    out = np.empty(len(x), np.uint8)
+    _native.check(_native.lib().pe_geom_backups_multi(
+        _native.ptr(x), len(x), _native.ptr(map_of), len(engines), _native.ptr(obs),
+        _native.ptr(off), _native.ptr(halves), _native.ptr(params), _native.ptr(out)))
+    return np.split(out.astype(bool), np.cumsum(sizes)[:-1])
+
+
+def actions_many(engines, state_lists, controls, dt):
+    """GeometryEngine.actions of several maps in one launch
+    (pe_geom_actions_multi); returns (ok, next) per map."""
+    x, map_of, obs, off, halves, params, sizes = _many(engines, state_lists)
+    u = np.ascontiguousarray(controls, float).reshape(-1, 2)
+    ok = np.empty((len(x), len(u)), np.uint8)
+    nxt = np.empty((len(x), len(u), 4))
+    _native.check(_native.lib().pe_geom_actions_multi(
+        _native.ptr(x), len(x), _native.ptr(map_of), len(engines), _native.ptr(u), len(u),
+        float(dt), _native.ptr(obs), _native.ptr(off), _native.ptr(halves),
+        _native.ptr(params), _native.ptr(ok), _native.ptr(nxt)))
+    cut = np.cumsum(sizes)[:-1]
+    return list(zip(np.split(ok.astype(bool), cut), np.split(nxt, cut)))

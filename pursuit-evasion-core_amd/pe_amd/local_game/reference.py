@@ -336,6 +336,28 @@ def benchmark_game(seed, *, positions=11, headings=12, speeds=5, half_width=3.0,
     return config, arena, p, e, game_tables(p, e, arena, exit_radius)
 
 
+def benchmark_games(seeds, *, positions=11, headings=12, speeds=5, half_width=3.0,
+                    exit_radius=3.0, rectangles=30):
+    """benchmark_game for several seeds, with the geometry of all maps in one
+    GPU launch per phase (make_grids); equal to benchmark_game seed by
+    seed.  Returns a list of (config, arena, p, e, tables)."""
+    from .geometry import Footprint, random_rectangle_arena
+    from .grid_game import make_grids
+
+    config = contact_config()
+    body = Footprint(0.5, 0.3)
+    arenas = [random_rectangle_arena(seed, pursuer=body, evader=body, count=rectangles,
+                                     half_width=half_width) for seed in seeds]
+    kw = dict(headings=headings, positions=positions, half_width=half_width,
+              dt=config.control_dt)
+    ps = make_grids(arenas, config.pursuer, "pursuer", speeds=np.linspace(
+        config.pursuer.speed_min, config.pursuer.speed_max, speeds), **kw)
+    es = make_grids(arenas, config.evader, "evader", speeds=np.linspace(
+        config.evader.speed_min, config.evader.speed_max, speeds), **kw)
+    return [(config, a, p, e, game_tables(p, e, a, exit_radius))
+            for a, p, e in zip(arenas, ps, es)]
+
+
 class ContactReference:
     """Reader of a completed solve with retained history (reference.py:372-454).
 

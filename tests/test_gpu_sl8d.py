@@ -326,6 +326,38 @@ def test_batched_maps_equal_individual_solves(gpu):
             np.testing.assert_array_equal(s1.trace()[:, 0], traces[:, k])
 
 
+def test_multi_map_table_build_equals_single_maps(gpu):
+    """benchmark_games (legality and action tables of all maps in one launch
+    per phase, pe_geom_*_multi) gives the tables of benchmark_game map by
+    map, bit for bit."""
+    from pe_amd.local_game.reference import benchmark_game, benchmark_games
+
+    print("[gpu] multi-map table build", flush=True)
+    seeds = [1000, 1001, 1002]
+    many = benchmark_games(seeds, positions=7, headings=8, speeds=3, rectangles=12)
+    for seed, (_, _, p, e, t) in zip(seeds, many):
+        _, _, p1, e1, t1 = benchmark_game(seed, positions=7, headings=8, speeds=3, rectangles=12)
+        for a, b in ((p, p1), (e, e1)):
+            for name in ("states", "dense_to_compact", "successors", "weights", "allowed"):
+                np.testing.assert_array_equal(getattr(a, name), getattr(b, name),
+                                              err_msg=f"seed {seed} {name}")
+        np.testing.assert_array_equal(t["inside"], t1["inside"])
+        np.testing.assert_array_equal(t["near"], t1["near"])
+    # The multi-map kernels against the single-map entry points.
+    from pe_amd.local_game.native import actions_many, backup_safe_many
+
+    engines = [g[2]._engine for g in many]
+    dense = np.stack(np.meshgrid(*many[0][2].axes, indexing="ij"), -1).reshape(-1, 4)
+    for eng, legal in zip(engines, backup_safe_many(engines, [dense] * len(engines))):
+        np.testing.assert_array_equal(legal, eng.backup_safe(dense))
+    kept = [g[2].states for g in many]
+    controls = many[0][2].controls
+    for eng, x, (ok, nxt) in zip(engines, kept, actions_many(engines, kept, controls, 0.05)):
+        ok1, nxt1 = eng.actions(x, controls, 0.05)
+        np.testing.assert_array_equal(ok, ok1)
+        np.testing.assert_array_equal(nxt, nxt1)
+
+
 def test_freeze_stops_each_map_as_if_solved_alone(gpu):
     """pe_sl_run with PE_SL_FREEZE on a batch: each map stops being swept at
     the first check where its own residual is at most tol, so its field and
