@@ -73,3 +73,46 @@ def test_struct_mirrors_match_the_c_layout():
     lib = native.lib()
     for name, mirror in native.STRUCTS.items():
         assert lib.pe_abi_struct_size(name.encode()) == ctypes.sizeof(mirror), name
+
+
+def test_multi_map_geometry_arguments_rejected_before_any_device_work():
+    """pe_geom_*_multi check the map list (indices in range, offsets
+    ascending from 0, finite positive half-widths) before any device work."""
+    import numpy as np
+    import pe_amd._native as native
+
+    lib = native.lib()
+    P = native.ptr
+    states = np.zeros((3, 4))
+    param = np.array([-1.0, 2.0, 3.0, 1.0, 0.5, 0.3, 0.05])
+    obstacles = np.zeros((2, 3))
+    out = np.empty(3, np.uint8)
+
+    def backups(map_of, offsets, halves, n_maps=2):
+        m = np.ascontiguousarray(map_of, np.int32)
+        o = np.ascontiguousarray(offsets, np.int64)
+        h = np.ascontiguousarray(halves, np.float64)
+        return lib.pe_geom_backups_multi(P(states), len(states), P(m), n_maps, P(obstacles),
+                                         P(o), P(h), P(param), P(out))
+
+    for map_of, offsets, halves, text in [
+            ([0, 1, 2], [0, 1, 2], [3.0, 3.0], b"map index"),
+            ([0, -1, 1], [0, 1, 2], [3.0, 3.0], b"map index"),
+            ([0, 1, 1], [1, 1, 2], [3.0, 3.0], b"start at 0"),
+            ([0, 1, 1], [0, 2, 1], [3.0, 3.0], b"ascend"),
+            ([0, 1, 1], [0, 1, 2], [3.0, 0.0], b"half-width"),
+            ([0, 1, 1], [0, 1, 2], [3.0, np.nan], b"half-width")]:
+        rc = backups(map_of, offsets, halves)
+        assert rc == native.PE_ERR_INVALID, text
+        assert text in lib.pe_last_error()
+        with pytest.raises(ValueError):
+            native.check(rc)
+    controls = np.zeros((9, 2))
+    ok = np.empty((3, 9), np.uint8)
+    nxt = np.empty((3, 9, 4))
+    m = np.array([0, 1, 1], np.int32)
+    o = np.array([0, 1, 2], np.int64)
+    h = np.array([3.0, 3.0])
+    rc = lib.pe_geom_actions_multi(P(states), 3, P(m), 2, P(controls), 0, 0.05, P(obstacles),
+                                   P(o), P(h), P(param), P(ok), P(nxt))
+    assert rc == native.PE_ERR_INVALID and b"controls" in lib.pe_last_error()
