@@ -271,6 +271,10 @@ int pe_sl_query(pe_sl *h, int32_t map, int64_t n, const int32_t *p_ids,
 int pe_sl_trace(pe_sl *h, double *out, int64_t capacity, int64_t *count);
 int pe_sl_get_stats(pe_sl *h, pe_sl_stats *out);
 int pe_sl_set_profiling(pe_sl *h, int32_t on);
+/* Threads per block of the LDS E side (256, 512 or 1024), replacing the
+   size chosen at creation; no reference counterpart (a tuning and test
+   knob: every size gives the same values). */
+int pe_sl_set_lds_threads(pe_sl *h, int32_t threads);
 void pe_sl_destroy(pe_sl *h);
 
 /* ======================================================================

@@ -689,6 +689,20 @@ extern "C" int pe_sl_set_profiling(pe_sl *h, int32_t on) {
   return PE_OK;
 }
 
+extern "C" int pe_sl_set_lds_threads(pe_sl *h, int32_t threads) {
+  clear_error();
+  if (!h) return fail(PE_ERR_INVALID, "null handle");
+  if (threads != 256 && threads != 512 && threads != 1024)
+    return fail(PE_ERR_INVALID, "LDS E-side blocks have 256, 512 or 1024 threads");
+  if (h->graph) {  // captured launches carry the old block size
+    PE_HIP(hipStreamSynchronize(h->stream));
+    PE_HIP(hipGraphExecDestroy(h->graph));
+    h->graph = nullptr;
+  }
+  h->lds_threads = threads;
+  return PE_OK;
+}
+
 extern "C" void pe_sl_destroy(pe_sl *h) {
   if (!h) return;
   hipSetDevice(h->device);

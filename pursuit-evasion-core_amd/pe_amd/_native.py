@@ -269,6 +269,7 @@ SIGNATURES = {
     "pe_sl_trace": (ctypes.c_int, [P, P, I64, P]),
     "pe_sl_get_stats": (ctypes.c_int, [P, ctypes.POINTER(SLStats)]),
     "pe_sl_set_profiling": (ctypes.c_int, [P, I32]),
+    "pe_sl_set_lds_threads": (ctypes.c_int, [P, I32]),
     "pe_sl_destroy": (None, [P]),
     "pe_sl_warm_start": (ctypes.c_int, [P, P, ctypes.c_float]),
     "pe_sl_pair_inputs": (ctypes.c_int, [P, I32, I64, I64, P]),
@@ -527,6 +528,9 @@ class SL:
 
     def set_profiling(self, on):
         check(lib().pe_sl_set_profiling(self._h, int(bool(on))))
+
+    def set_lds_threads(self, threads):
+        check(lib().pe_sl_set_lds_threads(self._h, int(threads)))
 
     def pair_inputs(self, first, count, index=0):
         out = np.empty((int(count), 11), np.float32)

@@ -198,6 +198,30 @@ def test_small_map_visiting_orders_are_exact(gpu, small_game):
         np.testing.assert_array_equal(out[name][1], out["plain"][1], err_msg=name)
 
 
+def test_lds_block_sizes_are_exact(gpu, small_game):
+    """The LDS E side gives the same values at 256, 512 and 1024 threads per
+    block, in both visiting orders, with graph replay; other sizes are
+    rejected."""
+    tables = small_game[4]
+    out = {}
+    for threads in (256, 512, 1024):
+        for name, flags in [("one_pass", EARLY | ORDERED | GRAPH),
+                            ("u_major", EARLY | ORDERED | U_MAJOR)]:
+            with GameSolver(tables) as s:
+                s.sweep(4, flags)  # the handle's own size first
+                s.set_lds_threads(threads)
+                assert s.stats()["lds_e_side"] == threads
+                s.sweep(60, flags)
+                out[threads, name] = (s.read(), s.trace()[:, 0])
+    base = out[1024, "one_pass"]
+    for key, (v, tr) in out.items():
+        np.testing.assert_array_equal(v, base[0], err_msg=str(key))
+        np.testing.assert_array_equal(tr, base[1], err_msg=str(key))
+    with GameSolver(tables) as s:
+        with pytest.raises(ValueError):
+            s.set_lds_threads(128)
+
+
 def test_given_field_outside_the_radius_is_kept(gpu, small_game):
     """An initial field whose pairs beyond the exit radius hold another
     value than the exit value: those pairs keep it through an odd number of
