@@ -89,8 +89,14 @@ __global__ __launch_bounds__(kPThreads) void sl_p_side(
     const int *__restrict__ ppos, const int *__restrict__ nlist,
     const int *__restrict__ noff) {
   const SlMap m = maps[blockIdx.z];
-  const int ip = blockIdx.x;  // blockIdx.y of gridDim.y strided chunks of the list
-  if (ip >= m.np || !m.active) return;
+  // Consecutive blocks go to the 8 XCDs in turn; block x takes P state
+  // start(x % 8) + x / 8, so each XCD walks a contiguous range of P states
+  // and the V rows of their (nearby) successors stay in its L2.  A
+  // bijection of [0, gridDim.x).
+  const int gx = (int)gridDim.x, q = gx >> 3, r = gx & 7;
+  const int xcd = (int)blockIdx.x & 7;
+  const int ip = xcd * q + min(xcd, r) + ((int)blockIdx.x >> 3);
+  if (ip >= m.np || !m.active) return;  // blockIdx.y: strided chunk of the list
   const T *__restrict__ Vm = V + m.v_off;
   const int row = m.p_row + ip;
   const int k0 = noff[m.ipos_off + ppos[row]];
