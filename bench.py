@@ -314,6 +314,7 @@ def run_c4(args):
             "max_residual_after": float(res.max()),
             "p_side_ms": prof["p_ms"] / max(1, prof["p_launches"]),
             "e_side_ms": prof["e_ms"] / max(1, prof["e_launches"]),
+            "e_side_lds_threads": st0["lds_e_side"],
             # The reference's early exit in index order, without the stored
             # command order.
             "e_side_reference_order_ms": (prof_r["e_ms"] - prof["e_ms"]) /
