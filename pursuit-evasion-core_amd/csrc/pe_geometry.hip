@@ -19,6 +19,7 @@
 // does not depend on the order the circles are visited in, so every
 // decision matches the serial walk.
 #include <cmath>
+#include <initializer_list>
 #include <vector>
 
 #include "pe_common.h"
@@ -275,6 +276,7 @@ struct Scratch {
     return PE_OK;
   }
   int upload(void **p, const void *host, size_t bytes) {
+    if (bytes && !host) return fail(PE_ERR_INVALID, "null input buffer");
     int rc = alloc(p, bytes);
     if (rc) return rc;
     if (bytes) PE_HIP(hipMemcpy(*p, host, bytes, hipMemcpyHostToDevice));
@@ -288,6 +290,14 @@ static int check_args(const double *param, double half, int nc, long long n) {
   for (int k = 0; k < 7; k++)
     if (!std::isfinite(param[k]))
       return fail(PE_ERR_INVALID, "non-finite vehicle parameters");
+  return PE_OK;
+}
+
+// Buffers the caller passes for n > 0 items must not be null.
+static int check_buffers(long long n, std::initializer_list<const void *> ptrs) {
+  if (n > 0)
+    for (const void *p : ptrs)
+      if (!p) return fail(PE_ERR_INVALID, "null buffer");
   return PE_OK;
 }
 
@@ -314,6 +324,7 @@ extern "C" int pe_geom_clearance(const double *states, int64_t n,
                                  double *out) {
   clear_error();
   int rc = check_args(param, half, nc, n);
+  if (!rc) rc = check_buffers(n, {states, out});
   if (rc || (rc = require_device(nullptr))) return rc;
   if (n == 0) return PE_OK;
   Scratch s;
@@ -335,6 +346,7 @@ extern "C" int pe_geom_backups(const double *states, int64_t n,
                                uint8_t *out) {
   clear_error();
   int rc = check_args(param, half, nc, n);
+  if (!rc) rc = check_buffers(n, {states, out});
   if (rc || (rc = require_device(nullptr))) return rc;
   if (n == 0) return PE_OK;
   Scratch s;
@@ -357,6 +369,7 @@ extern "C" int pe_geom_actions(const double *states, int64_t n,
                                double *next) {
   clear_error();
   int rc = check_args(param, half, nc, n);
+  if (!rc) rc = check_buffers(n, {states, controls, ok, next});
   if (rc) return rc;
   if (na < 1 || !(dt >= 0) || !std::isfinite(dt))
     return fail(PE_ERR_INVALID, "invalid controls or duration");
@@ -403,6 +416,7 @@ extern "C" int pe_geom_backups_multi(const double *states, int64_t n, const int3
                                      const double *param, uint8_t *out) {
   clear_error();
   int rc = check_maps(param, map_of, n, n_maps, obstacle_offsets, halves);
+  if (!rc) rc = check_buffers(n, {states, out});
   if (rc || (rc = require_device(nullptr))) return rc;
   if (n == 0) return PE_OK;
   const long long nc = obstacle_offsets[n_maps];
@@ -429,6 +443,7 @@ extern "C" int pe_geom_actions_multi(const double *states, int64_t n, const int3
                                      const double *param, uint8_t *ok, double *next) {
   clear_error();
   int rc = check_maps(param, map_of, n, n_maps, obstacle_offsets, halves);
+  if (!rc) rc = check_buffers(n, {states, ok, next});
   if (rc) return rc;
   if (na < 1 || !controls || !(dt >= 0) || !std::isfinite(dt))
     return fail(PE_ERR_INVALID, "invalid controls or duration");
@@ -460,6 +475,7 @@ extern "C" int pe_geom_connections(const double *from, const double *to,
                                    const double *param, uint8_t *out) {
   clear_error();
   int rc = check_args(param, half, nc, n);
+  if (!rc) rc = check_buffers(n, {from, to, out});
   if (rc || (rc = require_device(nullptr))) return rc;
   if (n == 0) return PE_OK;
   Scratch s;

@@ -522,8 +522,11 @@ static int freeze_converged(pe_sl *h, double tol, bool *frozen) {
                           hipMemcpyDeviceToDevice, h->stream));
   }
   if (!changed) return PE_OK;
+  // Stream-ordered after the copies above and before the next sweep; the
+  // synchronisation keeps the host array stable until the copy has read it.
+  PE_HIP(hipMemcpyAsync(h->d_maps, h->maps.data(), sizeof(SlMap) * h->B,
+                        hipMemcpyHostToDevice, h->stream));
   PE_HIP(hipStreamSynchronize(h->stream));
-  PE_HIP(hipMemcpy(h->d_maps, h->maps.data(), sizeof(SlMap) * h->B, hipMemcpyHostToDevice));
   *frozen = true;
   return PE_OK;
 }
@@ -566,8 +569,9 @@ extern "C" int pe_sl_run(pe_sl *h, double tol, int32_t max_sweeps,
   }
   if (frozen) {  // every map is swept again after this call
     for (SlMap &m : h->maps) m.active = 1;
-    const hipError_t e = hipMemcpy(h->d_maps, h->maps.data(), sizeof(SlMap) * h->B,
-                                   hipMemcpyHostToDevice);
+    hipError_t e = hipMemcpyAsync(h->d_maps, h->maps.data(), sizeof(SlMap) * h->B,
+                                  hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (!rc && e != hipSuccess) rc = fail(PE_ERR_HIP, hipGetErrorString(e));
   }
   if (rc) return rc;

@@ -116,3 +116,41 @@ def test_multi_map_geometry_arguments_rejected_before_any_device_work():
     rc = lib.pe_geom_actions_multi(P(states), 3, P(m), 2, P(controls), 0, 0.05, P(obstacles),
                                    P(o), P(h), P(param), P(ok), P(nxt))
     assert rc == native.PE_ERR_INVALID and b"controls" in lib.pe_last_error()
+
+
+def test_geometry_null_buffers_rejected_before_any_device_work():
+    """Every geometry entry point refuses a null state or output buffer for
+    n > 0 items with PE_ERR_INVALID, before it touches a device."""
+    import numpy as np
+    import pe_amd._native as native
+
+    lib = native.lib()
+    P = native.ptr
+    null = None
+    states = np.zeros((3, 4))
+    param = np.array([-1.0, 2.0, 3.0, 1.0, 0.5, 0.3, 0.05])
+    obstacles = np.zeros((2, 3))
+    controls = np.zeros((9, 2))
+    out_d, out_u8 = np.empty(3), np.empty(3, np.uint8)
+    ok, nxt = np.empty((3, 9), np.uint8), np.empty((3, 9, 4))
+    m = np.array([0, 1, 1], np.int32)
+    o = np.array([0, 1, 2], np.int64)
+    h = np.array([3.0, 3.0])
+    calls = [
+        lambda: lib.pe_geom_clearance(null, 3, P(obstacles), 2, 3.0, P(param), P(out_d)),
+        lambda: lib.pe_geom_clearance(P(states), 3, P(obstacles), 2, 3.0, P(param), null),
+        lambda: lib.pe_geom_backups(P(states), 3, P(obstacles), 2, 3.0, P(param), null),
+        lambda: lib.pe_geom_actions(P(states), 3, P(controls), 9, 0.05, P(obstacles), 2, 3.0,
+                                    P(param), null, P(nxt)),
+        lambda: lib.pe_geom_actions(P(states), 3, null, 9, 0.05, P(obstacles), 2, 3.0,
+                                    P(param), P(ok), P(nxt)),
+        lambda: lib.pe_geom_connections(P(states), null, 3, P(obstacles), 2, 3.0, P(param),
+                                        P(out_u8)),
+        lambda: lib.pe_geom_backups_multi(null, 3, P(m), 2, P(obstacles), P(o), P(h), P(param),
+                                          P(out_u8)),
+        lambda: lib.pe_geom_actions_multi(P(states), 3, P(m), 2, P(controls), 9, 0.05,
+                                          P(obstacles), P(o), P(h), P(param), P(ok), null),
+    ]
+    for k, call in enumerate(calls):
+        assert call() == native.PE_ERR_INVALID, k
+        assert b"null buffer" in lib.pe_last_error(), k
