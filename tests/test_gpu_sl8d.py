@@ -102,7 +102,9 @@ def fake_tables(p, e, radius):
                 exit_value=float(radius), p_body=(0.8, 0.5), e_body=(0.4, 0.25)), inside
 
 
-@pytest.mark.parametrize("seed,np_,ne,na", [(114, 5, 7, 3), (57, 5, 7, 4), (8, 300, 257, 9)])
+# The last case has E ids beyond 16 bits (32-bit tables, global-gather E side).
+@pytest.mark.parametrize("seed,np_,ne,na", [(114, 5, 7, 3), (57, 5, 7, 4), (8, 300, 257, 9),
+                                           (5, 3, 70000, 3)])
 def test_sweeps_match_fp64_oracle_on_random_tables(gpu, oracle_lib, seed, np_, ne, na):
     rng = np.random.default_rng(seed)
     p, e = fake_grid(np_, na, rng), fake_grid(ne, na, rng)
@@ -476,3 +478,40 @@ def test_c2_fp64_sweep_bit_exact_and_converged_fp32_vs_fp64(gpu, oracle_lib, c2_
           f"at |V64| {np.abs(v64.ravel()[flip])[:10].tolist()}")
     assert d.max() <= 1e-3
     assert (np.abs(v64.ravel()[flip]) <= 1e-3).all()
+
+
+def identity_car(count, x):
+    """`count` states at (x, 0, 0, 0), one command whose 16 stencil vertices
+    are the state itself with the whole weight on the first."""
+    states = np.zeros((count, 4))
+    states[:, 0] = x
+    ids = np.repeat(np.arange(count, dtype=np.int32)[:, None, None], 16, axis=2)
+    w = np.zeros((count, 1, 16), np.float32)
+    w[:, :, 0] = 1
+    return SimpleNamespace(states=states, successors=ids, weights=w,
+                           allowed=np.ones((count, 1), bool))
+
+
+@pytest.mark.parametrize("n_p,n_e", [(65536, 1), (1, 65536)])
+def test_more_than_65535_vehicle_states(gpu, oracle_lib, n_p, n_e):
+    """(tests/test_compact_dp.py:121-133) 65,536 states of one car at x = 0
+    against the other car at x = 1: bodies 0.8 and 0.4 long give the
+    initial value 1 - 0.4 - 0.2 = 0.4 on every pair.  With 65,536 E states
+    the 16-bit packed E records cannot index the E side and the 32-bit
+    tables are used.  Identity stencils keep every value after a sweep; the
+    sweep is compared with the FP64 oracle's on every pair, for both E-side
+    kernels."""
+    p, e = identity_car(n_p, 0.0), identity_car(n_e, 1.0)
+    tables, inside = fake_tables(p, e, 3.0)
+    print(f"[gpu] identity tables {n_p}x{n_e}", flush=True)
+    expect0 = oracle_lib.sl_init(p.states, e.states, inside, 3.0,
+                                 Footprint(0.8, 0.5), Footprint(0.4, 0.25))
+    np.testing.assert_allclose(expect0, 0.4, atol=1e-12)
+    expect1, _ = oracle_lib.sl_sweep(expect0, p, e, inside)
+    for flags in (EARLY, EARLY | GLOBAL):
+        with GameSolver(tables) as s:
+            V0 = s.read()
+            assert V0.shape == (n_p, n_e)
+            np.testing.assert_allclose(V0, 0.4, atol=1e-6)
+            s.sweep(1, flags)
+            np.testing.assert_allclose(s.read(), expect1, atol=1e-6)
