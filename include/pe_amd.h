@@ -75,12 +75,18 @@ typedef struct pe_hji5d_config {
 /* pe_hji5d_config.flags: use the generic stage template for FP32 too
    (comparison runs; the default FP32 kernel is instruction-lean). */
 #define PE_HJI5D_GENERIC 2
-/* pe_hji5d_config.flags: the FP32 kernel loads each node's neighbours just
-   before computing it instead of one vP node ahead (comparison runs). */
+/* pe_hji5d_config.flags: the single-plane FP32 kernel loads each node's
+   neighbours just before computing it instead of one vP node ahead
+   (comparison runs; the two-plane kernel always loads just before). */
 #define PE_HJI5D_NO_PREFETCH 4
 /* pe_hji5d_config.flags: FP32 kernel block order (psi, xy tile, vE), so that
    y-adjacent tiles run close together (comparison runs). */
 #define PE_HJI5D_PSI_OUTER 8
+/* pe_hji5d_config.flags: each FP32 kernel thread walks the column of one psi
+   plane (comparison runs).  By default, with an even heading count, a thread
+   walks the columns of two adjacent psi planes, which share their psi
+   neighbour loads; the results are the same bits. */
+#define PE_HJI5D_PSI_SINGLE 16
 
 typedef struct pe_hji5d_stats {
   int64_t states;           /* grid nodes                                  */

@@ -10,6 +10,8 @@
 //   raw differences, folded scale factors, mask-free interior
 //     vE blocks and vP iterations (~130 VALU per node)        176 / 146 us
 //   + next vP node's neighbours requested one node early      169 / 141 us
+//   two adjacent psi planes per thread (shared psi loads, 16
+//     instead of 18 loads per blend node), loads not early    165 / 142 us
 // So the stage is not VALU bound.  Nor is it HBM bound: the (psi, tile, vE)
 // block order fetches 2.2x the bytes from memory in the same time.  What
 // scales is the number of vector memory instructions per node (about 19):
@@ -132,50 +134,191 @@ struct Neighbours {
   float x[4], y[4], e[4], h[4], prev;
 };
 
+// Sliding vP window of one column: w2, w3, w4 = V(iv), V(iv + 1), V(iv + 2);
+// dpc = the forward difference at iv, dpp = the forward difference at
+// iv - 1, c2p / r2p = the second differences at iv - 1 / iv.
+struct VpWindow {
+  float w2, w3, w4, dpc, dpp, c2p, r2p;
+
+  __device__ __forceinline__ void start(const float *__restrict__ s0, unsigned j, unsigned sv) {
+    w2 = ld(s0, j << 2);
+    w3 = ld(s0, (j + sv) << 2);
+    w4 = ld(s0, (j + 2 * sv) << 2);
+    dpc = w3 - w2;
+    dpp = c2p = r2p = 0.f;
+  }
+
+  // vP slopes of node iv in value units (s = em + ep, d = ep - em); moves
+  // the differences on by one node.
+  template <int ORDER, bool VINT>
+  __device__ __forceinline__ void slopes(int iv, int nvp, float &s, float &d) {
+    const float dn = w4 - w3;
+    const float r2 = dn - dpc, c2 = r2p, l2 = c2p;
+    float dm = dpp, dp = dpc;
+    if (ORDER == 1) {
+      if (!VINT) {
+        dm = iv == 0 ? dp : dm;
+        dp = iv == nvp - 1 ? dm : dp;
+      }
+      s = dm + dp;
+      d = dp - dm;
+    } else {
+      const float sl2 = (VINT || iv >= 2) ? l2 : c2;
+      const float sr2 = (VINT || iv + 2 < nvp) ? r2 : c2;
+      float em = fmaf(.5f, fabsf(sl2) < fabsf(c2) ? sl2 : c2, dm);
+      float ep = fmaf(-.5f, fabsf(c2) < fabsf(sr2) ? c2 : sr2, dp);
+      if (!VINT) {
+        const bool first = iv == 0, last = iv == nvp - 1;
+        em = first ? dp : (last ? dm : em);
+        ep = first ? dp : (last ? dm : ep);
+      }
+      s = em + ep;
+      d = ep - em;
+    }
+    dpp = dpc;
+    dpc = dn;
+    c2p = c2;
+    r2p = r2;
+  }
+
+  __device__ __forceinline__ void advance(float w5) {
+    w2 = w3;
+    w3 = w4;
+    w4 = w5;
+  }
+};
+
+// The slopes of one node along x, y, psi, vE (S = em + ep, D = ep - em in
+// value units) and vP (sv, dv).
+struct Slopes {
+  float Sx, Dx, Sy, Dy, Sp, Dp, Se, De, sv, dv;
+};
+
+// Hamiltonian (native_relative.cpp:124-165) of one node in terms of its
+// slopes, the RK step and the store; in the final stage also the step's
+// maxima and the arrival time.  ve_cs, cby and caby are the psi-dependent
+// column constants (those of `c`, or of the paired column).
+//   bx gx + by gy - wp |y gx - x gy - gp| + we |gp|
+//   + min(plow gv, phigh gv) + max(elow ge, ehigh ge)
+//   + 1/2 [(|bx| + wp |y|)(xp - xm) + (|by| + wp |x|)(yp - ym)
+//          + (wp + we)(pp - pm) + ap (vp+ - vp-) + ae (ve+ - ve-)]
+template <bool BLEND, bool VINT, bool EINT>
+__device__ __forceinline__ void update_node(const StageArgs<float> &a, const Column &c,
+                                            float ve_cs, float cby, float caby, int iv,
+                                            int nvp, float apv, float v, float prev,
+                                            const Slopes &k, unsigned jb, int final_stage,
+                                            float &inc_max, float &chg_max, bool &nonfinite) {
+  const float vp = a.vpk[2 * iv], wp = a.vpk[2 * iv + 1];  // uniform: scalar loads
+  const float bxc = (ve_cs - vp) * c.cx;
+  float H = fmaf(bxc, k.Sx, cby * k.Sy);
+  const float inner = fmaf(c.cxy, k.Sx, fmaf(-c.cxx, k.Sy, -c.cp * k.Sp));
+  H = fmaf(-wp, fabsf(inner), H);
+  H = fmaf(c.wecp, fabsf(k.Sp), H);
+  // Control terms: at interior speeds min(-a g, a g) = -a |g| and
+  // max(-a g, a g) = a |g| exactly.
+  if (VINT) {
+    H = fmaf(-apv, fabsf(k.sv), H);
+  } else {
+    const float p_lo = iv == 0 ? 0.f : -apv, p_hi = iv == nvp - 1 ? 0.f : apv;
+    const float pa = p_lo * k.sv, pb = p_hi * k.sv;
+    H += pb < pa ? pb : pa;
+  }
+  if (EINT) {
+    H = fmaf(c.aev, fabsf(k.Se), H);
+  } else {
+    const float ea = c.e_lo * k.Se, eb = c.e_hi * k.Se;
+    H += ea < eb ? eb : ea;
+  }
+  H = fmaf(fmaf(wp, c.cay, fabsf(bxc)), k.Dx, H);
+  H = fmaf(fmaf(wp, c.cax, caby), k.Dy, H);
+  H = fmaf(fmaf(wp, c.cp, c.wecp), k.Dp, H);
+  H = fmaf(apv, k.dv, H);
+  H = fmaf(c.aev, k.De, H);
+  const float previous = BLEND ? prev : v;
+  float value = fmaf(a.dt, H < 0.f ? H : 0.f, v);
+  if (BLEND) value = .5f * (previous + value);
+  nonfinite |= !isfinite(value);
+  st(a.dst, jb, value);
+  if (final_stage) {
+    // Maxima on the bit patterns: inc_max >= 0, so signed integer order
+    // agrees with float order against any inc; |inc| >= 0 likewise.
+    const float inc = value - previous;
+    inc_max = __int_as_float(max(__float_as_int(inc_max), __float_as_int(inc)));
+    chg_max = __int_as_float(max(__float_as_int(chg_max), __float_as_int(fabsf(inc))));
+    if (previous > 0.f && value <= 0.f) {
+      const float old_arrival = ld(a.arrival, jb);
+      if (isnan(old_arrival))
+        st(a.arrival, jb,
+           float(a.t + double(a.dt) * double(previous) / (double(previous) - double(value))));
+    }
+  }
+}
+
+// ENO slopes along x, y and vE of one node from its neighbours.
+template <int ORDER, bool EINT>
+__device__ __forceinline__ void xye_slopes(float v, const float x[4], const float y[4],
+                                           const float e[4], const Column &c, Slopes &k) {
+  if (ORDER == 1) {
+    eno1_raw<true>(v, x[1], x[2], c.X, k.Sx, k.Dx);
+    eno1_raw<true>(v, y[1], y[2], c.Y, k.Sy, k.Dy);
+    eno1_raw<!EINT>(v, e[1], e[2], c.E, k.Se, k.De);
+  } else {
+    eno2_raw<true>(v, x[0], x[1], x[2], x[3], c.X, k.Sx, k.Dx);
+    eno2_raw<true>(v, y[0], y[1], y[2], y[3], c.Y, k.Sy, k.Dy);
+    eno2_raw<!EINT>(v, e[0], e[1], e[2], e[3], c.E, k.Se, k.De);
+  }
+}
+
+// Pointers to the x, y and vE neighbours at -2, -1, +1, +2 of a node whose
+// byte offset from `s0` is added at the load.
+struct NeighbourBases {
+  const float *__restrict__ x[4];
+  const float *__restrict__ y[4];
+  const float *__restrict__ e[4];
+
+  __device__ __forceinline__ NeighbourBases(const float *__restrict__ s0, int nx, int se)
+      : x{s0 - 2, s0 - 1, s0 + 1, s0 + 2},
+        y{s0 - 2 * nx, s0 - nx, s0 + nx, s0 + 2 * nx},
+        e{s0 - 2 * se, s0 - se, s0 + se, s0 + 2 * se} {}
+
+  template <int ORDER>
+  __device__ __forceinline__ void load(unsigned jb, float xv[4], float yv[4], float ev[4]) const {
+    if (ORDER == 2) {
+      xv[0] = ld(x[0], jb);
+      xv[3] = ld(x[3], jb);
+      yv[0] = ld(y[0], jb);
+      yv[3] = ld(y[3], jb);
+      ev[0] = ld(e[0], jb);
+      ev[3] = ld(e[3], jb);
+    }
+    xv[1] = ld(x[1], jb);
+    xv[2] = ld(x[2], jb);
+    yv[1] = ld(y[1], jb);
+    yv[2] = ld(y[2], jb);
+    ev[1] = ld(e[1], jb);
+    ev[2] = ld(e[2], jb);
+  }
+};
+
 template <bool BLEND, int ORDER, bool EINT, bool PREFETCH>
 __device__ __forceinline__ void hji5d_column(const StageArgs<float> &a, const Hji5dGrid &g,
                                              const Column &c, const float *__restrict__ hb[4],
                                              int final_stage, float &inc_max, float &chg_max,
                                              bool &nonfinite) {
   const float *__restrict__ s0 = a.src;
-  const int nx = g.nx, se = (int)g.se;
-  const float *__restrict__ sxm2 = s0 - 2, *__restrict__ sxm1 = s0 - 1,
-                            *__restrict__ sxp1 = s0 + 1, *__restrict__ sxp2 = s0 + 2;
-  const float *__restrict__ sym2 = s0 - 2 * nx, *__restrict__ sym1 = s0 - nx,
-                            *__restrict__ syp1 = s0 + nx, *__restrict__ syp2 = s0 + 2 * nx;
-  const float *__restrict__ sem2 = s0 - 2 * se, *__restrict__ sem1 = s0 - se,
-                            *__restrict__ sep1 = s0 + se, *__restrict__ sep2 = s0 + 2 * se;
-  const float dvp = float(g.dvp);
-  const float ap = float(g.ap);
-  const float cv = .5f / dvp, apv = ap * cv;
-  const float dt = a.dt;
+  const NeighbourBases nbase(s0, g.nx, (int)g.se);
+  const float cv = .5f / float(g.dvp), apv = float(g.ap) * cv;
   const int nvp = g.nvp;
   const unsigned sv = (unsigned)g.sv;
   unsigned j = c.j;
-  // vP window w2..w5 = V(iv .. iv+3); dpc = w3 - w2 (forward difference at
-  // iv), dpp = the forward difference at iv - 1, c2p / r2p = second
-  // differences at iv - 1 / iv.
-  float w2 = ld(s0, j << 2);
-  float w3 = ld(s0, (j + sv) << 2);
-  float w4 = ld(s0, (j + 2 * sv) << 2);
-  float dpc = w3 - w2, dpp = 0.f, c2p = 0.f, r2p = 0.f;
+  VpWindow win;
+  win.start(s0, j, sv);
   auto load = [&](unsigned jb, Neighbours &n) {
+    nbase.load<ORDER>(jb, n.x, n.y, n.e);
     if (ORDER == 2) {
-      n.x[0] = ld(sxm2, jb);
-      n.x[3] = ld(sxp2, jb);
-      n.y[0] = ld(sym2, jb);
-      n.y[3] = ld(syp2, jb);
-      n.e[0] = ld(sem2, jb);
-      n.e[3] = ld(sep2, jb);
       n.h[0] = ld(hb[0], jb);
       n.h[3] = ld(hb[3], jb);
     }
-    n.x[1] = ld(sxm1, jb);
-    n.x[2] = ld(sxp1, jb);
-    n.y[1] = ld(sym1, jb);
-    n.y[2] = ld(syp1, jb);
-    n.e[1] = ld(sem1, jb);
-    n.e[2] = ld(sep1, jb);
     n.h[1] = ld(hb[1], jb);
     n.h[2] = ld(hb[2], jb);
     if (BLEND) n.prev = ld(a.dst, jb);
@@ -196,103 +339,17 @@ __device__ __forceinline__ void hji5d_column(const StageArgs<float> &a, const Hj
     } else {
       load(j << 2, nb);
     }
-    // vP slopes (sliding): dm = dpp, dp = dpc, centre second difference c2
-    float sv_, dv_;
-    {
-      const float dn = w4 - w3;
-      const float r2 = dn - dpc, c2 = r2p, l2 = c2p;
-      float dm = dpp, dp = dpc;
-      if (ORDER == 1) {
-        if (!VINT) {
-          dm = iv == 0 ? dp : dm;
-          dp = iv == nvp - 1 ? dm : dp;
-        }
-        sv_ = dm + dp;
-        dv_ = dp - dm;
-      } else {
-        const float sl2 = (VINT || iv >= 2) ? l2 : c2;
-        const float sr2 = (VINT || iv + 2 < nvp) ? r2 : c2;
-        float em = fmaf(.5f, fabsf(sl2) < fabsf(c2) ? sl2 : c2, dm);
-        float ep = fmaf(-.5f, fabsf(c2) < fabsf(sr2) ? c2 : sr2, dp);
-        if (!VINT) {
-          const bool first = iv == 0, last = iv == nvp - 1;
-          em = first ? dp : (last ? dm : em);
-          ep = first ? dp : (last ? dm : ep);
-        }
-        sv_ = em + ep;
-        dv_ = ep - em;
-      }
-      dpp = dpc;
-      dpc = dn;
-      c2p = c2;
-      r2p = r2;
-    }
-    const unsigned jb = j << 2;
-    const float v = w2;
-    float Sx, Dx, Sy, Dy, Sp, Dp, Se, De;
-    if (ORDER == 1) {
-      eno1_raw<true>(v, nb.x[1], nb.x[2], c.X, Sx, Dx);
-      eno1_raw<true>(v, nb.y[1], nb.y[2], c.Y, Sy, Dy);
-      eno1_raw<!EINT>(v, nb.e[1], nb.e[2], c.E, Se, De);
-      eno1_raw<false>(v, nb.h[1], nb.h[2], c.E, Sp, Dp);
-    } else {
-      eno2_raw<true>(v, nb.x[0], nb.x[1], nb.x[2], nb.x[3], c.X, Sx, Dx);
-      eno2_raw<true>(v, nb.y[0], nb.y[1], nb.y[2], nb.y[3], c.Y, Sy, Dy);
-      eno2_raw<!EINT>(v, nb.e[0], nb.e[1], nb.e[2], nb.e[3], c.E, Se, De);
-      eno2_raw<false>(v, nb.h[0], nb.h[1], nb.h[2], nb.h[3], c.E, Sp, Dp);
-    }
-    // Hamiltonian (native_relative.cpp:124-165) in terms of S and D:
-    //   bx gx + by gy - wp |y gx - x gy - gp| + we |gp|
-    //   + min(plow gv, phigh gv) + max(elow ge, ehigh ge)
-    //   + 1/2 [(|bx| + wp |y|)(xp - xm) + (|by| + wp |x|)(yp - ym)
-    //          + (wp + we)(pp - pm) + ap (vp+ - vp-) + ae (ve+ - ve-)]
-    const float vp = a.vpk[2 * iv], wp = a.vpk[2 * iv + 1];  // uniform: scalar loads
-    const float bxc = (c.ve_cs - vp) * c.cx;
-    float H = fmaf(bxc, Sx, c.cby * Sy);
-    const float inner = fmaf(c.cxy, Sx, fmaf(-c.cxx, Sy, -c.cp * Sp));
-    H = fmaf(-wp, fabsf(inner), H);
-    H = fmaf(c.wecp, fabsf(Sp), H);
-    // Control terms: at interior speeds min(-a g, a g) = -a |g| and
-    // max(-a g, a g) = a |g| exactly.
-    if (VINT) {
-      H = fmaf(-apv, fabsf(sv_), H);
-    } else {
-      const float p_lo = iv == 0 ? 0.f : -apv, p_hi = iv == nvp - 1 ? 0.f : apv;
-      const float pa = p_lo * sv_, pb = p_hi * sv_;
-      H += pb < pa ? pb : pa;
-    }
-    if (EINT) {
-      H = fmaf(c.aev, fabsf(Se), H);
-    } else {
-      const float ea = c.e_lo * Se, eb = c.e_hi * Se;
-      H += ea < eb ? eb : ea;
-    }
-    H = fmaf(fmaf(wp, c.cay, fabsf(bxc)), Dx, H);
-    H = fmaf(fmaf(wp, c.cax, c.caby), Dy, H);
-    H = fmaf(fmaf(wp, c.cp, c.wecp), Dp, H);
-    H = fmaf(apv, dv_, H);
-    H = fmaf(c.aev, De, H);
-    const float previous = BLEND ? nb.prev : v;
-    float value = fmaf(dt, H < 0.f ? H : 0.f, v);
-    if (BLEND) value = .5f * (previous + value);
-    nonfinite |= !isfinite(value);
-    st(a.dst, jb, value);
-    if (final_stage) {
-      // Maxima on the bit patterns: inc_max >= 0, so signed integer order
-      // agrees with float order against any inc; |inc| >= 0 likewise.
-      const float inc = value - previous;
-      inc_max = __int_as_float(max(__float_as_int(inc_max), __float_as_int(inc)));
-      chg_max = __int_as_float(max(__float_as_int(chg_max), __float_as_int(fabsf(inc))));
-      if (previous > 0.f && value <= 0.f) {
-        const float old_arrival = ld(a.arrival, jb);
-        if (isnan(old_arrival))
-          st(a.arrival, jb,
-             float(a.t + double(a.dt) * double(previous) / (double(previous) - double(value))));
-      }
-    }
-    w2 = w3;
-    w3 = w4;
-    w4 = w5;
+    Slopes k;
+    win.slopes<ORDER, VINT>(iv, nvp, k.sv, k.dv);
+    const float v = win.w2;
+    xye_slopes<ORDER, EINT>(v, nb.x, nb.y, nb.e, c, k);
+    if (ORDER == 1)
+      eno1_raw<false>(v, nb.h[1], nb.h[2], c.E, k.Sp, k.Dp);
+    else
+      eno2_raw<false>(v, nb.h[0], nb.h[1], nb.h[2], nb.h[3], c.E, k.Sp, k.Dp);
+    update_node<BLEND, VINT, EINT>(a, c, c.ve_cs, c.cby, c.caby, iv, nvp, apv, v, nb.prev, k,
+                                   j << 2, final_stage, inc_max, chg_max, nonfinite);
+    win.advance(w5);
     j += sv;
   };
 
@@ -303,7 +360,96 @@ __device__ __forceinline__ void hji5d_column(const StageArgs<float> &a, const Hj
   for (; iv < nvp; iv++) node(iv, std::false_type{});
 }
 
-template <bool BLEND, int ORDER, bool PREFETCH>
+// The psi-dependent constants of the second column of a pair.
+struct PsiConstants {
+  float ve_cs, cby, caby;
+};
+
+// Neighbour values of a node pair at psi planes ih and ih + 1 (ih even):
+// x, y, vE neighbours of each node, the psi planes ih - 2, ih - 1, ih + 2,
+// ih + 3 shared by both, and in the blend stage the previous values.
+struct PairNeighbours {
+  float xa[4], ya[4], ea[4], xb[4], yb[4], eb[4], h[4], preva, prevb;
+};
+
+// Two columns at the adjacent psi planes ih (even) and ih + 1, walked
+// together along vP.  Their psi neighbours overlap: the pair loads planes
+// ih - 2, ih - 1, ih + 2 and ih + 3, and each node takes the other's value
+// as its psi neighbour at +1 or -1, so a node pair issues 4 psi loads
+// instead of 8 (16 loads per node in the blend stage instead of 18).
+// Every node is computed exactly as by hji5d_column.  The neighbours are
+// loaded just before each node pair: requesting the next pair's early, as
+// hji5d_column does, needs 155 VGPRs (3 waves per SIMD) and was slower.
+template <bool BLEND, int ORDER, bool EINT>
+__device__ __forceinline__ void hji5d_column_pair(const StageArgs<float> &a, const Hji5dGrid &g,
+                                                  const Column &c, const PsiConstants &cb,
+                                                  const float *__restrict__ hb[4],
+                                                  int final_stage, float &inc_max,
+                                                  float &chg_max, bool &nonfinite) {
+  const float *__restrict__ s0 = a.src;
+  const NeighbourBases nbase(s0, g.nx, (int)g.se);
+  const float cv = .5f / float(g.dvp), apv = float(g.ap) * cv;
+  const int nvp = g.nvp;
+  const unsigned sv = (unsigned)g.sv, pb = (unsigned)g.nx2 << 2;  // column b: + pb bytes
+  unsigned j = c.j;
+  VpWindow wa, wb;
+  wa.start(s0, j, sv);
+  wb.start(s0, j + (unsigned)g.nx2, sv);
+  auto load = [&](unsigned jb, PairNeighbours &n) {
+    nbase.load<ORDER>(jb, n.xa, n.ya, n.ea);
+    nbase.load<ORDER>(jb + pb, n.xb, n.yb, n.eb);
+    if (ORDER == 2) {
+      n.h[0] = ld(hb[0], jb);
+      n.h[3] = ld(hb[3], jb);
+    }
+    n.h[1] = ld(hb[1], jb);
+    n.h[2] = ld(hb[2], jb);
+    if (BLEND) {
+      n.preva = ld(a.dst, jb);
+      n.prevb = ld(a.dst, jb + pb);
+    }
+  };
+  auto node = [&](int iv, auto interior) {
+    constexpr bool VINT = decltype(interior)::value;
+    const bool more = iv + 3 < nvp;
+    const float w5a = more ? ld(s0, (j + 3 * sv) << 2) : 0.f;
+    const float w5b = more ? ld(s0, ((j + 3 * sv) << 2) + pb) : 0.f;
+    PairNeighbours nb;
+    load(j << 2, nb);
+    Slopes ka, kb;
+    wa.slopes<ORDER, VINT>(iv, nvp, ka.sv, ka.dv);
+    wb.slopes<ORDER, VINT>(iv, nvp, kb.sv, kb.dv);
+    const float va = wa.w2, vb = wb.w2;
+    xye_slopes<ORDER, EINT>(va, nb.xa, nb.ya, nb.ea, c, ka);
+    xye_slopes<ORDER, EINT>(vb, nb.xb, nb.yb, nb.eb, c, kb);
+    if (ORDER == 1) {
+      eno1_raw<false>(va, nb.h[1], vb, c.E, ka.Sp, ka.Dp);
+      eno1_raw<false>(vb, va, nb.h[2], c.E, kb.Sp, kb.Dp);
+    } else {
+      eno2_raw<false>(va, nb.h[0], nb.h[1], vb, nb.h[2], c.E, ka.Sp, ka.Dp);
+      eno2_raw<false>(vb, nb.h[1], va, nb.h[2], nb.h[3], c.E, kb.Sp, kb.Dp);
+    }
+    const unsigned jb = j << 2;
+    update_node<BLEND, VINT, EINT>(a, c, c.ve_cs, c.cby, c.caby, iv, nvp, apv, va, nb.preva, ka,
+                                   jb, final_stage, inc_max, chg_max, nonfinite);
+    update_node<BLEND, VINT, EINT>(a, c, cb.ve_cs, cb.cby, cb.caby, iv, nvp, apv, vb, nb.prevb,
+                                   kb, jb + pb, final_stage, inc_max, chg_max, nonfinite);
+    wa.advance(w5a);
+    wb.advance(w5b);
+    j += sv;
+  };
+
+  int iv = 0;
+  for (; iv < 2 && iv < nvp; iv++) node(iv, std::false_type{});
+#pragma unroll 2
+  for (; iv + 2 < nvp; iv++) node(iv, std::true_type{});
+  for (; iv < nvp; iv++) node(iv, std::false_type{});
+}
+
+// PAIR: each thread walks the columns of two adjacent psi planes (ih even
+// and ih + 1; nh even), the block index counts plane pairs (PREFETCH is
+// then unused).
+template <bool BLEND, int ORDER, bool PREFETCH, bool PAIR>
 __global__ __launch_bounds__(256) void hji5d_stage_fast(StageArgs<float> a, Hji5dGrid g,
                                                         int final_stage) {
   const int hw = blockIdx.x;
@@ -311,12 +457,15 @@ __global__ __launch_bounds__(256) void hji5d_stage_fast(StageArgs<float> a, Hji5
   float inc_max = 0.f, chg_max = 0.f;
   bool nonfinite = false;
   if (block < g.blocks) {
-    // Block order, fastest first: vE, then psi and the xy tile (psi before
-    // the tile by default; the tile before psi with psi_outer).
+    // Block order, fastest first: vE, then psi (or psi pair) and the xy
+    // tile (psi before the tile by default; the tile before psi with
+    // psi_outer).
+    const int nhb = PAIR ? g.nh / 2 : g.nh;
     const int ie = block % g.nve;
     const int rest = block / g.nve;
-    const int ih = g.psi_outer ? rest / g.xytiles : rest % g.nh;
-    const int tile = g.psi_outer ? rest % g.xytiles : rest / g.nh;
+    const int ihb = g.psi_outer ? rest / g.xytiles : rest % nhb;
+    const int tile = g.psi_outer ? rest % g.xytiles : rest / nhb;
+    const int ih = PAIR ? 2 * ihb : ihb;
     const int r = tile * 256 + (int)threadIdx.x;  // (y, x) index
     if (r < g.nx2) {
       const int iy = r / g.nx;
@@ -329,7 +478,10 @@ __global__ __launch_bounds__(256) void hji5d_stage_fast(StageArgs<float> a, Hji5
         // Outside the local exit radius every vP node takes the exit value
         // (gpu_relative.py:126-131).
         const float ev = float(g.exit_value);
-        for (int iv = 0; iv < g.nvp; iv++) a.dst[j0 + (unsigned)iv * (unsigned)g.sv] = ev;
+        for (int iv = 0; iv < g.nvp; iv++) {
+          a.dst[j0 + (unsigned)iv * (unsigned)g.sv] = ev;
+          if (PAIR) a.dst[j0 + (unsigned)g.nx2 + (unsigned)iv * (unsigned)g.sv] = ev;
+        }
       } else {
         const float ve = float(g.vemin) + float(ie) * dve;
         const float cs = a.cs[ih], sn = a.sn[ih];
@@ -354,20 +506,34 @@ __global__ __launch_bounds__(256) void hji5d_stage_fast(StageArgs<float> a, Hji5
         c.X = axis_masks(ix, g.nx);
         c.Y = axis_masks(iy, g.nx);
         c.E = axis_masks(ie, g.nve);
-        // psi neighbours (periodic; ih is uniform per block)
+        // psi neighbours (periodic; ih is uniform per block): planes ih - 2,
+        // ih - 1, ih + 1, ih + 2, or for a pair ih - 2, ih - 1, ih + 2, ih + 3
         const int nh = g.nh, nx2 = g.nx2;
         const int hm1 = ih == 0 ? nh - 1 : ih - 1;
         const int hm2 = ih >= 2 ? ih - 2 : ih - 2 + nh;
         const int hp1 = ih == nh - 1 ? 0 : ih + 1;
         const int hp2 = ih + 2 < nh ? ih + 2 : ih + 2 - nh;
+        const int hp3 = ih + 3 < nh ? ih + 3 : ih + 3 - nh;
         const float *hb[4] = {a.src + (hm2 - ih) * nx2, a.src + (hm1 - ih) * nx2,
-                              a.src + (hp1 - ih) * nx2, a.src + (hp2 - ih) * nx2};
-        if (ie >= 2 && ie + 2 < g.nve)
+                              a.src + ((PAIR ? hp2 : hp1) - ih) * nx2,
+                              a.src + ((PAIR ? hp3 : hp2) - ih) * nx2};
+        const bool eint = ie >= 2 && ie + 2 < g.nve;
+        if (PAIR) {
+          const float by1 = ve * a.sn[ih + 1];
+          const PsiConstants cb{ve * a.cs[ih + 1], c.cx * by1, c.cx * fabsf(by1)};
+          if (eint)
+            hji5d_column_pair<BLEND, ORDER, true>(a, g, c, cb, hb, final_stage, inc_max,
+                                                  chg_max, nonfinite);
+          else
+            hji5d_column_pair<BLEND, ORDER, false>(a, g, c, cb, hb, final_stage, inc_max,
+                                                   chg_max, nonfinite);
+        } else if (eint) {
           hji5d_column<BLEND, ORDER, true, PREFETCH>(a, g, c, hb, final_stage, inc_max,
                                                      chg_max, nonfinite);
-        else
+        } else {
           hji5d_column<BLEND, ORDER, false, PREFETCH>(a, g, c, hb, final_stage, inc_max,
                                                       chg_max, nonfinite);
+        }
       }
     }
   }

@@ -49,7 +49,8 @@ struct Hji5dGrid {
   int fast;         // FP32 launches use hji5d_stage_fast (hji5d_fast.h)
   int prefetch;     // fast kernel requests the next vP node's neighbours early
   int psi_outer;    // fast kernel block order (psi, xy tile, vE) instead of (xy tile, psi, vE)
-  int blocks;       // tiles * nve * nvpc (layout 0) or xytiles*nh*nve*nvpc
+  int psi_pair;     // fast kernel threads walk two adjacent psi planes (nh even)
+  int blocks;       // tiles * nve * nvpc (layout 0) or xytiles*nh*nve*nvpc (nh/2 with psi_pair)
   int per_xcd;      // ceil(blocks / 8)
 };
 

@@ -98,6 +98,7 @@ PE_HJI5D_LAYOUT_PLANE = 1
 PE_HJI5D_GENERIC = 2
 PE_HJI5D_NO_PREFETCH = 4
 PE_HJI5D_PSI_OUTER = 8
+PE_HJI5D_PSI_SINGLE = 16
 
 
 class HJI5DStats(ctypes.Structure):

@@ -258,18 +258,26 @@ def test_profiling_counts_launches(gpu):
 @pytest.mark.parametrize("grid,steps,order", [(RelativeGrid(17, 12, 7, 7, 4.0), 18, 2),
                                               (RelativeGrid(33, 24, 9, 9, 4.0), 12, 2),
                                               (RelativeGrid(65, 36, 15, 15, 8.0), 3, 2),
-                                              (RelativeGrid(33, 24, 9, 9, 4.0), 12, 1)])
+                                              (RelativeGrid(33, 24, 9, 9, 4.0), 12, 1),
+                                              (RelativeGrid(17, 7, 7, 7, 4.0), 18, 2),
+                                              (RelativeGrid(17, 4, 7, 7, 4.0), 18, 2)])
 def test_fp32_kernel_variants_are_bit_identical(gpu, grid, steps, order):
     """The FP32 kernel's launch variants do the same arithmetic on the same
-    neighbour values: the next vP node's neighbours requested early or not,
-    and either block order.  Fields and arrival times must agree bit for bit (the generic
+    neighbour values: two psi planes per thread (even heading counts; with
+    four, the planes at -2 and +2 coincide, as do -1 and +3) or one,
+    the next vP node's neighbours requested early or not, and either block
+    order.  Fields and arrival times must agree bit for bit (the generic
     template, a different operation order, is compared by tolerance in the
     oracle tests)."""
+    single = _native.PE_HJI5D_PSI_SINGLE
     variants = {
         "default": 0,
-        "no_prefetch": _native.PE_HJI5D_NO_PREFETCH,
         "psi_outer": _native.PE_HJI5D_PSI_OUTER,
-        "no_prefetch_psi_outer": _native.PE_HJI5D_NO_PREFETCH | _native.PE_HJI5D_PSI_OUTER,
+        "single": single,
+        "single_no_prefetch": single | _native.PE_HJI5D_NO_PREFETCH,
+        "single_psi_outer": single | _native.PE_HJI5D_PSI_OUTER,
+        "single_no_prefetch_psi_outer":
+            single | _native.PE_HJI5D_NO_PREFETCH | _native.PE_HJI5D_PSI_OUTER,
     }
     out = {}
     for name, flags in variants.items():
