@@ -409,7 +409,7 @@ def main() -> int:
     traffic = load_traffic()
     roofline = {
         "bound": "hbm",
-        "kernel": "hji5d_stage_fast<true,2> (RK2 blend stage)",
+        "kernel": "hji5d_stage_fast<true,2,false,true> (RK2 blend stage, two psi planes per thread)",
         "achieved": round(achieved[1], 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
