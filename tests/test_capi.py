@@ -154,3 +154,27 @@ def test_geometry_null_buffers_rejected_before_any_device_work():
     for k, call in enumerate(calls):
         assert call() == native.PE_ERR_INVALID, k
         assert b"null buffer" in lib.pe_last_error(), k
+
+
+def test_hji5d_flag_bits_checked_before_any_device_work():
+    """Unknown PE_HJI5D_* bits are rejected as invalid options; every known
+    bit, PE_HJI5D_PSI_SINGLE included, passes the option check."""
+    import pe_amd._native as native
+    from pe_amd.reversible.free_dp import RelativeGrid, native_config
+    from pe_amd.reversible.model import ReversibleConfig, VehicleLimits
+
+    lib = native.lib()
+    config = ReversibleConfig(name="a3", pursuer=VehicleLimits(-1.0, 2.0, 3.0, 1.0),
+                              evader=VehicleLimits(-0.5, 1.0, 3.0, 2.0), capture_radius=0.3)
+    known = (native.PE_HJI5D_LAYOUT_PLANE | native.PE_HJI5D_GENERIC |
+             native.PE_HJI5D_NO_PREFETCH | native.PE_HJI5D_PSI_OUTER | native.PE_HJI5D_PSI_SINGLE)
+    for flags, rejected in [(32, True), (known | 32, True), (native.PE_HJI5D_PSI_SINGLE, False),
+                            (known, False)]:
+        cfg = native_config(config, RelativeGrid(17, 12, 7, 7, 4.0))
+        cfg.flags = flags
+        h = ctypes.c_void_p()
+        rc = lib.pe_hji5d_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc == 0:  # a device is present: the handle was made
+            lib.pe_hji5d_destroy(h)
+        invalid = rc == native.PE_ERR_INVALID and b"Invalid solver options" in lib.pe_last_error()
+        assert invalid == rejected, (flags, rc, lib.pe_last_error())
