@@ -322,6 +322,17 @@ int pe_rtable_create(const float *fields, int32_t frames, const int32_t *shape,
    range (relative_table.py:118-131). */
 int pe_rtable_field(pe_rtable *h, const double *states, const double *budgets,
                     int64_t n, double *out);
+/* pe_rtable_field for callers that cannot allocate device memory (code under
+   HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+   No counterpart in the reference.  The batch is staged through a fixed
+   device array of the library (PE_RTABLE_NOALLOC_MAX queries, 3.5 MiB, held
+   by each device it runs on until the process ends) instead of the handle's
+   growable buffer; the call makes no hipMalloc / hipFree.  n above the limit
+   is rejected with PE_ERR_INVALID.  All calls on one device share that array:
+   calls that overlap on one device (any handles) race. */
+#define PE_RTABLE_NOALLOC_MAX 65536
+int pe_rtable_field_noalloc(pe_rtable *h, const double *states,
+                            const double *budgets, int64_t n, double *out);
 /* First crossing of zero of the interpolated history (relative_table.py:
    133-161); 0 for captured physical states; speed_limits = P min, P max,
    E min, E max. */

@@ -14,6 +14,7 @@
 // Vertex sums are accumulated in vertex order in float64.
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "pe_common.h"
@@ -269,6 +270,63 @@ extern "C" int pe_rtable_value(pe_rtable *h, const double *states, int64_t n,
                      h->fields, h->times, h->g, s, (long long)n, capture_radius,
                      speed_limits[0], speed_limits[1], speed_limits[2], speed_limits[3], o);
   PE_LAUNCHED(rt_value);
+  PE_HIP(hipMemcpyAsync(out, o, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+  PE_HIP(hipStreamSynchronize(h->stream));
+  return PE_OK;
+}
+
+// Staging of pe_rtable_field_noalloc: states [n, 5] | budgets [n] | out [n]
+// doubles for at most PE_RTABLE_NOALLOC_MAX queries (3.5 MiB), one copy per
+// device for the life of the process.  Host code never passes `scratch`
+// itself to a kernel or copy (in host code it is a stand-in, not a device
+// address): it uses the device address hipGetSymbolAddress returns, looked
+// up once per device.
+static __device__ __align__(256) double scratch[7 * PE_RTABLE_NOALLOC_MAX];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// Device address of `scratch` on the current device `dev`.
+static int rt_scratch(int dev, double **out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(PE_ERR_INVALID, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return fail(PE_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                  hipGetErrorString(e));
+    }
+    g_scratch_ptr[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return PE_OK;
+}
+
+extern "C" int pe_rtable_field_noalloc(pe_rtable *h, const double *states,
+                                       const double *budgets, int64_t n, double *out) {
+  clear_error();
+  if (!h || n < 0 || (n > 0 && (!states || !budgets || !out)))
+    return fail(PE_ERR_INVALID, "invalid argument");
+  if (n > PE_RTABLE_NOALLOC_MAX)
+    return fail(PE_ERR_INVALID, "pe_rtable_field_noalloc: " + std::to_string(n) +
+                                    " queries exceed the limit of " +
+                                    std::to_string(PE_RTABLE_NOALLOC_MAX));
+  if (n == 0) return PE_OK;
+  PE_HIP(hipSetDevice(h->device));
+  double *buf = nullptr;
+  int rc = rt_scratch(h->device, &buf);
+  if (rc) return rc;
+  double *s = buf, *b = s + 5 * n, *o = b + n;
+  PE_HIP(hipMemcpyAsync(s, states, sizeof(double) * 5 * n, hipMemcpyHostToDevice, h->stream));
+  PE_HIP(hipMemcpyAsync(b, budgets, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(rt_field, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream,
+                     h->fields, h->times, h->g, s, b, (long long)n, o);
+  PE_LAUNCHED(rt_field);
   PE_HIP(hipMemcpyAsync(out, o, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
   PE_HIP(hipStreamSynchronize(h->stream));
   return PE_OK;

@@ -228,6 +228,7 @@ class World8DStats(ctypes.Structure):
 PE_WORLD8D_GENERIC = 1
 PE_WORLD8D_SLAB_256 = 2
 PE_WORLD8D_SLAB_1024 = 4
+PE_RTABLE_NOALLOC_MAX = 65536
 
 
 P = ctypes.c_void_p
@@ -303,6 +304,7 @@ SIGNATURES = {
     "pe_rtable_create": (ctypes.c_int, [P, I32, P, P, P, P, P, P]),
     "pe_rtable_field": (ctypes.c_int, [P, P, P, I64, P]),
     "pe_rtable_value": (ctypes.c_int, [P, P, I64, F64, P, P]),
+    "pe_rtable_field_noalloc": (ctypes.c_int, [P, P, P, I64, P]),
     "pe_rtable_destroy": (None, [P]),
 }
 
@@ -702,6 +704,15 @@ class RelTable:
         b = np.ascontiguousarray(budgets, np.float64).reshape(-1)
         out = np.empty(len(s), np.float64)
         check(lib().pe_rtable_field(self._h, ptr(s), ptr(b), len(s), ptr(out)))
+        return out
+
+    def field_noalloc(self, states, budgets):
+        """field() through pe_rtable_field_noalloc (no device allocation;
+        at most PE_RTABLE_NOALLOC_MAX queries per call)."""
+        s = np.ascontiguousarray(states, np.float64).reshape(-1, 5)
+        b = np.ascontiguousarray(budgets, np.float64).reshape(-1)
+        out = np.empty(len(s), np.float64)
+        check(lib().pe_rtable_field_noalloc(self._h, ptr(s), ptr(b), len(s), ptr(out)))
         return out
 
     def value(self, states, capture_radius, speed_limits):
