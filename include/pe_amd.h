@@ -170,6 +170,45 @@ int pe_geom_connections(const double *from, const double *to, int64_t n,
                         const double *param, uint8_t *out);
 
 /* ======================================================================
+ * Successor tables of one car grid (the stencil step of make_grid).
+ *
+ * Replaces the host numpy stencil VehicleGrid.stencil
+ * (local_game/grid_game.py:81-123) and its cKDTree fallback
+ * _nearest_supported (:38-79) that make_grid (:158-196) applies to every
+ * held-command successor.  Axes: x, y, heading, speed, each ascending;
+ * heading is periodic with cells of 2 pi / n starting at -pi.  The dense
+ * grid is row-major with axis 0 slowest; dense_to_compact [prod n] holds
+ * each legal state's compact index or -1.
+ * ====================================================================== */
+typedef struct pe_car_axes {
+  int32_t n[4];           /* axis lengths (heading >= 1, others >= 2)      */
+  const double *axis[4];  /* axis values                                   */
+} pe_car_axes;
+
+/* For m successor states next [m,4]: the 16-vertex multilinear stencil
+   (vertex c has the upper node along axis k when bit 3 - k of c is set)
+   with illegal vertices dropped and the rest renormalised: ids [m,16]
+   compact ids (0 where dropped), weights [m,16] float32, mass [m] the
+   retained weight before renormalisation, valid [m] = finite, within the
+   non-periodic axes (1e-9 slack) and supported.  A successor inside the
+   axes whose vertices are all illegal (mass <= 1e-9) takes the fallback:
+   among the min(64, n) legal states nearest in the scaled features
+   (x/dx, y/dx, cos h/dh, sin h/dh, v/dv; dx = x spacing, dh = 2 sin(pi/nh),
+   dv = smallest speed spacing; ties by compact index), the first 16 whose
+   straight configuration segment from the successor is clear of the map
+   (obstacles, half, param as pe_geom_connections), weighted by
+   1 / max(d, 1e-8)^2 normalised; valid = 0 when none is.  param == NULL
+   skips the fallback (such successors keep zero weights and valid = 0, as
+   the reference's stencil of a grid without a geometry engine).
+   *fallbacks (may be NULL) receives the number of successors whose
+   vertices were all illegal. */
+int pe_car_stencil(const pe_car_axes *axes, const int32_t *dense_to_compact,
+                   const double *next, int64_t m, const double *obstacles,
+                   int32_t nc, double half, const double *param, int32_t *ids,
+                   float *weights, uint8_t *valid, double *mass,
+                   int64_t *fallbacks);
+
+/* ======================================================================
  * Compact 8D semi-Lagrangian min-max backup of the contact game.
  *
  * Value V over the dense product of the two cars' legal states, row-major

@@ -140,6 +140,10 @@ def _declare_8d(L):
     L.pe_oracle_step.restype = None
     L.pe_oracle_sl_init.argtypes = [I64, I64, P, P, P, D, D, D, D, D, P, I32]
     L.pe_oracle_sl_init.restype = None
+    L.pe_oracle_car_stencil.restype = ctypes.c_int64
+    L.pe_oracle_car_stencil.argtypes = [P, P, P, P, P, P, P, I64, P, I32, D, P, P, P, P, P, P]
+    L.pe_oracle_nearest.restype = None
+    L.pe_oracle_nearest.argtypes = [P, I32, P, I64, I32, D, D, D, P, P]
     L.pe_oracle_sl_sweep.restype = ctypes.c_double
     L.pe_oracle_sl_sweep.argtypes = [
         ctypes.c_int64,  # np
@@ -358,3 +362,38 @@ def sl_sweep(V, p, e, inside, threads=None):
         V.shape[0], V.shape[1], p.successors.shape[1], e.successors.shape[1],
         *[_ptr(a) for a in arrs], _ptr(ins), _ptr(V), _ptr(out), threads or os.cpu_count())
     return out, change
+
+
+# ------------------------------------------------------------ car tables
+
+def car_stencil(axes, dense_to_compact, nxt, obstacles, half, params):
+    """Successor stencils of one car grid (stencil.c): ids [m,16] int32,
+    weights [m,16] float32, valid [m] bool, mass [m], raw vertex weights
+    [m,16] before renormalisation, and the number of fallback rows."""
+    n = np.array([len(a) for a in axes], np.int32)
+    ax = [np.ascontiguousarray(a, np.float64) for a in axes]
+    d2c = np.ascontiguousarray(dense_to_compact, np.int32).reshape(-1)
+    q = _states(nxt)
+    o = _obs(obstacles)
+    m = len(q)
+    ids = np.empty((m, 16), np.int32)
+    w = np.empty((m, 16), np.float32)
+    valid = np.empty(m, np.uint8)
+    mass = np.empty(m)
+    raw = np.empty((m, 16))
+    fb = lib().pe_oracle_car_stencil(_ptr(n), *[_ptr(a) for a in ax], _ptr(d2c), _ptr(q), m,
+                                     _ptr(o), len(o), half, _ptr(params), _ptr(ids), _ptr(w),
+                                     _ptr(valid), _ptr(mass), _ptr(raw))
+    return ids, w, valid.astype(bool), mass, raw, int(fb)
+
+
+def nearest(features, queries, k, dx, dh, dv):
+    """k nearest feature rows of each query state (ties by index):
+    ids [m,k], distances [m,k]."""
+    f = np.ascontiguousarray(features, np.float64).reshape(-1, 5)
+    q = _states(queries)
+    ids = np.empty((len(q), k), np.int32)
+    dist = np.empty((len(q), k))
+    lib().pe_oracle_nearest(_ptr(f), len(f), _ptr(q), len(q), k, dx, dh, dv, _ptr(ids),
+                            _ptr(dist))
+    return ids, dist

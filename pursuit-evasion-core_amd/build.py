@@ -41,6 +41,7 @@ PER_FILE = {
     "pe_sl8d_f64.hip": ["-ffp-contract=off"],
     "pe_world8d.hip": ["-ffp-contract=off"],
     "pe_rtable.hip": ["-ffp-contract=off"],
+    "pe_tables.hip": ["-ffp-contract=off"],
 }
 
 

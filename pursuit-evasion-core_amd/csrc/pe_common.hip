@@ -47,6 +47,7 @@ extern "C" int pe_abi_struct_size(const char *name) {
       {"pe_mlp_config", (int)sizeof(pe_mlp_config)},
       {"pe_world8d_config", (int)sizeof(pe_world8d_config)},
       {"pe_world8d_stats", (int)sizeof(pe_world8d_stats)},
+      {"pe_car_axes", (int)sizeof(pe_car_axes)},
   };
   for (const auto &t : table)
     if (std::strcmp(t.name, name) == 0) return t.size;

@@ -129,6 +129,10 @@ class SLCar(ctypes.Structure):
     ]
 
 
+class CarAxes(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32 * 4), ("axis", ctypes.c_void_p * 4)]
+
+
 class SLGame(ctypes.Structure):
     _fields_ = [
         ("p", SLCar),
@@ -259,6 +263,8 @@ SIGNATURES = {
     "pe_geom_backups_multi": (ctypes.c_int, [P, I64, P, I32, P, P, P, P, P]),
     "pe_geom_actions_multi": (ctypes.c_int, [P, I64, P, I32, P, I32, F64, P, P, P, P, P, P]),
     "pe_geom_connections": (ctypes.c_int, [P, P, I64, P, I32, F64, P, P]),
+    "pe_car_stencil": (ctypes.c_int, [ctypes.POINTER(CarAxes), P, P, I64, P, I32, F64, P, P, P,
+                                      P, P, P]),
     "pe_sl_create": (ctypes.c_int, [ctypes.POINTER(SLGame), I32, P]),
     "pe_sl_create_ex": (ctypes.c_int, [ctypes.POINTER(SLGame), I32, I32, P]),
     "pe_sl_initialize_f64": (ctypes.c_int, [P, P]),
@@ -310,6 +316,7 @@ SIGNATURES = {
 
 
 STRUCTS = {
+    "pe_car_axes": CarAxes,
     "pe_hji5d_config": HJI5DConfig,
     "pe_hji5d_stats": HJI5DStats,
     "pe_sl_car": SLCar,
@@ -340,6 +347,38 @@ def copy_bandwidth(nbytes, repeats=10):
     gbps = ctypes.c_double()
     check(lib().pe_copy_bandwidth(int(nbytes), int(repeats), ctypes.byref(gbps)))
     return gbps.value
+
+
+def car_stencil(axes, dense_to_compact, states, geometry=None):
+    """pe_car_stencil: ids [m,16] int32, weights [m,16] float32, valid [m]
+    bool, mass [m] float64 and the count of rows whose vertices were all
+    illegal, for successor states [m,4].  geometry = (obstacles [nc,3],
+    half-width, params[7]) enables the nearest-supported fallback."""
+    ax = [np.ascontiguousarray(a, np.float64) for a in axes]
+    spec = CarAxes()
+    for k, a in enumerate(ax):
+        spec.n[k] = len(a)
+        spec.axis[k] = a.ctypes.data
+    d2c = np.ascontiguousarray(dense_to_compact, np.int32).reshape(-1)
+    if d2c.size != int(np.prod([len(a) for a in ax])):
+        raise ValueError("dense_to_compact does not match the axes")
+    q = np.ascontiguousarray(states, np.float64).reshape(-1, 4)
+    m = len(q)
+    ids = np.empty((m, 16), np.int32)
+    w = np.empty((m, 16), np.float32)
+    valid = np.empty(m, np.uint8)
+    mass = np.empty(m, np.float64)
+    fallbacks = ctypes.c_int64()
+    if geometry is None:
+        obs, half, par = None, 1.0, None
+    else:
+        obs = np.ascontiguousarray(geometry[0], np.float64).reshape(-1, 3)
+        half = float(geometry[1])
+        par = np.ascontiguousarray(geometry[2], np.float64)
+    check(lib().pe_car_stencil(ctypes.byref(spec), ptr(d2c), ptr(q), m, ptr(obs),
+                               0 if obs is None else len(obs), half, ptr(par), ptr(ids), ptr(w),
+                               ptr(valid), ptr(mass), ctypes.byref(fallbacks)))
+    return ids, w, valid.astype(bool), mass, int(fallbacks.value)
 
 
 class HJI5D:

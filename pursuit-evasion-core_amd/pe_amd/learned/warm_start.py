@@ -66,7 +66,7 @@ def game(seed, **grid):
     (the network's map features)."""
     config, arena, p, e, tables = benchmark_game(seed, **grid)
     for side, g in (("p", p), ("e", e)):
-        tables[side]["clearance"] = g._engine.clearance(g.states)
+        tables[side]["clearance"] = g.geometry.clearance(g.states)
     return p, e, tables
 
 

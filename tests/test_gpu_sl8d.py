@@ -369,7 +369,7 @@ def test_multi_map_table_build_equals_single_maps(gpu):
     # The multi-map kernels against the single-map entry points.
     from pe_amd.local_game.native import actions_many, backup_safe_many
 
-    engines = [g[2]._engine for g in many]
+    engines = [g[2].geometry for g in many]
     dense = np.stack(np.meshgrid(*many[0][2].axes, indexing="ij"), -1).reshape(-1, 4)
     for eng, legal in zip(engines, backup_safe_many(engines, [dense] * len(engines))):
         np.testing.assert_array_equal(legal, eng.backup_safe(dense))
