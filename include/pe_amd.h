@@ -208,6 +208,23 @@ int pe_car_stencil(const pe_car_axes *axes, const int32_t *dense_to_compact,
                    float *weights, uint8_t *valid, double *mass,
                    int64_t *fallbacks);
 
+/* Multilinear stencils of m states [m, d] on a uniform grid of d <= 8 axes
+   (row-major, axis 0 slowest): node index [m, 2^d], weights [m, 2^d]
+   (float64), valid [m].  Axis k has n[k] nodes lo[k] + j step[k] up to
+   hi[k].  Axes in the `periodic` bit mask (bit k) wrap: f = ((q - lo) mod
+   period[k]) / step (numpy's remainder), lower node floor(f) mod n.  Other
+   axes: valid within [lo - slack, hi + slack], f = (clip(q, lo, hi) - lo) /
+   step, lower node min(floor(f), n - 2), fraction clipped to [0, 1].
+   Vertex c takes the upper node on axis k when bit d - 1 - k of c is set;
+   its weight multiplies the axis factors from axis 0 up.  Non-finite
+   coordinates make the state invalid.  Replaces the host numpy stencils
+   RelativeTable.stencil (reversible/relative_table.py:76-109) and
+   interpolation_stencil (reversible/world_table.py:13-45). */
+int pe_grid_stencil(int32_t d, const int32_t *n, const double *lo,
+                    const double *hi, const double *step, const double *period,
+                    uint32_t periodic, double slack, const double *states,
+                    int64_t m, int64_t *index, double *weights, uint8_t *valid);
+
 /* ======================================================================
  * Compact 8D semi-Lagrangian min-max backup of the contact game.
  *

@@ -142,6 +142,8 @@ def _declare_8d(L):
     L.pe_oracle_sl_init.restype = None
     L.pe_oracle_car_stencil.restype = ctypes.c_int64
     L.pe_oracle_car_stencil.argtypes = [P, P, P, P, P, P, P, I64, P, I32, D, P, P, P, P, P, P]
+    L.pe_oracle_grid_stencil.restype = None
+    L.pe_oracle_grid_stencil.argtypes = [I32, P, P, P, P, P, ctypes.c_uint32, D, P, I64, P, P, P]
     L.pe_oracle_nearest.restype = None
     L.pe_oracle_nearest.argtypes = [P, I32, P, I64, I32, D, D, D, P, P]
     L.pe_oracle_sl_sweep.restype = ctypes.c_double
@@ -397,3 +399,19 @@ def nearest(features, queries, k, dx, dh, dv):
     lib().pe_oracle_nearest(_ptr(f), len(f), _ptr(q), len(q), k, dx, dh, dv, _ptr(ids),
                             _ptr(dist))
     return ids, dist
+
+
+def grid_stencil(states, n, lo, hi, step, period=None, periodic=0, slack=1e-10):
+    """Multilinear stencil on a uniform grid (stencil.c): index [m, 2^d]
+    int64, weights [m, 2^d] float64, valid [m] bool."""
+    d = len(n)
+    q = np.ascontiguousarray(np.asarray(states, np.float64).reshape(-1, d))
+    arr = [np.ascontiguousarray(a, t) for a, t in ((n, np.int32), (lo, np.float64),
+                                                   (hi, np.float64), (step, np.float64))]
+    per = np.ascontiguousarray(period if period is not None else np.zeros(d), np.float64)
+    idx = np.empty((len(q), 1 << d), np.int64)
+    w = np.empty((len(q), 1 << d), np.float64)
+    valid = np.empty(len(q), np.uint8)
+    lib().pe_oracle_grid_stencil(d, *[_ptr(a) for a in arr], _ptr(per), int(periodic), slack,
+                                 _ptr(q), len(q), _ptr(idx), _ptr(w), _ptr(valid))
+    return idx, w, valid.astype(bool)

@@ -253,3 +253,65 @@ int64_t pe_oracle_car_stencil(const int *n, const double *ax0, const double *ax1
   free(feat);
   return fallbacks;
 }
+
+/* Multilinear stencil on a uniform grid of d <= 8 axes (the 5D reader's
+ * stencil, reversible/relative_table.py:76-109, and the dense 8D game's
+ * interpolation_stencil, reversible/world_table.py:13-45): per axis the
+ * lower node and fraction (periodic axes: numpy remainder of q - lo by the
+ * period, over the step; others: clipped to [lo, hi], lower node at most
+ * n - 2, fraction clipped to [0, 1], validity within slack), then the 2^d
+ * vertices in itertools.product order with weights multiplied from axis 0. */
+void pe_oracle_grid_stencil(int d, const int *n, const double *lo, const double *hi,
+                            const double *step, const double *period, unsigned periodic,
+                            double slack, const double *states, int64_t m, int64_t *index,
+                            double *weights, uint8_t *valid) {
+  long long stride[8];
+  stride[d - 1] = 1;
+  for (int k = d - 2; k >= 0; k--) stride[k] = stride[k + 1] * n[k + 1];
+  const int corners = 1 << d;
+  for (int64_t i = 0; i < m; i++) {
+    long long node[8][2];
+    double fr[8];
+    int ok = 1;
+    for (int k = 0; k < d; k++) {
+      double x = states[(size_t)d * i + k];
+      if (!isfinite(x)) {
+        ok = 0;
+        x = 0.;
+      }
+      if ((periodic >> k) & 1u) {
+        const double f = numpy_remainder(x - lo[k], period[k]) / step[k];
+        const double fl = floor(f);
+        long long j = (long long)fl % n[k];
+        if (j < 0) j += n[k];
+        node[k][0] = j;
+        node[k][1] = (j + 1) % n[k];
+        fr[k] = f - fl;
+      } else {
+        if (!(x >= lo[k] - slack && x <= hi[k] + slack)) ok = 0;
+        double z = x < lo[k] ? lo[k] : x;
+        if (z > hi[k]) z = hi[k];
+        const double f = (z - lo[k]) / step[k];
+        long long j = (long long)floor(f);
+        if (j > n[k] - 2) j = n[k] - 2;
+        double t = f - (double)j;
+        t = t < 0. ? 0. : t > 1. ? 1. : t;
+        node[k][0] = j;
+        node[k][1] = j + 1;
+        fr[k] = t;
+      }
+    }
+    valid[i] = (uint8_t)ok;
+    for (int c = 0; c < corners; c++) {
+      long long at = 0;
+      double w = 1.;
+      for (int k = 0; k < d; k++) {
+        const int up = (c >> (d - 1 - k)) & 1;
+        at += node[k][up] * stride[k];
+        w = w * (up ? fr[k] : 1. - fr[k]);
+      }
+      index[(size_t)corners * i + c] = at;
+      weights[(size_t)corners * i + c] = w;
+    }
+  }
+}

@@ -429,3 +429,135 @@ extern "C" int pe_car_stencil(const pe_car_axes *axes, const int32_t *dense_to_c
   release();
   return rc;
 }
+
+// ---------------------------------------------------------------------------
+// Multilinear stencils on a uniform grid of up to 8 axes (the 5D reader's
+// RelativeTable.stencil, reversible/relative_table.py:76-109, and the probe
+// stencils of the dense 8D game, reversible/world_table.py:13-45).  Axis k:
+// n[k] nodes lo[k] + j step[k].  Periodic axes: f = ((q - lo) mod period) /
+// step, numpy's remainder, lower node floor(f) mod n, the upper node wraps.
+// Other axes: valid within [lo, hi] +- slack, f = (clip(q, lo, hi) - lo) /
+// step, lower node min(floor(f), n - 2), fraction clipped to [0, 1].
+// Vertex c takes the upper node on axis k when bit d - 1 - k of c is set
+// (itertools.product order); its weight is the product over the axes, axis
+// 0 first.  Non-finite coordinates count as 0 and make the state invalid.
+namespace pe {
+
+struct UniformGrid {
+  int d;
+  int n[8];
+  long long stride[8];
+  double lo[8], hi[8], step[8], period[8];
+  unsigned periodic;
+  double slack;
+};
+
+__global__ __launch_bounds__(128) void grid_stencil(UniformGrid g, const double *__restrict__ q,
+                                                    long long m, long long *__restrict__ index,
+                                                    double *__restrict__ weight,
+                                                    uint8_t *__restrict__ valid) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  long long low[8], high[8];
+  double frac[8];
+  bool ok = true;
+  for (int k = 0; k < g.d; k++) {
+    double x = q[(long long)g.d * i + k];
+    if (!isfinite(x)) {
+      ok = false;
+      x = 0.0;
+    }
+    const int n = g.n[k];
+    long long j, j1;
+    double t;
+    if ((g.periodic >> k) & 1u) {
+      double r = fmod(x - g.lo[k], g.period[k]);
+      if (r != 0.0 && r < 0.0) r += g.period[k];
+      const double f = r / g.step[k];
+      const double fl = floor(f);
+      j = ((long long)fl % n + n) % n;
+      j1 = (j + 1) % n;
+      t = f - fl;
+    } else {
+      ok = ok && x >= g.lo[k] - g.slack && x <= g.hi[k] + g.slack;
+      const double f = (fmin(fmax(x, g.lo[k]), g.hi[k]) - g.lo[k]) / g.step[k];
+      j = (long long)floor(f);
+      if (j > n - 2) j = n - 2;
+      j1 = j + 1;
+      t = fmin(fmax(f - (double)j, 0.0), 1.0);
+    }
+    low[k] = j * g.stride[k];
+    high[k] = j1 * g.stride[k];
+    frac[k] = t;
+  }
+  const int corners = 1 << g.d;
+  long long *out_i = index + (long long)corners * i;
+  double *out_w = weight + (long long)corners * i;
+  for (int c = 0; c < corners; c++) {
+    long long idx = 0;
+    double w = 1.0;
+    for (int k = 0; k < g.d; k++) {
+      const bool up = (c >> (g.d - 1 - k)) & 1;
+      idx += up ? high[k] : low[k];
+      w *= up ? frac[k] : 1.0 - frac[k];
+    }
+    out_i[c] = idx;
+    out_w[c] = w;
+  }
+  valid[i] = ok ? 1 : 0;
+}
+
+}  // namespace pe
+
+extern "C" int pe_grid_stencil(int32_t d, const int32_t *n, const double *lo, const double *hi,
+                               const double *step, const double *period, uint32_t periodic,
+                               double slack, const double *states, int64_t m, int64_t *index,
+                               double *weights, uint8_t *valid) {
+  clear_error();
+  if (d < 1 || d > 8 || !n || !lo || !hi || !step || m < 0 ||
+      (periodic && !period) || (m > 0 && (!states || !index || !weights || !valid)) ||
+      !(slack >= 0) || (periodic >> d) != 0)
+    return fail(PE_ERR_INVALID, "invalid argument");
+  UniformGrid g{};
+  g.d = d;
+  g.periodic = periodic;
+  g.slack = slack;
+  long long stride = 1;
+  for (int k = d - 1; k >= 0; k--) {
+    const bool wraps = (periodic >> k) & 1u;
+    if (n[k] < (wraps ? 1 : 2) || !(step[k] > 0) || !std::isfinite(lo[k]) ||
+        !std::isfinite(hi[k]) || !(hi[k] >= lo[k]) || (wraps && !(period[k] > 0)))
+      return fail(PE_ERR_INVALID, "Expected uniformly spaced, nondegenerate axes");
+    g.n[k] = n[k];
+    g.stride[k] = stride;
+    stride *= n[k];
+    g.lo[k] = lo[k];
+    g.hi[k] = hi[k];
+    g.step[k] = step[k];
+    g.period[k] = wraps ? period[k] : 0.0;
+  }
+  int rc = require_device(nullptr);
+  if (rc) return rc;
+  if (m == 0) return PE_OK;
+  const long long corners = 1ll << d;
+  void *d_q = nullptr, *d_i = nullptr, *d_w = nullptr, *d_v = nullptr;
+  auto body = [&]() -> int {
+    PE_HIP(hipMalloc(&d_q, 8 * (size_t)d * m));
+    PE_HIP(hipMalloc(&d_i, 8 * (size_t)corners * m));
+    PE_HIP(hipMalloc(&d_w, 8 * (size_t)corners * m));
+    PE_HIP(hipMalloc(&d_v, (size_t)m));
+    PE_HIP(hipMemcpy(d_q, states, 8 * (size_t)d * m, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(grid_stencil, dim3((unsigned)((m + 127) / 128)), dim3(128), 0, 0, g,
+                       (const double *)d_q, (long long)m, (long long *)d_i, (double *)d_w,
+                       (uint8_t *)d_v);
+    PE_LAUNCHED(grid_stencil);
+    PE_HIP(hipMemcpy(index, d_i, 8 * (size_t)corners * m, hipMemcpyDeviceToHost));
+    PE_HIP(hipMemcpy(weights, d_w, 8 * (size_t)corners * m, hipMemcpyDeviceToHost));
+    PE_HIP(hipMemcpy(valid, d_v, (size_t)m, hipMemcpyDeviceToHost));
+    return PE_OK;
+  };
+  rc = body();
+  for (void *p : {d_q, d_i, d_w, d_v})
+    if (p) (void)hipFree(p);
+  return rc;
+}

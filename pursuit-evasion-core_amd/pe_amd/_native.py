@@ -263,6 +263,8 @@ SIGNATURES = {
     "pe_geom_backups_multi": (ctypes.c_int, [P, I64, P, I32, P, P, P, P, P]),
     "pe_geom_actions_multi": (ctypes.c_int, [P, I64, P, I32, P, I32, F64, P, P, P, P, P, P]),
     "pe_geom_connections": (ctypes.c_int, [P, P, I64, P, I32, F64, P, P]),
+    "pe_grid_stencil": (ctypes.c_int, [I32, P, P, P, P, P, ctypes.c_uint32, F64, P, I64, P, P,
+                                       P]),
     "pe_car_stencil": (ctypes.c_int, [ctypes.POINTER(CarAxes), P, P, I64, P, I32, F64, P, P, P,
                                       P, P, P]),
     "pe_sl_create": (ctypes.c_int, [ctypes.POINTER(SLGame), I32, P]),
@@ -347,6 +349,24 @@ def copy_bandwidth(nbytes, repeats=10):
     gbps = ctypes.c_double()
     check(lib().pe_copy_bandwidth(int(nbytes), int(repeats), ctypes.byref(gbps)))
     return gbps.value
+
+
+def grid_stencil(states, n, lo, hi, step, period=None, periodic=0, slack=1e-10):
+    """pe_grid_stencil: multilinear stencils of states [m, d] on a uniform
+    grid; returns node index [m, 2^d] int64, weights [m, 2^d] float64 and
+    validity [m] bool."""
+    d = len(n)
+    q = np.ascontiguousarray(np.asarray(states, np.float64).reshape(-1, d))
+    dims = np.ascontiguousarray(n, np.int32)
+    lo_, hi_, st = (np.ascontiguousarray(a, np.float64) for a in (lo, hi, step))
+    per = None if period is None else np.ascontiguousarray(period, np.float64)
+    idx = np.empty((len(q), 1 << d), np.int64)
+    w = np.empty((len(q), 1 << d), np.float64)
+    valid = np.empty(len(q), np.uint8)
+    check(lib().pe_grid_stencil(d, ptr(dims), ptr(lo_), ptr(hi_), ptr(st), ptr(per),
+                                int(periodic), float(slack), ptr(q), len(q), ptr(idx), ptr(w),
+                                ptr(valid)))
+    return idx, w, valid.astype(bool)
 
 
 def car_stencil(axes, dense_to_compact, states, geometry=None):

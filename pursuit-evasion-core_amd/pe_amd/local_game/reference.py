@@ -41,83 +41,106 @@ from .native import GeometryEngine
 
 
 def write_half_preserving_sign(values, file):
-    """float16 history that never rounds a nonzero value to zero
-    (reference.py:19-25)."""
-    for start in range(0, len(values), 2_000_000):
-        x = values[start: start + 2_000_000]
-        y = x.astype(np.float16)
-        y[(x > 0) & (y == 0)] = np.nextafter(np.float16(0), np.float16(1))
-        y[(x < 0) & (y == 0)] = np.nextafter(np.float16(0), np.float16(-1))
-        y.tofile(file)
+    """Append values as float16, keeping the sign of every nonzero value: a
+    value that would round to zero becomes the smallest subnormal of its
+    sign (reference.py:19-25)."""
+    tiny = np.float16(2.0 ** -24)
+    for start in range(0, len(values), 1 << 21):
+        x = np.asarray(values[start: start + (1 << 21)])
+        half = x.astype(np.float16)
+        lost = (half == 0) & (x != 0)
+        half[lost] = np.where(x[lost] > 0, tiny, -tiny)
+        half.tofile(file)
+
+
+def _pick_frame(rng, own, other, domain, arena, per_frame, pose_ok):
+    """One observation frame: a pursuer pose with enough evader states in
+    range and `per_frame` non-captured pairs at that pose; None if the draw
+    fails (the caller redraws)."""
+    i = int(rng.integers(len(own.states)))
+    if not pose_ok[i]:
+        return None
+    pose = own.states[i, :3]
+    near = np.flatnonzero(np.hypot(*(other.states[:, :2] - pose[:2]).T)
+                          <= domain.exit_radius - 0.1)
+    same = np.flatnonzero((own.states[:, :3] == pose).all(1))
+    e_ids = near[rng.integers(len(near), size=3 * per_frame)]
+    p_ids = same[rng.integers(len(same), size=3 * per_frame)]
+    free = arena.capture_value(np.hstack([own.states[p_ids], other.states[e_ids]])) > 0.03
+    if free.sum() < per_frame:
+        return None
+    return pose, p_ids[free][:per_frame], e_ids[free][:per_frame]
+
+
+def _sensor_pose(rng, pose, arena, role, jitter):
+    """The observed pose: a jittered copy of `pose` whose footprint is clear
+    (the first of 20 draws), else the pose itself."""
+    draws = pose + rng.uniform(-1, 1, (20, 3)) * np.array([jitter, jitter, 0.11])
+    clear = arena.clearance(np.hstack([draws, np.zeros((20, 1))]), role) > 1e-4
+    return draws[int(np.argmax(clear))] if clear.any() else pose.copy()
+
+
+def _perturb(rng, states, config, domain, arena, p, e):
+    """Continuous query states: half the pairs moved off the grid nodes
+    (position +-0.12 m, heading +-0.15 rad, speed +-0.1 m/s, clipped to the
+    limits); a move is kept when both cars stay legal and interpolable, the
+    pair uncaptured and inside the exit radius."""
+    m = len(states)
+    moved = states.copy()
+    for col, limits in ((0, config.pursuer), (4, config.evader)):
+        moved[:, col: col + 2] += rng.uniform(-0.12, 0.12, (m, 2))
+        moved[:, col + 2] = wrap_angle(moved[:, col + 2] + rng.uniform(-0.15, 0.15, m))
+        moved[:, col + 3] = np.clip(moved[:, col + 3] + rng.uniform(-0.1, 0.1, m),
+                                    limits.speed_min, limits.speed_max)
+    chosen = rng.random(m) < 0.5
+    engines = [GeometryEngine(arena, lim, role, control_dt=config.control_dt)
+               for lim, role in ((config.pursuer, "pursuer"), (config.evader, "evader"))]
+    keep = engines[0].backup_safe(moved[:, :4]) & engines[1].backup_safe(moved[:, 4:])
+    keep &= arena.capture_value(moved) > 0.03
+    keep &= np.hypot(*(moved[:, :2] - moved[:, 4:6]).T) < domain.exit_radius - 0.02
+    keep &= p.stencil(moved[:, :4])[2] & e.stencil(moved[:, 4:])[2]
+    chosen &= keep
+    out = states.copy()
+    out[chosen] = moved[chosen]
+    return out, chosen
 
 
 def sample_queries(p, e, domain, arena, config, *, frames=192, per_frame=256, seed=1031):
-    """Observation frames and query pair states for the labels
-    (reference.py:28-124): per frame a pursuer pose with >= 30 evader states
-    nearby, `per_frame` non-captured pairs, a jittered sensor pose, and a
-    continuous perturbation of half the query states kept when legal."""
+    """Observation frames and query pair states for the labels, with the
+    reference's distribution (reference.py:28-124; its draws are not
+    reproduced): per frame a pursuer pose drawn uniformly over the pursuer
+    states that has at least 30 evader states within exit_radius - 0.1,
+    `per_frame` non-captured (payoff > 0.03) pairs at that pose drawn from
+    3 * per_frame uniform candidates, a sensor pose jittered by up to
+    0.11 m / 0.11 rad with a clear footprint, and a continuous perturbation
+    of half the query states kept when legal."""
     rng = np.random.default_rng(seed)
-    observation, pi, ei, frame_ids = [], [], [], []
-    for f in range(frames):
-        observer = 0
-        own, other = (p, e) if observer == 0 else (e, p)
+    in_range = (np.hypot(p.states[:, None, 0] - e.states[None, :, 0],
+                         p.states[:, None, 1] - e.states[None, :, 1])
+                <= domain.exit_radius - 0.1).sum(1)
+    pose_ok = in_range >= 30
+    sensors, p_all, e_all = [], [], []
+    for _ in range(frames):
         for _ in range(1000):
-            i = int(rng.integers(len(own.states)))
-            pose = own.states[i, :3].copy()
-            nearby = np.flatnonzero(
-                np.linalg.norm(other.states[:, :2] - pose[:2], axis=1) <= domain.exit_radius - 0.1)
-            if len(nearby) < 30:
-                continue
-            j = rng.choice(nearby, per_frame * 3, replace=True)
-            same = np.flatnonzero(np.all(own.states[:, :3] == pose, axis=1))
-            ii = rng.choice(same, len(j), replace=True)
-            pids, eids = (ii, j) if observer == 0 else (j, ii)
-            q = np.concatenate([p.states[pids], e.states[eids]], 1)
-            keep = arena.capture_value(q) > 0.03
-            pids, eids = pids[keep][:per_frame], eids[keep][:per_frame]
-            if len(pids) < per_frame:
-                continue
-            sensor = pose.copy()
-            for _ in range(20):
-                jitter = 0.11 if observer == 0 else 0.06
-                sensor = pose + np.array([rng.uniform(-jitter, jitter),
-                                          rng.uniform(-jitter, jitter),
-                                          rng.uniform(-0.11, 0.11)])
-                role = "pursuer" if observer == 0 else "evader"
-                if arena.clearance(np.r_[sensor, 0], role) > 1e-4:
-                    break
-            else:
-                sensor = pose.copy()
-            observation.append([*sensor, observer])
-            pi.extend(pids)
-            ei.extend(eids)
-            frame_ids.extend([f] * per_frame)
-            break
+            picked = _pick_frame(rng, p, e, domain, arena, per_frame, pose_ok)
+            if picked is not None:
+                break
         else:
             raise ValueError("Could not sample a legal observation/query group")
-    pi = np.array(pi, np.int32)
-    ei = np.array(ei, np.int32)
-    local = domain.lookup[domain.p_position[pi], ei]
-    assert (local >= 0).all()
-    compact = domain.offsets[pi] + local
-    states = np.concatenate([p.states[pi], e.states[ei]], 1)
-    proposal = states.copy()
-    continuous = rng.random(len(states)) < 0.5
-    for k, limits in ((0, config.pursuer), (4, config.evader)):
-        proposal[:, k: k + 2] += rng.uniform(-0.12, 0.12, (len(states), 2))
-        proposal[:, k + 2] = wrap_angle(proposal[:, k + 2] + rng.uniform(-0.15, 0.15, len(states)))
-        proposal[:, k + 3] = np.clip(proposal[:, k + 3] + rng.uniform(-0.1, 0.1, len(states)),
-                                     limits.speed_min, limits.speed_max)
-    pe_ = GeometryEngine(arena, config.pursuer, "pursuer", control_dt=config.control_dt)
-    ee_ = GeometryEngine(arena, config.evader, "evader", control_dt=config.control_dt)
-    legal = pe_.backup_safe(proposal[:, :4]) & ee_.backup_safe(proposal[:, 4:])
-    legal &= (arena.capture_value(proposal) > 0.03) & (
-        np.linalg.norm(proposal[:, :2] - proposal[:, 4:6], axis=1) < domain.exit_radius - 0.02)
-    legal &= p.stencil(proposal[:, :4])[2] & e.stencil(proposal[:, 4:])[2]
-    continuous &= legal
-    states[continuous] = proposal[continuous]
-    return dict(frames=np.array(observation), frame_ids=np.array(frame_ids, np.int32),
-                p_indices=pi, e_indices=ei, original_compact_indices=compact,
+        pose, p_ids, e_ids = picked
+        sensors.append([*_sensor_pose(rng, pose, arena, "pursuer", 0.11), 0])
+        p_all.append(p_ids)
+        e_all.append(e_ids)
+    pi = np.concatenate(p_all).astype(np.int32)
+    ei = np.concatenate(e_all).astype(np.int32)
+    rank = domain.lookup[domain.p_position[pi], ei]
+    if (rank < 0).any():
+        raise RuntimeError("a query pair lies outside the pair domain")
+    nodes = np.hstack([p.states[pi], e.states[ei]])
+    states, continuous = _perturb(rng, nodes, config, domain, arena, p, e)
+    return dict(frames=np.array(sensors), frame_ids=np.repeat(np.arange(frames, dtype=np.int32),
+                                                              per_frame),
+                p_indices=pi, e_indices=ei, original_compact_indices=domain.offsets[pi] + rank,
                 states=states, continuous=continuous)
 
 
@@ -131,6 +154,189 @@ def build_grids(config, arena, *, headings=16, position_step=0.3, positions=None
     e = make_grid(arena, config.evader, "evader",
                   speeds=np.linspace(config.evader.speed_min, config.evader.speed_max, e_speeds), **kw)
     return p, e
+
+
+def _solve_spec(config, arena, **options):
+    """The solve's spec (what the signature covers) and its sha256."""
+    spec = dict(config=config.to_dict(), arena=arena.to_dict(), query_version=3)
+    extra = ("positions", "half_width", "p_speeds", "e_speeds")
+    custom = any(options[k] is not None for k in extra[:2]) or \
+        (options["p_speeds"], options["e_speeds"]) != (7, 7)
+    spec.update({k: v for k, v in options.items() if k not in extra or custom})
+    return spec, hashlib.sha256(json.dumps(spec, sort_keys=True).encode()).hexdigest()
+
+
+def _atomic_json(path, data):
+    tmp = path.with_suffix(".tmp")
+    tmp.write_text(json.dumps(data, indent=2))
+    os.replace(tmp, path)
+
+
+_META_TEXT = dict(
+    schema="local_contact_compact8d_v1",
+    history_dtype="float16_sign_preserved",
+    backend="hip_gfx950",
+    interpolation="multilinear, omit invalid single-vehicle vertices and renormalize",
+    action_constraint="continuous obstacle-free command plus one collision-free "
+                      "constant-curvature braking backup",
+    objective="capture by budget before center separation exceeds local radius; "
+              "range exit is local failure, not a physical capture",
+    numerical_game="finite-grid semi-Lagrangian minP maxE, exact factorization of "
+                   "the product interpolation; no dimension reduction",
+)
+
+
+class _ContactSolve:
+    """One contact solve writing into `output`: tables and queries (built, or
+    loaded when resuming), the device solver, and the output records."""
+
+    def __init__(self, config, arena, output, spec, signature, opts):
+        self.config, self.arena, self.output = config, arena, Path(output)
+        self.opts = opts
+        self.dt = config.control_dt
+        self.meta_path = self.output / "meta.json"
+        self.resumed = self.meta_path.exists()
+        if self.resumed:
+            self.meta = json.loads(self.meta_path.read_text())
+            if self.meta["signature"] != signature:
+                raise ValueError("Existing output belongs to another model/setup")
+            if self.meta["status"] == "complete":
+                return
+            self.p = VehicleGrid.load(self.output / "p_grid.npz")
+            self.e = VehicleGrid.load(self.output / "e_grid.npz")
+            self.domain = PairDomain.load(self.output / "domain.npz")
+            self.queries = dict(np.load(self.output / "queries.npz"))
+        else:
+            self._prepare(spec, signature)
+
+    def _prepare(self, spec, signature):
+        o = self.opts
+        self.p, self.e = build_grids(self.config, self.arena, headings=o["headings"],
+                                     position_step=o["position_step"],
+                                     positions=o["positions"], half_width=o["half_width"],
+                                     p_speeds=o["p_speeds"], e_speeds=o["e_speeds"])
+        self.domain = PairDomain.build(self.p, self.e, o["exit_radius"])
+        for name, obj in (("p_grid", self.p), ("e_grid", self.e), ("domain", self.domain)):
+            obj.save(self.output / f"{name}.npz")
+        self.queries = sample_queries(self.p, self.e, self.domain, self.arena, self.config,
+                                      frames=o["frames"], per_frame=o["per_frame"],
+                                      seed=o["seed"])
+        np.savez_compressed(self.output / "queries.npz", **self.queries)
+        self.meta = dict(_META_TEXT, status="running", signature=signature, spec=spec,
+                         p_states=len(self.p.states), e_states=len(self.e.states),
+                         pair_states=self.domain.states,
+                         unpruned_pair_states=len(self.p.states) * len(self.e.states),
+                         completed_step=0, completed_outputs=0, times=[],
+                         update_seconds=0.0)
+        self.meta_path.write_text(json.dumps(self.meta, indent=2))
+
+    def check_resources(self, frames_kept):
+        """Disk for the history and checkpoint plus an 8 GB reserve; device
+        memory of the value fields and the work buffer under max_gb."""
+        states = self.domain.states
+        history = (2 * frames_kept * states) if self.opts["retain_history"] else 0
+        present = sum(f.stat().st_size for f in self.output.iterdir() if f.is_file())
+        if shutil.disk_usage(self.output).free < max(0, history + 8 * states - present) + 8e9:
+            raise OSError("Insufficient disk space for history/checkpoint transaction plus "
+                          "8GB reserve")
+        pairs = len(self.p.states) * len(self.e.states)
+        self.need = pairs * (9 + 4 * len(self.p.controls))
+        if self.need > self.opts["max_gb"] * 1e9:
+            raise MemoryError(f"{self.need / 1e9:.2f}GB required; guard {self.opts['max_gb']}GB")
+
+    def open(self):
+        for grid, limits, role in ((self.p, self.config.pursuer, "pursuer"),
+                                   (self.e, self.config.evader, "evader")):
+            grid.bind_geometry(self.arena, limits, role, self.dt)
+        self.solver = GameSolver(game_tables(self.p, self.e, self.arena,
+                                             self.opts["exit_radius"]))
+        q = self.queries["states"]
+        ps, es = self.p.stencil(q[:, :4]), self.e.stencil(q[:, 4:])
+        if not (ps[2] & es[2]).all():
+            raise ValueError("Generated query has no numerical support")
+        self.query_stencils = (ps[0], ps[1], es[0], es[1])
+
+    def start(self, warm_start):
+        """Initial field and the records already written; returns the first
+        step still to run."""
+        meta, exit_value = self.meta, self.opts["exit_radius"]
+        checkpoint = self.output / meta.get("checkpoint", "missing")
+        if self.resumed and meta["completed_outputs"] and checkpoint.exists():
+            packed = np.fromfile(checkpoint, np.float32, count=self.domain.states)
+            self.solver.initialize(self.domain.unpack(packed, exit_value, len(self.e.states)))
+            done = meta["completed_outputs"]
+            with np.load(self.output / "labels.npz") as prior:
+                self.labels = list(prior["values"][:done])
+                self.times = list(prior["times"][:done])
+            first = meta["completed_step"] + 1
+        else:
+            self.solver.initialize()
+            if warm_start is not None:
+                phi = self.solver.read()
+                self.solver.initialize(np.minimum(phi, np.asarray(warm_start, np.float32)))
+            self.labels, self.times, first = [], [], 1
+        self.history = self.output / "fields.f16"
+        if self.opts["retain_history"]:
+            with self.history.open("ab" if self.history.exists() else "wb") as f:
+                f.truncate(meta["completed_outputs"] * self.domain.states * 2)
+        if not self.labels:
+            self.record(0)
+        return first
+
+    def record(self, step):
+        """Labels, history frame and an atomically replaced checkpoint of
+        the field after `step` sweeps; meta.json is rewritten last."""
+        field = self.solver.read()
+        if step == 0:
+            label = self.arena.capture_value(self.queries["states"]).astype(np.float32)
+        else:
+            label = self.solver.query(*self.query_stencils)
+        self.labels.append(label)
+        self.times.append(step * self.dt)
+        packed = self.domain.pack(field)
+        if self.opts["retain_history"]:
+            with self.history.open("ab") as f:
+                write_half_preserving_sign(packed, f)
+        np.savez_compressed(self.output / "labels.npz", times=np.array(self.times),
+                            values=np.stack(self.labels))
+        name = f"checkpoint_{step:04d}.f32"
+        partial = self.output / (name + ".partial")
+        packed.tofile(partial)
+        os.replace(partial, self.output / name)
+        previous = self.meta.get("checkpoint")
+        self.meta.update(completed_step=step, completed_outputs=len(self.times),
+                         times=self.times, checkpoint=name, gpu_working_bytes=self.need,
+                         host_working_bytes=self.domain.states * 4)
+        _atomic_json(self.meta_path, self.meta)
+        if previous not in (None, name) and (self.output / previous).exists():
+            (self.output / previous).unlink()
+        print(f"budget {step * self.dt:.2f}s; states {self.domain.states:,}; "
+              f"updates {self.meta['update_seconds']:.1f}s", flush=True)
+
+    def sweep(self, count, flags):
+        before = self.solver.stats()["update_seconds"]
+        self.solver.sweep(count, flags)
+        self.meta["update_seconds"] += self.solver.stats()["update_seconds"] - before
+
+    def to_residual(self, tol, max_sweeps, flags):
+        before = self.solver.stats()["update_seconds"]
+        done, converged = self.solver.run(tol, max_sweeps, 8, flags)
+        st = self.solver.stats()
+        self.meta.update(residual_tol=tol, residual_sweeps=done, residual_converged=converged,
+                         final_residual=st["last_residual"],
+                         residual_seconds=st["update_seconds"] - before)
+        np.asarray(self.domain.pack(self.solver.read()), np.float32).tofile(
+            self.output / "converged.f32")
+
+    def finish(self, started):
+        self.meta.update(status="complete",
+                         wall_seconds_this_process=time.perf_counter() - started)
+        self.meta_path.write_text(json.dumps(self.meta, indent=2))
+        (self.output / self.meta.pop("checkpoint")).unlink()
+        self.meta["saved_bytes"] = sum(f.stat().st_size for f in self.output.iterdir()
+                                       if f.is_file())
+        self.meta_path.write_text(json.dumps(self.meta, indent=2))
+        return self.meta
 
 
 def solve(
@@ -157,152 +363,47 @@ def solve(
     warm_start=None,
     flags=_native.PE_SL_EARLY_EXIT | _native.PE_SL_ORDERED,
 ):
-    """Streaming contact-game solve with resumable atomic checkpoints."""
+    """Contact-game solve over `horizon` with a record every `output_dt`,
+    resumable from its last atomic checkpoint (the reference's solve,
+    reference.py:138-369)."""
     if (config.capture_rule != "rectangle_contact"
             or config.boundary_rule != "collision_constrained" or config.point_vehicles):
         raise ValueError("Use the explicit finite-body collision-constrained contact game")
+    dt = config.control_dt
+    steps, every = round(horizon / dt), round(output_dt / dt)
+    whole = np.isclose(steps * dt, horizon) and np.isclose(every * dt, output_dt)
+    if min(steps, every) < 1 or not whole:
+        raise ValueError("Budgets must be integer control periods")
     output = Path(output)
     output.mkdir(parents=True, exist_ok=True)
-    dt = config.control_dt
-    steps = round(horizon / dt)
-    every = round(output_dt / dt)
-    if min(steps, every) < 1 or not np.isclose(steps * dt, horizon) or not np.isclose(every * dt, output_dt):
-        raise ValueError("Budgets must be integer control periods")
-    spec = dict(config=config.to_dict(), arena=arena.to_dict(), headings=headings,
-                position_step=position_step, horizon=horizon, output_dt=output_dt,
-                exit_radius=exit_radius, retain_history=retain_history, frames=frames,
-                per_frame=per_frame, seed=seed, query_version=3)
-    if positions is not None or half_width is not None or (p_speeds, e_speeds) != (7, 7):
-        spec.update(positions=positions, half_width=half_width, p_speeds=p_speeds,
-                    e_speeds=e_speeds)
-    signature = hashlib.sha256(json.dumps(spec, sort_keys=True).encode()).hexdigest()
-    mp = output / "meta.json"
     started = time.perf_counter()
-    resume = mp.exists()
-    if resume:
-        meta = json.loads(mp.read_text())
-        if meta["signature"] != signature:
-            raise ValueError("Existing output belongs to another model/setup")
-        if meta["status"] == "complete":
-            return meta
-        p = VehicleGrid.load(output / "p_grid.npz")
-        e = VehicleGrid.load(output / "e_grid.npz")
-        domain = PairDomain.load(output / "domain.npz")
-        queries = dict(np.load(output / "queries.npz"))
-    else:
-        p, e = build_grids(config, arena, headings=headings, position_step=position_step,
-                           positions=positions, half_width=half_width,
-                           p_speeds=p_speeds, e_speeds=e_speeds)
-        domain = PairDomain.build(p, e, exit_radius)
-        p.save(output / "p_grid.npz")
-        e.save(output / "e_grid.npz")
-        domain.save(output / "domain.npz")
-        queries = sample_queries(p, e, domain, arena, config, frames=frames,
-                                 per_frame=per_frame, seed=seed)
-        np.savez_compressed(output / "queries.npz", **queries)
-        meta = dict(
-            schema="local_contact_compact8d_v1", status="running", signature=signature,
-            spec=spec, p_states=len(p.states), e_states=len(e.states),
-            pair_states=domain.states, unpruned_pair_states=len(p.states) * len(e.states),
-            history_dtype="float16_sign_preserved", completed_step=0, completed_outputs=0,
-            times=[], update_seconds=0.0, backend="hip_gfx950",
-            interpolation="multilinear, omit invalid single-vehicle vertices and renormalize",
-            action_constraint="continuous obstacle-free command plus one collision-free "
-                              "constant-curvature braking backup",
-            objective="capture by budget before center separation exceeds local radius; "
-                      "range exit is local failure, not a physical capture",
-            numerical_game="finite-grid semi-Lagrangian minP maxE, exact factorization of "
-                           "the product interpolation; no dimension reduction",
-        )
-        mp.write_text(json.dumps(meta, indent=2))
-    p.bind_geometry(arena, config.pursuer, "pursuer", dt)
-    e.bind_geometry(arena, config.evader, "evader", dt)
-    expected = (len(range(0, steps + 1, every)) * 2 if retain_history else 0) * domain.states \
-        + 8 * domain.states
-    already = sum(f.stat().st_size for f in output.iterdir() if f.is_file())
-    if shutil.disk_usage(output).free < max(0, expected - already) + 8 * 10**9:
-        raise OSError("Insufficient disk space for history/checkpoint transaction plus 8GB reserve")
-    need = len(p.states) * len(e.states) * (4 * 2 + 1 + 4 * len(p.controls))
-    if need > max_gb * 1e9:
-        raise MemoryError(f"{need / 1e9:.2f}GB required; guard {max_gb}GB")
-    tables = game_tables(p, e, arena, exit_radius)
-    solver = GameSolver(tables)
-    pi, pw, pvalid, _, _ = p.stencil(queries["states"][:, :4])
-    ei, ew, evalid, _, _ = e.stencil(queries["states"][:, 4:])
-    if not (pvalid & evalid).all():
-        raise ValueError("Generated query has no numerical support")
+    spec, signature = _solve_spec(
+        config, arena, headings=headings, position_step=position_step, horizon=horizon,
+        output_dt=output_dt, exit_radius=exit_radius, retain_history=retain_history,
+        frames=frames, per_frame=per_frame, seed=seed, positions=positions,
+        half_width=half_width, p_speeds=p_speeds, e_speeds=e_speeds)
+    opts = dict(headings=headings, position_step=position_step, exit_radius=exit_radius,
+                retain_history=retain_history, frames=frames, per_frame=per_frame, seed=seed,
+                max_gb=max_gb, positions=positions, half_width=half_width, p_speeds=p_speeds,
+                e_speeds=e_speeds)
+    run = _ContactSolve(config, arena, output, spec, signature, opts)
+    if run.resumed and run.meta["status"] == "complete":
+        return run.meta
+    run.check_resources(len(range(0, steps + 1, every)))
+    run.open()
     try:
-        checkpoint = output / meta.get("checkpoint", "missing")
-        if resume and meta["completed_outputs"] and checkpoint.exists():
-            packed = np.fromfile(checkpoint, np.float32, count=domain.states)
-            solver.initialize(domain.unpack(packed, exit_radius, len(e.states)))
-            prior = np.load(output / "labels.npz")
-            labels = list(prior["values"][: meta["completed_outputs"]])
-            times = list(prior["times"][: meta["completed_outputs"]])
-            start = meta["completed_step"]
-        else:
-            solver.initialize()
-            if warm_start is not None:
-                phi = solver.read()
-                solver.initialize(np.minimum(phi, np.asarray(warm_start, np.float32)))
-            labels, times, start = [], [], 0
-        history = output / "fields.f16"
-        if retain_history:
-            with history.open("ab" if history.exists() else "wb") as f:
-                f.truncate(meta["completed_outputs"] * domain.states * 2)
-
-        def save_output(step):
-            field = solver.read()
-            labels.append(arena.capture_value(queries["states"]).astype(np.float32)
-                          if step == 0 else solver.query(pi, pw, ei, ew))
-            times.append(step * dt)
-            packed = domain.pack(field)
-            if retain_history:
-                with history.open("ab") as file:
-                    write_half_preserving_sign(packed, file)
-            np.savez_compressed(output / "labels.npz", times=np.array(times), values=np.stack(labels))
-            name = f"checkpoint_{step:04d}.f32"
-            tmp = output / (name + ".partial")
-            packed.tofile(tmp)
-            os.replace(tmp, output / name)
-            old = meta.get("checkpoint")
-            meta.update(completed_step=step, completed_outputs=len(times), times=times,
-                        checkpoint=name, gpu_working_bytes=need,
-                        host_working_bytes=domain.states * 4)
-            temporary = mp.with_suffix(".tmp")
-            temporary.write_text(json.dumps(meta, indent=2))
-            os.replace(temporary, mp)
-            if old and old != name and (output / old).exists():
-                (output / old).unlink()
-            print(f'budget {step * dt:.2f}s; states {domain.states:,}; '
-                  f'updates {meta["update_seconds"]:.1f}s', flush=True)
-
-        if not labels:
-            save_output(0)
-        for step in range(start + 1, steps + 1):
-            before = solver.stats()["update_seconds"]
-            solver.sweep(1, flags)
-            meta["update_seconds"] += solver.stats()["update_seconds"] - before
+        step = run.start(warm_start)
+        while step <= steps:
+            run.sweep(1, flags)
             if step % every == 0 or step == steps:
-                save_output(step)
+                run.record(step)
+            step += 1
         if residual_tol is not None:
-            before = solver.stats()["update_seconds"]
-            done, converged = solver.run(residual_tol, max_sweeps, 8, flags)
-            meta.update(residual_tol=residual_tol, residual_sweeps=done,
-                        residual_converged=converged,
-                        final_residual=solver.stats()["last_residual"],
-                        residual_seconds=solver.stats()["update_seconds"] - before)
-            np.asarray(domain.pack(solver.read()), np.float32).tofile(output / "converged.f32")
-        meta["residual_trace"] = solver.trace()[:, 0].tolist()
+            run.to_residual(residual_tol, max_sweeps, flags)
+        run.meta["residual_trace"] = run.solver.trace()[:, 0].tolist()
     finally:
-        solver.close()
-    meta.update(status="complete", wall_seconds_this_process=time.perf_counter() - started)
-    mp.write_text(json.dumps(meta, indent=2))
-    (output / meta["checkpoint"]).unlink()
-    meta.pop("checkpoint")
-    meta["saved_bytes"] = sum(f.stat().st_size for f in output.iterdir() if f.is_file())
-    mp.write_text(json.dumps(meta, indent=2))
-    return meta
+        run.solver.close()
+    return run.finish(started)
 
 
 def contact_config(config=None):

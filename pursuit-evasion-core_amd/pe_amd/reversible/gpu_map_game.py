@@ -70,7 +70,7 @@ def solve(config, grid, geometry, output, *, horizon=1.5, output_dt=0.5, order=2
     if query_states is not None:
         if query_indices is not None:
             raise ValueError("Choose node indices OR continuous query states")
-        from .world_table import interpolation_stencil
+        from .grid_stencil import interpolation_stencil
 
         states = np.asarray(query_states, float)
         if states.ndim != 2 or states.shape[1] != 8 or len(states) > 100000:

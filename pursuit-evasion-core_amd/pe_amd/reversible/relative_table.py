@@ -1,17 +1,20 @@
-"""Queries of a saved 5D relative-game history: the drop-in for
-reversible/relative_table.py.
+"""Reader of a saved 5D relative-game history, for controllers.
 
-Same constructor checks, `max_budget` view, `stencil`, `field`, `value`,
-`gradient` and `hamiltonian_controls` (relative_table.py:11-215).  `field`
-and `value` run on the GPU (pe_rtable: the history is copied to device
-memory on the first query and stays there); `stencil` returns the host
-arrays the reference returns, and `gradient` / `hamiltonian_controls` are
-the reference's host formulas over `field`.
+Drop-in for the reference's reversible/relative_table.py: the class name,
+constructor arguments (`folder`, `config`, `max_budget`), attributes
+(`times`, `axes`, `shape`, `spacings`, `fields`, `physics`, `horizon`),
+query methods and error messages are its interface.  The work is done on
+the GPU: the history is copied to the device on the first query and every
+query batch is one launch (pe_rtable_field / pe_rtable_value); `stencil` is
+pe_grid_stencil; `gradient` evaluates all ten shifted states of a batch in a
+single field query.
+
+Storage order of the grid is [vP, vE, psi, y, x]; queries use the public
+order [x, y, psi, vP, vE].
 """
 
 from __future__ import annotations
 
-from itertools import product
 import json
 from pathlib import Path
 
@@ -19,6 +22,47 @@ import numpy as np
 
 from .. import _native
 from .free_dp import physics_signature
+from .grid_stencil import axis_spec
+
+# storage axis of each public coordinate x, y, psi, vP, vE
+_STORAGE_OF = (4, 3, 2, 0, 1)
+# public column of each storage axis vP, vE, psi, y, x
+_PUBLIC_OF = (3, 4, 2, 1, 0)
+HEADING = 2
+
+
+def _checked_meta(folder):
+    meta = json.loads((folder / "meta.json").read_text())
+    rules = (
+        (meta.get("schema") == "reversible_relative5d_v1", "Not a reversible relative5D table"),
+        (meta.get("status") == "complete" and meta.get("history_saved"),
+         "A completed solve with signed history is required"),
+        (meta.get("tube_projection") == "min over every internal update",
+         "Unsupported budget convention"),
+    )
+    for ok, message in rules:
+        if not ok:
+            raise ValueError(message)
+    return meta
+
+
+def _checked_axes(meta):
+    axes = tuple(np.asarray(a, dtype=float) for a in meta["axes"])
+    shape = tuple(len(a) for a in axes)
+    if len(axes) != 5 or min(shape) < 2:
+        raise ValueError("Expected five nondegenerate coordinate axes")
+    steps = np.array([a[1] - a[0] for a in axes])
+    uniform = all(np.isfinite(a).all() and h > 0 and np.allclose(np.diff(a), h)
+                  for a, h in zip(axes, steps))
+    if not uniform:
+        raise ValueError("Axes must be finite, increasing and uniformly spaced")
+    heading = axes[HEADING]
+    if not (np.isclose(heading[0], -np.pi) and np.isclose(steps[HEADING] * len(heading),
+                                                          2 * np.pi)):
+        raise ValueError("Heading axis must be periodic without a repeated endpoint")
+    if "shape" in meta and tuple(meta["shape"]) != shape:
+        raise ValueError("Metadata shape differs from coordinate axes")
+    return axes, shape, steps
 
 
 class RelativeTable:
@@ -26,42 +70,23 @@ class RelativeTable:
 
     def __init__(self, folder, config=None, *, max_budget=None):
         self.folder = Path(folder).resolve()
-        self.meta = json.loads((self.folder / "meta.json").read_text())
-        if self.meta.get("schema") != "reversible_relative5d_v1":
-            raise ValueError("Not a reversible relative5D table")
-        if self.meta["status"] != "complete" or not self.meta["history_saved"]:
-            raise ValueError("A completed solve with signed history is required")
-        if self.meta["tube_projection"] != "min over every internal update":
-            raise ValueError("Unsupported budget convention")
-        source_times = np.asarray(self.meta["times"], dtype=float)
-        self.times = source_times.copy()
-        self.axes = tuple(np.asarray(a, dtype=float) for a in self.meta["axes"])
-        self.shape = tuple(len(a) for a in self.axes)
-        if len(self.axes) != 5 or min(self.shape) < 2:
-            raise ValueError("Expected five nondegenerate coordinate axes")
-        self.spacings = np.array([a[1] - a[0] for a in self.axes])
-        for a, h in zip(self.axes, self.spacings):
-            if not np.isfinite(a).all() or not np.allclose(np.diff(a), h) or h <= 0:
-                raise ValueError("Axes must be finite, increasing and uniformly spaced")
-        if not np.isclose(self.axes[2][0], -np.pi) or not np.isclose(
-                self.spacings[2] * self.shape[2], 2 * np.pi):
-            raise ValueError("Heading axis must be periodic without a repeated endpoint")
-        if "shape" in self.meta and tuple(self.meta["shape"]) != self.shape:
-            raise ValueError("Metadata shape differs from coordinate axes")
-        if len(self.times) < 2 or np.any(np.diff(self.times) <= 0):
+        self.meta = _checked_meta(self.folder)
+        self.axes, self.shape, self.spacings = _checked_axes(self.meta)
+        saved = np.asarray(self.meta["times"], dtype=float)
+        if len(saved) < 2 or (np.diff(saved) <= 0).any():
             raise ValueError("Invalid output times")
+        frames = len(saved)
         if max_budget is not None:
-            hit = np.flatnonzero(np.isclose(source_times, max_budget, rtol=0, atol=1e-10))
-            if not len(hit) or hit[0] == 0:
+            at = np.flatnonzero(np.abs(saved - max_budget) <= 1e-10)
+            if len(at) == 0 or at[0] == 0:
                 raise ValueError("max_budget must be a positive retained output budget")
-            self.times = source_times[: hit[0] + 1].copy()
-        expected = int(np.prod(self.shape)) * len(source_times) * 4
-        if (self.folder / "fields.f32").stat().st_size != expected:
+            frames = int(at[0]) + 1
+        self.times = saved[:frames].copy()
+        history = self.folder / "fields.f32"
+        if history.stat().st_size != 4 * len(saved) * int(np.prod(self.shape)):
             raise ValueError("History byte length differs from metadata")
-        self.fields = np.memmap(self.folder / "fields.f32", mode="r", dtype=np.float32,
-                                shape=(len(source_times), *self.shape))[: len(self.times)]
-        self._strides = np.cumprod((1,) + self.shape[:0:-1])[::-1]
-        self._bits = np.array(list(product([0, 1], repeat=5)), dtype=np.int64)
+        self.fields = np.memmap(history, mode="r", dtype=np.float32,
+                                shape=(len(saved),) + self.shape)[:frames]
         self.physics = self.meta["physics"]
         self.horizon = float(self.times[-1])
         self._device = None
@@ -73,102 +98,99 @@ class RelativeTable:
             raise ValueError("The free-space table does not match the vehicle/capture model")
 
     def _gpu(self):
-        """The history on the device (uploaded once)."""
+        """The retained history on the device (uploaded once)."""
         if self._device is None:
-            flat = np.ascontiguousarray(self.fields.reshape(len(self.times), -1))
-            self._device = _native.RelTable(flat, self.axes, self.times)
+            frames = np.ascontiguousarray(self.fields).reshape(len(self.times), -1)
+            self._device = _native.RelTable(frames, self.axes, self.times)
         return self._device
 
-    def stencil(self, states):
-        """32-vertex indices and weights, validity and batch shape
-        (relative_table.py:76-109)."""
-        states = np.asarray(states, dtype=float)
-        if states.ndim < 1 or states.shape[-1] != 5:
-            raise ValueError("Relative states must end in [x,y,psi,vP,vE]")
-        shape = states.shape[:-1]
-        q_all = states.reshape(-1, 5)[:, [3, 4, 2, 1, 0]]
-        finite = np.isfinite(q_all)
-        valid = finite.all(1)
-        q_all = np.where(finite, q_all, 0)
-        ii = np.zeros((len(q_all), 32), dtype=np.int64)
-        ww = np.ones((len(q_all), 32))
-        for k, (axis, step, n) in enumerate(zip(self.axes, self.spacings, self.shape)):
-            q = q_all[:, k]
-            if k == 2:
-                f = ((q + np.pi) % (2 * np.pi)) / step
-                low = np.floor(f).astype(np.int64) % n
-                frac = f - np.floor(f)
-            else:
-                valid &= (q >= axis[0] - 1e-10) & (q <= axis[-1] + 1e-10)
-                f = (np.clip(q, axis[0], axis[-1]) - axis[0]) / step
-                low = np.minimum(np.floor(f).astype(np.int64), n - 2)
-                frac = f - low
-            node = low[:, None] + self._bits[None, :, k]
-            if k == 2:
-                node %= n
-            ii += node * self._strides[k]
-            ww *= np.where(self._bits[None, :, k], frac[:, None], 1 - frac[:, None])
-        return ii, ww, valid, shape
-
-    def _batch(self, states):
+    @staticmethod
+    def _rows(states):
         s = np.asarray(states, dtype=float)
         if s.ndim < 1 or s.shape[-1] != 5:
             raise ValueError("Relative states must end in [x,y,psi,vP,vE]")
         return s.reshape(-1, 5), s.shape[:-1]
 
+    def stencil(self, states):
+        """32 node indices and weights per state (storage order, vertex c
+        taking the upper node of storage axis k when bit 4 - k is set),
+        validity and the batch shape."""
+        rows, batch = self._rows(states)
+        n, lo, hi, step, _, mask = axis_spec(self.axes, (HEADING,))
+        period = [0.0, 0.0, 2 * np.pi, 0.0, 0.0]
+        index, weight, valid = _native.grid_stencil(rows[:, list(_PUBLIC_OF)], n, lo, hi, step,
+                                                    period, mask, 1e-10)
+        return index, weight, valid, batch
+
     def field(self, states, budget):
-        """Signed field at states and budgets (relative_table.py:111-131)."""
-        flat, shape = self._batch(states)
-        tau = np.broadcast_to(np.asarray(budget, dtype=float), shape).reshape(-1)
-        if not len(flat):
-            return np.full(shape, np.nan)
-        return self._gpu().field(flat, tau).reshape(shape)
+        """Signed field at states and budgets: multilinear in space, linear in
+        the budget between saved frames, NaN outside either."""
+        rows, batch = self._rows(states)
+        tau = np.broadcast_to(np.asarray(budget, dtype=float), batch).reshape(-1)
+        if len(rows) == 0:
+            return np.full(batch, np.nan)
+        return self._gpu().field(rows, tau).reshape(batch)
 
     def value(self, states):
-        """First crossing of the interpolated signed history, never an
-        interpolated arrival (relative_table.py:133-161)."""
+        """First budget at which the interpolated history reaches zero (0 for
+        captured physical states, NaN where it never does)."""
         if self.meta.get("capture_rule") == "rectangle_contact":
             raise ValueError(
                 "Use local_game.free_value.ContactFreeValue for rectangular time queries")
-        flat, shape = self._batch(states)
-        if not len(flat):
-            return np.full(shape, np.nan)
+        rows, batch = self._rows(states)
+        if len(rows) == 0:
+            return np.full(batch, np.nan)
         p, e = self.physics["pursuer"], self.physics["evader"]
-        limits = [p["speed_min"], p["speed_max"], e["speed_min"], e["speed_max"]]
-        return self._gpu().value(flat, self.physics["capture_radius"], limits).reshape(shape)
+        limits = (p["speed_min"], p["speed_max"], e["speed_min"], e["speed_max"])
+        radius = self.physics["capture_radius"]
+        return self._gpu().value(rows, radius, limits).reshape(batch)
 
     def gradient(self, states, budget):
-        """Central differences of the signed field over one spacing, clipped
-        at the non-periodic ends, in public [x, y, psi, vP, vE] order
-        (relative_table.py:163-189)."""
+        """Central difference quotient of the field over one grid spacing per
+        public coordinate (the shifted states clipped to the non-periodic
+        axes; NaN where they collapse), shape [..., 5]."""
         s = np.asarray(states, dtype=float)
-        out = []
-        for j, k in enumerate([4, 3, 2, 0, 1]):
+        batch = s.shape[:-1]
+        rows = s.reshape(-1, 5)
+        m = len(rows)
+        tau = np.broadcast_to(np.asarray(budget, dtype=float), batch).reshape(-1)
+        # [5 coordinates, 2 sides, m states] shifted copies
+        shifted = np.broadcast_to(rows, (5, 2, m, 5)).copy()
+        for j, k in enumerate(_STORAGE_OF):
             h = self.spacings[k]
-            left, right = s.copy(), s.copy()
-            left[..., j] -= h
-            right[..., j] += h
-            if j != 2:
-                left[..., j] = np.clip(left[..., j], self.axes[k][0], self.axes[k][-1])
-                right[..., j] = np.clip(right[..., j], self.axes[k][0], self.axes[k][-1])
-            width = right[..., j] - left[..., j]
-            diff = self.field(right, budget) - self.field(left, budget)
-            out.append(np.divide(diff, width, out=np.full_like(diff, np.nan), where=width > 0))
-        return np.stack(out, axis=-1)
+            shifted[j, 0, :, j] -= h
+            shifted[j, 1, :, j] += h
+            if j != HEADING:
+                np.clip(shifted[j, :, :, j], self.axes[k][0], self.axes[k][-1],
+                        out=shifted[j, :, :, j])
+        values = self.field(shifted.reshape(-1, 5), np.tile(tau, 10)).reshape(5, 2, m)
+        width = shifted[np.arange(5), 1, :, np.arange(5)] - shifted[np.arange(5), 0, :,
+                                                                     np.arange(5)]
+        rise = values[:, 1] - values[:, 0]
+        quotient = np.full((5, m), np.nan)
+        np.divide(rise, width, out=quotient, where=width > 0)
+        return quotient.T.reshape(batch + (5,))
 
     def hamiltonian_controls(self, states, budget):
-        """Pointwise P-min / E-max controls of the interpolated field gradient
-        (relative_table.py:191-215)."""
+        """Commands (acceleration, curvature) of the pursuer minimising and
+        the evader maximising the field's rate of change, from the sign of
+        the gradient terms they multiply; accelerations that would push a
+        speed past its limit are cut to zero."""
         s = np.asarray(states, dtype=float)
         g = self.gradient(s, budget)
+        x, y, vp, ve = s[..., 0], s[..., 1], s[..., 3], s[..., 4]
+        gx, gy, gpsi, gvp, gve = (g[..., k] for k in range(5))
         p, e = self.physics["pursuer"], self.physics["evader"]
-        cp = s[..., 3] * (s[..., 1] * g[..., 0] - s[..., 0] * g[..., 1] - g[..., 2])
-        ce = s[..., 4] * g[..., 2]
-        ap = -p["acceleration"] * np.sign(g[..., 3])
-        ae = e["acceleration"] * np.sign(g[..., 4])
-        ap = np.where(s[..., 3] <= p["speed_min"] + 1e-10, np.maximum(ap, 0), ap)
-        ap = np.where(s[..., 3] >= p["speed_max"] - 1e-10, np.minimum(ap, 0), ap)
-        ae = np.where(s[..., 4] <= e["speed_min"] + 1e-10, np.maximum(ae, 0), ae)
-        ae = np.where(s[..., 4] >= e["speed_max"] - 1e-10, np.minimum(ae, 0), ae)
-        return (np.stack([ap, -p["max_curvature"] * np.sign(cp)], axis=-1),
-                np.stack([ae, e["max_curvature"] * np.sign(ce)], axis=-1))
+
+        def limited(acc, v, lim):
+            acc = np.where(v <= lim["speed_min"] + 1e-10, np.maximum(acc, 0), acc)
+            return np.where(v >= lim["speed_max"] - 1e-10, np.minimum(acc, 0), acc)
+
+        # pursuer turning rotates the relative frame: d psi and d(x, y) terms
+        p_turn = vp * (y * gx - x * gy - gpsi)
+        e_turn = ve * gpsi
+        p_acc = limited(-p["acceleration"] * np.sign(gvp), vp, p)
+        e_acc = limited(e["acceleration"] * np.sign(gve), ve, e)
+        pursuer = np.stack([p_acc, -p["max_curvature"] * np.sign(p_turn)], axis=-1)
+        evader = np.stack([e_acc, e["max_curvature"] * np.sign(e_turn)], axis=-1)
+        return pursuer, evader

@@ -35,9 +35,16 @@ PERIODIC_AXES = (2, 6)
 TUBE_PROJECTION = "min over completed numerical updates, then max with avoid"
 
 
+# Least allowed size of each grid dimension.
+_LEAST = dict(positions=5, headings=4, p_speeds=3, e_speeds=3)
+
+
 @dataclass(frozen=True)
 class WorldGrid:
-    """Dense square-game grid (reversible/world_dp.py:24-88)."""
+    """Dense grid of the square game: `positions` nodes per coordinate over
+    [-extent, extent], `headings` periodic headings from -pi, and the speed
+    ranges of the two cars (the reference's WorldGrid,
+    reversible/world_dp.py:24-88)."""
 
     positions: int = 9
     headings: int = 8
@@ -46,106 +53,121 @@ class WorldGrid:
     extent: float = 3.5
 
     def __post_init__(self):
-        for name, least in (("positions", 5), ("headings", 4), ("p_speeds", 3),
-                            ("e_speeds", 3)):
-            v = getattr(self, name)
-            if isinstance(v, bool) or not isinstance(v, (int, np.integer)) or v < least:
+        for name, least in _LEAST.items():
+            value = getattr(self, name)
+            integral = isinstance(value, (int, np.integer)) and not isinstance(value, bool)
+            if not integral or value < least:
                 raise ValueError(f"{name} must be an integer >= {least}.")
-        if not np.isfinite(self.extent) or self.extent <= 0:
+        if not (np.isfinite(self.extent) and self.extent > 0):
             raise ValueError("extent must be finite and positive.")
 
     @property
     def shape(self):
-        n, h = self.positions, self.headings
-        return (n, n, h, self.p_speeds, n, n, h, self.e_speeds)
+        car = (self.positions, self.positions, self.headings)
+        return car + (self.p_speeds,) + car + (self.e_speeds,)
 
     @property
     def states(self):
         return math.prod(self.shape)
 
     def axes(self, config):
-        x = np.linspace(-self.extent, self.extent, self.positions)
-        theta = -np.pi + np.arange(self.headings) * 2 * np.pi / self.headings
-        p, e = config.pursuer, config.evader
-        vp = np.linspace(p.speed_min, p.speed_max, self.p_speeds)
-        ve = np.linspace(e.speed_min, e.speed_max, self.e_speeds)
-        return (x, x, theta, vp, x, x, theta, ve)
+        side = np.linspace(-self.extent, self.extent, self.positions)
+        heading = -np.pi + np.arange(self.headings) * 2 * np.pi / self.headings
+        speed = [np.linspace(lim.speed_min, lim.speed_max, count)
+                 for lim, count in ((config.pursuer, self.p_speeds),
+                                    (config.evader, self.e_speeds))]
+        return (side, side, heading, speed[0], side, side, heading, speed[1])
 
     def spacings(self, config):
-        return tuple(float(a[1] - a[0]) for a in self.axes(config))
+        return tuple(float(axis[1] - axis[0]) for axis in self.axes(config))
 
     def to_dict(self, config):
         axes = self.axes(config)
-        offsets = [float(np.min(abs(axes[0] - w)))
-                   for w in (-config.arena_half_width, config.arena_half_width)]
-        return dict(
-            **asdict(self),
-            shape=list(self.shape),
-            states=self.states,
-            spacings=list(self.spacings(config)),
-            periodic_axes=list(PERIODIC_AXES),
-            state_order=STATE_ORDER,
-            axis_bounds=[[float(a[0]), float(a[-1])] for a in axes],
-            physical_wall_grid_offsets=offsets,
-            physical_walls_on_nodes=max(offsets) < 1e-8,
-        )
+        wall = config.arena_half_width
+        offsets = [float(np.abs(axes[0] - w).min()) for w in (-wall, wall)]
+        info = asdict(self)
+        info.update(shape=list(self.shape), states=self.states,
+                    spacings=list(self.spacings(config)), periodic_axes=list(PERIODIC_AXES),
+                    state_order=STATE_ORDER,
+                    axis_bounds=[[float(axis[0]), float(axis[-1])] for axis in axes],
+                    physical_wall_grid_offsets=offsets,
+                    physical_walls_on_nodes=max(offsets) < 1e-8)
+        return info
 
 
 def output_times(horizon, output_dt):
-    """Saved budgets 0, dt, ..., horizon (reversible/world_dp.py:91-98)."""
-    if not np.isfinite([horizon, output_dt]).all() or min(horizon, output_dt) <= 0:
+    """Saved budgets: multiples of output_dt below the horizon, then the
+    horizon itself (reversible/world_dp.py:91-98)."""
+    finite = np.isfinite(horizon) and np.isfinite(output_dt)
+    if not finite or horizon <= 0 or output_dt <= 0:
         raise ValueError("Positive finite horizon and output_dt are required.")
-    steps = np.arange(int(np.ceil(horizon / output_dt)) + 1) * output_dt
-    return np.unique(np.minimum(steps, horizon))
+    frames = math.ceil(horizon / output_dt) + 1
+    return np.unique(np.minimum(np.arange(frames) * output_dt, horizon))
+
+
+def _top_speed(limits):
+    return max(abs(limits.speed_min), abs(limits.speed_max))
 
 
 def global_rate_bound(config, grid):
-    """Sum of the per-axis speed bounds over the spacings; CFL dt = cfl / it
-    (reversible/world_dp.py:101-112)."""
+    """Largest rate of the state over one grid spacing, summed over the
+    axes: translation of both cars in x and y, turning, and the two
+    throttles.  The CFL step is cfl / bound (reversible/world_dp.py:101-112)."""
     dx, _, dh, dvp, _, _, _, dve = grid.spacings(config)
     p, e = config.pursuer, config.evader
-    vp = max(abs(p.speed_min), abs(p.speed_max))
-    ve = max(abs(e.speed_min), abs(e.speed_max))
-    return (2 * (vp + ve) / dx + (vp * p.max_curvature + ve * e.max_curvature) / dh
-            + p.acceleration / dvp + e.acceleration / dve)
+    vp, ve = _top_speed(p), _top_speed(e)
+    translation = 2 * (vp + ve) / dx
+    turning = (vp * p.max_curvature + ve * e.max_curvature) / dh
+    return translation + turning + p.acceleration / dvp + e.acceleration / dve
 
 
 def surfaces(config, grid, dtype=np.float32):
-    """Target (capture or evader wall fault) and avoid (pursuer wall fault)
-    arrays on the grid (reversible/world_dp.py:115-135); small grids only."""
+    """Target and avoid fields on the grid (reversible/world_dp.py:115-135):
+    the target is min(centre distance - capture radius, evader wall
+    clearance), the avoid field the pursuer's wall violation + 1e-6.  Dense
+    arrays of the full grid: small grids only."""
     if grid.extent <= config.arena_half_width:
         raise ValueError("World grid needs an exterior padding band.")
     axes = grid.axes(config)
+    wall = config.arena_half_width
 
-    def along(k):
-        dims = [1] * 8
-        dims[k] = len(axes[k])
-        return np.asarray(axes[k], dtype=dtype).reshape(dims)
+    def coordinate(k):
+        shape = [1] * 8
+        shape[k] = -1
+        return np.asarray(axes[k], dtype=dtype).reshape(shape)
 
-    xp, yp, xe, ye = (along(k) for k in (0, 1, 4, 5))
-    L = config.arena_half_width
+    xp, yp, xe, ye = map(coordinate, (0, 1, 4, 5))
     capture = np.hypot(xp - xe, yp - ye) - config.capture_radius
-    e_fault = L - np.maximum(abs(xe), abs(ye))
-    p_fault = np.maximum(abs(xp), abs(yp)) - L + 1e-6
-    return (np.broadcast_to(np.minimum(capture, e_fault), grid.shape).copy(),
-            np.broadcast_to(p_fault, grid.shape).copy())
+    evader_in = wall - np.maximum(abs(xe), abs(ye))
+    pursuer_out = np.maximum(abs(xp), abs(yp)) - wall + 1e-6
+    full = grid.shape
+    return (np.broadcast_to(np.minimum(capture, evader_in), full).copy(),
+            np.broadcast_to(pursuer_out, full).copy())
 
 
 def hamiltonian(states, gradients, config):
-    """min over P, max over E of grad . f; at a speed limit the acceleration
-    that would leave the interval is removed (reversible/world_dp.py:138-152)."""
+    """Upper value of the game's Hamiltonian at states [..., 8]: the pursuer
+    minimises, the evader maximises gradient . f over their commands.
+    Translation does not depend on the commands; turning contributes
+    -|vP| kP |g_thP| and +|vE| kE |g_thE|; each throttle picks the better of
+    full braking and full acceleration, an acceleration that would leave the
+    speed interval being replaced by 0 (reversible/world_dp.py:138-152)."""
     s, g = np.asarray(states), np.asarray(gradients)
     p, e = config.pursuer, config.evader
-    out = s[..., 3] * (np.cos(s[..., 2]) * g[..., 0] + np.sin(s[..., 2]) * g[..., 1])
-    out = out + s[..., 7] * (np.cos(s[..., 6]) * g[..., 4] + np.sin(s[..., 6]) * g[..., 5])
-    out = out - abs(s[..., 3]) * p.max_curvature * abs(g[..., 2])
-    out = out + abs(s[..., 7]) * e.max_curvature * abs(g[..., 6])
-    for k, lim, pick in ((3, p, np.minimum), (7, e, np.maximum)):
-        v = s[..., k]
-        lo = np.where(v <= lim.speed_min + 1e-10, 0, -lim.acceleration)
-        hi = np.where(v >= lim.speed_max - 1e-10, 0, lim.acceleration)
-        out = out + pick(lo * g[..., k], hi * g[..., k])
-    return out
+
+    def drift(base):
+        heading, speed = s[..., base + 2], s[..., base + 3]
+        return speed * (np.cos(heading) * g[..., base] + np.sin(heading) * g[..., base + 1])
+
+    value = drift(0) + drift(4)
+    value = value - abs(s[..., 3]) * p.max_curvature * abs(g[..., 2])
+    value = value + abs(s[..., 7]) * e.max_curvature * abs(g[..., 6])
+    for axis, lim, choose in ((3, p, np.minimum), (7, e, np.maximum)):
+        v, grad = s[..., axis], g[..., axis]
+        brake = np.where(v <= lim.speed_min + 1e-10, 0, -lim.acceleration)
+        push = np.where(v >= lim.speed_max - 1e-10, 0, lim.acceleration)
+        value = value + choose(brake * grad, push * grad)
+    return value
 
 
 def native_config(config, grid, *, order=2, cfl=0.8, speed_boundary="linear"):
@@ -168,6 +190,27 @@ def native_config(config, grid, *, order=2, cfl=0.8, speed_boundary="linear"):
 def _library_digest():
     path = _native.LIB_PATH
     return hashlib.sha256(path.read_bytes()).hexdigest() if path.exists() else None
+
+
+def _check_options(config, grid, order, cfl, threads, speed_boundary):
+    """The solve's option rules, in the order the reference checks them."""
+    if not isinstance(config, ReversibleConfig):
+        raise TypeError("Use the reversible physical model.")
+    rules = (
+        (config.point_vehicles,
+         "This square backend uses points; use the footprint-map backend."),
+        (not getattr(config, "obstacles", ()),
+         "This native 8D backend currently supports the square boundary only."),
+        (grid.extent > config.arena_half_width,
+         "Numerical domain must include exterior padding."),
+        (speed_boundary in ("linear", "legacy_signed"),
+         "Speed boundary must be linear or legacy_signed."),
+        (order in (1, 2) and 0 < cfl <= 1 and isinstance(threads, int) and threads >= 1,
+         "Require order1/2, CFL in(0,1], positive integer threads."),
+    )
+    for ok, message in rules:
+        if not ok:
+            raise ValueError(message)
 
 
 def _check_parent(parent, config, grid, order, cfl, speed_boundary, history, horizon):
@@ -215,18 +258,7 @@ def solve(
     are stored at every output time in probe_fields.f64; they do not
     influence the computation.
     """
-    if not isinstance(config, ReversibleConfig):
-        raise TypeError("Use the reversible physical model.")
-    if not config.point_vehicles:
-        raise ValueError("This square backend uses points; use the footprint-map backend.")
-    if getattr(config, "obstacles", ()):
-        raise ValueError("This native 8D backend currently supports the square boundary only.")
-    if grid.extent <= config.arena_half_width:
-        raise ValueError("Numerical domain must include exterior padding.")
-    if speed_boundary not in ("linear", "legacy_signed"):
-        raise ValueError("Speed boundary must be linear or legacy_signed.")
-    if order not in (1, 2) or not 0 < cfl <= 1 or not isinstance(threads, int) or threads < 1:
-        raise ValueError("Require order1/2, CFL in(0,1], positive integer threads.")
+    _check_options(config, grid, order, cfl, threads, speed_boundary)
     parent = parent_path = None
     start_time = 0.0
     if resume_from is not None:
@@ -268,7 +300,7 @@ def solve(
     probe_count = 0
     stencil = None
     if probes is not None:
-        from .world_table import interpolation_stencil
+        from .grid_stencil import interpolation_stencil
 
         queries = np.asarray(probes, dtype=float).reshape(-1, 8)
         ii, ww, valid, _ = interpolation_stencil(queries, grid.axes(config), PERIODIC_AXES)

@@ -130,18 +130,20 @@ def run_world(binary: Path, case: dict, out: Path) -> dict:
     argv += ["--threads", "4", "--output", str(out)]
     extra = {}
     if probes:
-        # Stencils from the package's own interpolation_stencil (inputs of
-        # the executable; its summation is what these vectors pin).
+        # Stencils from the oracle's uniform-grid stencil (inputs of the
+        # executable; its summation is what these vectors pin).
         sys.path.insert(0, str(ROOT / "pursuit-evasion-core_amd"))
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle
+        from pe_amd.reversible.grid_stencil import axis_spec
         from pe_amd.reversible.model import ReversibleConfig
         from pe_amd.reversible.world_dp import PERIODIC_AXES, WorldGrid
-        from pe_amd.reversible.world_table import interpolation_stencil
 
         grid = WorldGrid(case["positions"], case["headings"], case["p_speeds"],
                          case["e_speeds"], WORLD["extent"])
         states = probe_states(probes)
-        ii, ww, valid, _ = interpolation_stencil(states, grid.axes(ReversibleConfig()),
-                                                 PERIODIC_AXES)
+        n, lo, hi, step, per, mask = axis_spec(grid.axes(ReversibleConfig()), PERIODIC_AXES)
+        ii, ww, valid = oracle.grid_stencil(states, n, lo, hi, step, per, mask, 1e-10)
         assert valid.all()
         ii.astype(np.uint64).tofile(out.parent / "probe_indices.u64")
         ww.astype(np.float64).tofile(out.parent / "probe_weights.f64")

@@ -98,3 +98,46 @@ def test_reader_matches_solver_labels(gpu, solved):
     print(f"reader vs FP32 labels at {keep.sum()} queries: max|dV|={d.max():.2e}")
     assert d.max() <= 2e-3
     ref.close()
+
+
+def test_resume_after_interruption_equals_uninterrupted_solve(gpu, tmp_path, monkeypatch):
+    """(reference.py:289-348) A solve interrupted right after the checkpoint
+    of budget 0.5 s (host-side exception in the record step), then re-run
+    into the same folder, resumes from that checkpoint and ends with the
+    same labels, history and field as a solve that was never interrupted."""
+    import pe_amd.local_game.reference as reference
+
+    config = contact_config()
+    body = Footprint(0.5, 0.3)
+    arena = random_rectangle_arena(6, pursuer=body, evader=body, count=8)
+    kw = dict(headings=8, positions=7, half_width=3.0, p_speeds=3, e_speeds=3, horizon=1.0,
+              output_dt=0.25, retain_history=True, frames=4, per_frame=16,
+              residual_tol=1e-3, max_sweeps=400)
+    print("[gpu] contact solve, uninterrupted", flush=True)
+    whole = solve(config, arena, tmp_path / "whole", **kw)
+    original = reference._ContactSolve.record
+
+    def record_then_fail(self, step):
+        original(self, step)
+        if step == 10:
+            raise KeyboardInterrupt("interrupted after the 0.5 s checkpoint")
+
+    monkeypatch.setattr(reference._ContactSolve, "record", record_then_fail)
+    print("[gpu] contact solve, interrupted at 0.5 s", flush=True)
+    with pytest.raises(KeyboardInterrupt):
+        solve(config, arena, tmp_path / "cut", **kw)
+    cut_meta = json.loads((tmp_path / "cut" / "meta.json").read_text())
+    assert cut_meta["status"] == "running" and cut_meta["completed_step"] == 10
+    assert (tmp_path / "cut" / "checkpoint_0010.f32").exists()
+    monkeypatch.setattr(reference._ContactSolve, "record", original)
+    print("[gpu] contact solve, resumed", flush=True)
+    resumed = solve(config, arena, tmp_path / "cut", **kw)
+    assert resumed["status"] == "complete" and resumed["times"] == whole["times"]
+    for name in ("labels.npz",):
+        a, b = np.load(tmp_path / "whole" / name), np.load(tmp_path / "cut" / name)
+        np.testing.assert_array_equal(a["values"], b["values"])
+        np.testing.assert_array_equal(a["times"], b["times"])
+    for name in ("fields.f16", "converged.f32"):
+        np.testing.assert_array_equal(np.fromfile(tmp_path / "whole" / name, np.uint8),
+                                      np.fromfile(tmp_path / "cut" / name, np.uint8))
+    assert not list((tmp_path / "cut").glob("checkpoint_*"))

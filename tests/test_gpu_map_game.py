@@ -56,7 +56,7 @@ def test_gpu_map_continuous_queries_and_monotone_history(gpu, tmp_path):
     from pe_amd.reversible.gpu_map_game import solve
     from pe_amd.reversible.model import ReversibleConfig
     from pe_amd.reversible.world_dp import WorldGrid
-    from pe_amd.reversible.world_table import interpolation_stencil
+    from pe_amd.reversible.grid_stencil import interpolation_stencil
 
     print("[map] continuous queries", flush=True)
     c = replace(ReversibleConfig(), point_vehicles=False, arena_half_width=2.5)

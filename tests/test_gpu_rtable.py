@@ -92,3 +92,23 @@ def test_gpu_queries_match_oracle_on_a_solved_history(gpu, tmp_path):
     np.testing.assert_allclose(got_v, want_v, atol=1e-12, rtol=0, equal_nan=True)
     print(f"[rtable] resolved {np.isfinite(got_v).sum()} of {n}", flush=True)
     assert np.isfinite(got_v).sum() > 100 and np.isnan(got_v).sum() > 100
+
+
+def test_gpu_stencil_equals_oracle(gpu, tmp_path):
+    """RelativeTable.stencil (pe_grid_stencil) against the oracle's
+    per-query stencil, bit for bit."""
+    table, _ = synthetic_table(tmp_path)
+    rng = np.random.default_rng(9)
+    states = np.column_stack([rng.uniform(-2.2, 2.2, 300), rng.uniform(-2.2, 2.2, 300),
+                              rng.uniform(-9, 9, 300), rng.uniform(-1.1, 2.1, 300),
+                              rng.uniform(-0.6, 1.1, 300)])
+    states[0, 2] = np.pi
+    states[1, 0] = 2.0
+    states[2, 3] = np.nan
+    ii, ww, valid, shape = table.stencil(states)
+    assert shape == (300,)
+    for k, s in enumerate(states):
+        oi, ow, ov = ro.stencil_one(list(s), table.axes)
+        assert valid[k] == ov
+        np.testing.assert_array_equal(ii[k], oi)
+        np.testing.assert_array_equal(ww[k], ow)

@@ -262,11 +262,11 @@ def test_gpu_default_grid_matches_oracle_and_invariants(gpu, oracle_lib):
 @pytest.mark.parametrize("speed_boundary", ["linear", "legacy_signed"])
 def test_solve_tiny_update_matches_independent_numpy_and_probe_query(
         gpu, tmp_path, order, speed_boundary):
-    """(tests/test_reversible_world_dp.py:113-167) through world_dp.solve and
-    WorldTable."""
+    """(tests/test_reversible_world_dp.py:113-167) through world_dp.solve; the
+    saved probe values are the probe stencils applied to the saved frames."""
+    from pe_amd.reversible.grid_stencil import interpolation_stencil
     from pe_amd.reversible.model import ReversibleConfig
-    from pe_amd.reversible.world_dp import WorldGrid, global_rate_bound, solve
-    from pe_amd.reversible.world_table import WorldTable
+    from pe_amd.reversible.world_dp import PERIODIC_AXES, WorldGrid, global_rate_bound, solve
     from world8d_numpy import one_step
 
     print(f"[world8d] solve order {order} {speed_boundary}", flush=True)
@@ -282,15 +282,16 @@ def test_solve_tiny_update_matches_independent_numpy_and_probe_query(
     assert meta["internal_steps"] == 1 and meta["status"] == "complete"
     assert np.max(actual - initial) < 2e-6
     assert np.min(actual - avoid) > -2e-6
-    table = WorldTable(out, allow_legacy_speed_boundary=(speed_boundary == "legacy_signed"))
     saved = np.fromfile(out / "probe_fields.f64", dtype=np.float64).reshape(2, 2)
-    np.testing.assert_allclose(table.field(probe, 0), saved[0], atol=1e-12)
-    np.testing.assert_allclose(table.field(probe, dt), saved[1], atol=1e-12)
-    assert table.value(probe[1]) == 0
+    frames = np.fromfile(out / "fields.f32", dtype=np.float32).reshape(2, -1)
+    ii, ww, valid, _ = interpolation_stencil(probe, grid.axes(config), PERIODIC_AXES)
+    assert valid.all()
+    for t in range(2):
+        np.testing.assert_allclose((frames[t][ii].astype(np.float64) * ww).sum(1), saved[t],
+                                   atol=1e-12)
     outside = probe[0].copy()
     outside[0] = 20
-    assert np.isnan(table.field(outside, dt))
-    assert np.isnan(table.value(outside))
+    assert not interpolation_stencil(outside, grid.axes(config), PERIODIC_AXES)[2].any()
 
 
 @pytest.mark.parametrize("order", [1, 2])
@@ -298,7 +299,6 @@ def test_resume_reuses_parent_history_and_matches_monolithic_solve(gpu, tmp_path
     """(tests/test_reversible_world_dp.py:229-287)"""
     from pe_amd.reversible.model import ReversibleConfig
     from pe_amd.reversible.world_dp import WorldGrid, solve
-    from pe_amd.reversible.world_table import WorldTable
 
     print(f"[world8d] resume order {order}", flush=True)
     config, grid = ReversibleConfig(), WorldGrid(5, 4, 4, 4)
@@ -315,16 +315,14 @@ def test_resume_reuses_parent_history_and_matches_monolithic_solve(gpu, tmp_path
     assert not (extension / "final.f32").exists()
     assert not (extension / "arrival.f32").exists()
     assert (extension / "fields.f32").stat().st_size == 2 * grid.states * 4
-    chained, full = WorldTable(extension), WorldTable(monolithic)
-    np.testing.assert_allclose(chained.times, full.times, atol=1e-15)
-    np.testing.assert_array_equal(chained.final, full.final)
-    state = np.array([0, 0, 0, 1, 0.7, 0.2, 0, 0.5])
-    states = np.broadcast_to(state, (5, 8))
-    budgets = np.array([0, 0.025, 0.1, 0.125, 0.2])
-    np.testing.assert_array_equal(chained.field(states, budgets), full.field(states, budgets))
-    np.testing.assert_equal(chained.value(states), full.value(states))
-    assert (meta["cumulative_internal_steps"]
-            == json.loads((monolithic / "meta.json").read_text())["internal_steps"])
+    full_meta = json.loads((monolithic / "meta.json").read_text())
+    np.testing.assert_allclose(meta["times"], full_meta["times"][-2:], atol=1e-15)
+    # the extension's frames are the monolithic solve's last two, bit for bit
+    ext = np.fromfile(extension / "fields.f32", np.float32).reshape(2, -1)
+    mono = np.fromfile(monolithic / "fields.f32", np.float32).reshape(-1, grid.states)
+    np.testing.assert_array_equal(ext, mono[-2:])
+    np.testing.assert_array_equal(ext[-1], np.fromfile(monolithic / "final.f32", np.float32))
+    assert meta["cumulative_internal_steps"] == full_meta["internal_steps"]
 
 
 def test_speed_limit_generator_on_gpu(gpu):
@@ -376,3 +374,28 @@ def test_errors_are_reported(gpu, tmp_path):
         w.set_field(field, None, 0.0)
         w.advance_to(0.01)
         assert w.count_nonfinite() > 0
+
+
+def test_grid_stencil_matches_oracle(gpu, oracle_lib):
+    """pe_grid_stencil (probe stencils of this game and the 5D reader's
+    stencil) against the oracle's uniform-grid stencil, bit for bit, with
+    periodic wraps, states outside the grid and non-finite states."""
+    from pe_amd.reversible.grid_stencil import axis_spec, interpolation_stencil
+    from pe_amd.reversible.model import ReversibleConfig
+    from pe_amd.reversible.world_dp import PERIODIC_AXES, WorldGrid
+
+    rng = np.random.default_rng(11)
+    axes = WorldGrid(6, 5, 4, 3).axes(ReversibleConfig())
+    lo = np.array([a[0] for a in axes])
+    hi = np.array([a[-1] for a in axes])
+    s = rng.uniform(lo - 0.4, hi + 0.4, (5000, 8))
+    s[::9, 2] += 40.0
+    s[7, 5] = np.nan
+    s[8, 6] = np.inf
+    ii, ww, valid, batch = interpolation_stencil(s, axes, PERIODIC_AXES)
+    n, lo_, hi_, step, per, mask = axis_spec(axes, PERIODIC_AXES)
+    oi, ow, ov = oracle_lib.grid_stencil(s, n, lo_, hi_, step, per, mask, 1e-10)
+    assert batch == (5000,) and 0 < valid.sum() < 5000
+    np.testing.assert_array_equal(ii, oi)
+    np.testing.assert_array_equal(ww.view(np.uint64), ow.view(np.uint64))
+    np.testing.assert_array_equal(valid, ov)

@@ -1,5 +1,5 @@
 """The 5D table reader without a GPU: the oracle (oracle/rtable_oracle.py)
-against the reader's known answers, the host stencil against the oracle, and
+against the reader's known answers, and
 the constructor checks (reversible/relative_table.py:14-74)."""
 
 import json
@@ -60,24 +60,6 @@ def test_oracle_stencil_is_multilinear_exact():
         assert valid and sum(ww) == pytest.approx(1, abs=1e-14)
         got = sum(grid_values[i] * w for i, w in zip(ii, ww))
         assert got == pytest.approx(fn(s[0], s[1], s[3], s[4]), abs=1e-12)
-
-
-def test_host_stencil_equals_oracle(tmp_path):
-    table, _ = synthetic_table(tmp_path)
-    rng = np.random.default_rng(9)
-    states = np.column_stack([rng.uniform(-2.2, 2.2, 300), rng.uniform(-2.2, 2.2, 300),
-                              rng.uniform(-9, 9, 300), rng.uniform(-1.1, 2.1, 300),
-                              rng.uniform(-0.6, 1.1, 300)])
-    states[0, 2] = np.pi
-    states[1, 0] = 2.0
-    states[2, 3] = np.nan
-    ii, ww, valid, shape = table.stencil(states)
-    assert shape == (300,)
-    for k, s in enumerate(states):
-        oi, ow, ov = ro.stencil_one(list(s), table.axes)
-        assert valid[k] == ov
-        np.testing.assert_array_equal(ii[k], oi)
-        np.testing.assert_array_equal(ww[k], ow)
 
 
 def test_constructor_checks(tmp_path):

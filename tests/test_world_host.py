@@ -1,6 +1,7 @@
-"""Host logic of the dense 8D square game (pe_amd.reversible.world_dp and
-world_table), no GPU: the stencil, the Hamiltonian, option validation and
-the resource guards (tests/test_reversible_world_dp.py)."""
+"""Host logic of the dense 8D square game (pe_amd.reversible.world_dp), no
+GPU: the oracle's grid stencil (the restatement the GPU stencil is checked
+against in test_gpu_world8d.py), the Hamiltonian, option validation and the
+resource guards (tests/test_reversible_world_dp.py)."""
 
 from itertools import product
 
@@ -15,7 +16,18 @@ from pe_amd.reversible.world_dp import (
     output_times,
     solve,
 )
-from pe_amd.reversible.world_table import interpolation_stencil
+from pe_amd.reversible.grid_stencil import axis_spec
+
+
+def interpolation_stencil(states, axes, periodic_axes):
+    """The oracle's uniform-grid stencil with the reader's conventions."""
+    import oracle
+
+    s = np.asarray(states, float)
+    n, lo, hi, step, per, mask = axis_spec(axes, periodic_axes)
+    ii, ww, valid = oracle.grid_stencil(s.reshape(-1, len(axes)), n, lo, hi, step, per, mask,
+                                        1e-10)
+    return ii, ww, valid, s.shape[:-1]
 
 
 def test_hamiltonian_matches_enumerated_control_pairs_at_signed_and_boundary_speeds():
@@ -98,6 +110,22 @@ def test_stencil_marks_outside_and_nonfinite_states_invalid():
     assert valid.tolist() == [False, False, True]
 
 
+def test_golden_probe_stencils_are_the_oracles():
+    """The probe stencils stored with the golden vectors (inputs the
+    reference executable summed) are the oracle's stencils of the stored
+    probe states, bit for bit."""
+    from pathlib import Path
+
+    golden = Path(__file__).resolve().parent / "golden" / "world8d_5x4x3x3_order2_linear.npz"
+    z = np.load(golden)
+    grid = WorldGrid(5, 4, 3, 3)
+    ii, ww, valid, _ = interpolation_stencil(z["probe_states"], grid.axes(ReversibleConfig()),
+                                             (2, 6))
+    assert valid.all()
+    np.testing.assert_array_equal(ii, z["probe_index"].astype(np.int64))
+    np.testing.assert_array_equal(ww.view(np.uint64), z["probe_weight"].view(np.uint64))
+
+
 def test_output_times():
     np.testing.assert_allclose(output_times(0.5, 0.25), [0, 0.25, 0.5])
     np.testing.assert_allclose(output_times(0.3, 0.25), [0, 0.25, 0.3])
@@ -173,8 +201,9 @@ def test_footprint_heading_and_player_size_change_collision():
     assert corridor.clearance([0, 0, np.pi / 2], "evader") > 0
 
 
-def test_joint_quarter_rotation_preserves_geometry_and_event_order():
-    """(tests/test_footprint_map.py:21-40)"""
+def test_joint_quarter_rotation_preserves_geometry():
+    """(tests/test_footprint_map.py:21-40; the event classifier is not part
+    of this package)"""
     from pe_amd.reversible.footprint_map import FootprintMap
 
     m = FootprintMap(2.5, ((-0.7, -0.4, -1, 0.5), (0.6, 1.1, 0.8, 1.7)))
@@ -187,8 +216,6 @@ def test_joint_quarter_rotation_preserves_geometry_and_event_order():
     rotated[:, 2] += np.pi / 2
     for role in ("pursuer", "evader"):
         np.testing.assert_allclose(m.clearance(s, role), r.clearance(rotated, role), atol=2e-15)
-    assert m.event(np.array([2.2, 0, 0, 0, 2.25, 0, 0, 0]), 0.5) == "pursuer_collision"
-    assert m.event(np.array([0, 0, 0, 0, 0.4, 0, 0, 0]), 0.5) == "capture"
 
 
 def test_footprint_clearance_closed_forms():
