@@ -30,6 +30,42 @@ for p in (ROOT / "pursuit-evasion-core_amd", ROOT / "oracle"):
     if str(p) not in sys.path:
         sys.path.insert(0, str(p))
 
+# OpenMP placement of the CPU baselines, set before the oracle's runtime
+# starts: one thread per physical core, packed.
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
+
+def host_cores():
+    """(threads for the CPU baselines, physical cores this process may run
+    on, logical CPUs).  The threads are the physical cores, capped by
+    OMP_NUM_THREADS when the environment sets it (a GPU box grants each GPU
+    a share of its host cores that way)."""
+    cpus = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else \
+        list(range(os.cpu_count() or 1))
+    cores = set()
+    for c in cpus:
+        base = Path(f"/sys/devices/system/cpu/cpu{c}/topology")
+        try:
+            cores.add((base.joinpath("physical_package_id").read_text().strip(),
+                       base.joinpath("core_id").read_text().strip()))
+        except OSError:
+            cores.add(("cpu", str(c)))
+    physical = len(cores)
+    threads = physical
+    cap = os.environ.get("OMP_NUM_THREADS", "")
+    if cap.isdigit() and int(cap) > 0:
+        threads = min(threads, int(cap))
+    return threads, physical, len(cpus)
+
+
+def cores_note():
+    threads, physical, logical = host_cores()
+    return (f"{threads} OpenMP threads (OMP_PROC_BIND=close, OMP_PLACES=cores) on a host "
+            f"share of {physical} physical cores / {logical} logical CPUs"
+            + ("" if threads == physical else
+               f"; capped at OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS')}"))
+
 METRIC = ("node-updates/s + time-to-converge at max|ΔV|≤1e-3: "
           "MLP warm-start vs plain DP")
 HBM_PEAK_GBS = 8000.0
@@ -69,7 +105,7 @@ def cpu_baseline(cfg_native, sample_steps: int):
     """The oracle (C/OpenMP restatement, FP64 local arithmetic) on this host."""
     import oracle
 
-    threads = min(os.cpu_count() or 1, 16)
+    threads = host_cores()[0]
     ocfg = oracle.HJI5DCfg(
         cfg_native.positions, cfg_native.headings, cfg_native.p_speeds,
         cfg_native.e_speeds, cfg_native.order, threads, 0, cfg_native.extent,
@@ -88,6 +124,7 @@ def cpu_baseline(cfg_native, sample_steps: int):
         "kind": "port",
         "sample": f"{sample_steps} RK2 steps of the same 34,222,500-node grid from phi "
                   f"({seconds:.2f} s, FP64 local arithmetic, float storage, OpenMP)",
+        "host": cores_note(),
     }
 
 
@@ -96,7 +133,7 @@ def world8d_cpu_baseline(grid_dims, sample_steps: int):
     with the reference executable) timed for a few RK2 steps of the grid."""
     import oracle
 
-    threads = min(os.cpu_count() or 1, 16)
+    threads = host_cores()[0]
     n, h, np_, ne = grid_dims
     ocfg = oracle.World8DCfg(n, h, np_, ne, 2, 1, threads, 3.5, 3.0, 0.3, -1.0, 2.0, -0.5, 1.0,
                              2.0, 1.0, 1.0, 2.0, 0.8)
@@ -108,7 +145,8 @@ def world8d_cpu_baseline(grid_dims, sample_steps: int):
     seconds = time.perf_counter() - t0
     return {"value": field.size * sample_steps / seconds, "unit": "node-updates/s",
             "cores": threads, "kind": "port",
-            "sample": f"{sample_steps} RK2 steps ({seconds:.2f} s, float arithmetic, OpenMP)"}
+            "sample": f"{sample_steps} RK2 steps ({seconds:.2f} s, float arithmetic, OpenMP)",
+            "host": cores_note()}
 
 
 def run_world8d(grid_dims, steps: int, warmup: int, cpu_steps: int = 0):
@@ -199,7 +237,7 @@ def c2_cpu_baseline(p, e, tables, v0, sweeps: int):
     import oracle
     from pe_amd.local_game.grid_game import dense_inside
 
-    threads = min(os.cpu_count() or 1, 16)
+    threads = host_cores()[0]
     inside = dense_inside(p, e, tables["exit_value"])
     V = np.asarray(v0, np.float64)
     t0 = time.perf_counter()
@@ -210,7 +248,8 @@ def c2_cpu_baseline(p, e, tables, v0, sweeps: int):
     return {"value": legal * sweeps / seconds, "unit": "legal-pair updates/s",
             "cores": threads, "kind": "port",
             "sample": f"{sweeps} FP64 Jacobi sweeps of the same map from the payoff "
-                      f"({seconds:.2f} s, OpenMP)"}
+                      f"({seconds:.2f} s, OpenMP)",
+            "host": cores_note()}
 
 
 def run_c2(args):

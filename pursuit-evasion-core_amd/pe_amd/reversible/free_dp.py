@@ -40,7 +40,11 @@ from .. import _native
 
 @dataclass(frozen=True)
 class RelativeGrid:
-    """Relative-state grid (reference reversible/free_dp.py:23-72)."""
+    """Grid of the relative state (the reference's RelativeGrid,
+    reversible/free_dp.py:23-72): `positions` nodes per planar offset over
+    [-extent, extent], `headings` relative headings from -pi without the
+    repeated endpoint, and the two speed ranges.  Storage order [vP, vE, psi,
+    y, x], x fastest."""
 
     positions: int = 41
     headings: int = 24
@@ -49,17 +53,17 @@ class RelativeGrid:
     extent: float = 4.0
 
     def __post_init__(self):
-        for name in ("positions", "headings", "p_speeds", "e_speeds"):
+        least = dict(positions=3, headings=4, p_speeds=3, e_speeds=3)
+        for name, low in least.items():
             value = getattr(self, name)
-            minimum = 4 if name == "headings" else 3
-            if isinstance(value, bool) or int(value) != value or value < minimum:
+            whole = not isinstance(value, bool) and int(value) == value
+            if not whole or value < low:
                 raise ValueError(f"Invalid {name}: {value}")
-        if not np.isfinite(self.extent) or self.extent <= 0:
+        if not (np.isfinite(self.extent) and self.extent > 0):
             raise ValueError("extent must be finite and positive")
 
     @property
     def shape(self):
-        """Storage order [vP, vE, psi, y, x], x fastest."""
         return (self.p_speeds, self.e_speeds, self.headings, self.positions, self.positions)
 
     @property
@@ -67,49 +71,52 @@ class RelativeGrid:
         return math.prod(self.shape)
 
     def axes(self, config):
-        p, e = config.pursuer, config.evader
-        xs = np.linspace(-self.extent, self.extent, self.positions)
-        return (
-            np.linspace(p.speed_min, p.speed_max, self.p_speeds),
-            np.linspace(e.speed_min, e.speed_max, self.e_speeds),
-            np.linspace(-np.pi, np.pi, self.headings, endpoint=False),
-            xs,
-            xs.copy(),
-        )
+        offset = np.linspace(-self.extent, self.extent, self.positions)
+        speeds = [np.linspace(lim.speed_min, lim.speed_max, count)
+                  for lim, count in ((config.pursuer, self.p_speeds),
+                                     (config.evader, self.e_speeds))]
+        heading = np.linspace(-np.pi, np.pi, self.headings, endpoint=False)
+        return (speeds[0], speeds[1], heading, offset, offset.copy())
 
     def to_dict(self):
         return asdict(self)
 
 
 def output_times(horizon, output_dt):
-    """Saved budgets 0, dt, 2dt, ..., horizon (free_dp.py:75-77)."""
-    count = math.ceil(horizon / output_dt - 1e-10)
-    return np.minimum(np.arange(count + 1) * output_dt, horizon)
+    """Saved budgets: multiples of output_dt up to the horizon, the last
+    one clipped to it (free_dp.py:75-77)."""
+    frames = math.ceil(horizon / output_dt - 1e-10) + 1
+    return np.minimum(np.arange(frames) * output_dt, horizon)
+
+
+_LIMIT_KEYS = ("speed_min", "speed_max", "acceleration", "max_curvature")
 
 
 def physics_signature(config):
-    """Free-space PDE parameters of a configuration (free_dp.py:80-96)."""
-    keys = ("speed_min", "speed_max", "acceleration", "max_curvature")
-    out = {
-        "capture_radius": float(config.capture_radius),
-        "pursuer": {k: float(getattr(config.pursuer, k)) for k in keys},
-        "evader": {k: float(getattr(config.evader, k)) for k in keys},
-    }
+    """The parameters the free-space solution depends on: capture radius,
+    both cars' limits and a non-default capture rule (free_dp.py:80-96)."""
+    def limits(car):
+        return {key: float(getattr(car, key)) for key in _LIMIT_KEYS}
+
+    signature = dict(capture_radius=float(config.capture_radius),
+                     pursuer=limits(config.pursuer), evader=limits(config.evader))
     if config.capture_rule != "center_distance":
-        out["capture_rule"] = config.capture_rule
-    return out
+        signature["capture_rule"] = config.capture_rule
+    return signature
 
 
 def _validate(config, grid, horizon, output_dt, order, cfl, threads):
-    """Option checks of free_dp.py:99-128 (same messages and exceptions)."""
-    values = [horizon, output_dt, cfl, config.capture_radius, grid.extent]
-    for v in (config.pursuer, config.evader):
-        values += [v.speed_min, v.speed_max, v.acceleration, v.max_curvature]
-        if v.speed_min >= v.speed_max or v.acceleration <= 0 or v.max_curvature <= 0:
-            raise ValueError("Invalid vehicle limits")
-    if not np.isfinite(values).all() or horizon <= 0 or output_dt <= 0 or not 0 < cfl <= 1:
+    """Option checks with the reference's messages and exceptions
+    (free_dp.py:99-128)."""
+    cars = (config.pursuer, config.evader)
+    if any(c.speed_min >= c.speed_max or c.acceleration <= 0 or c.max_curvature <= 0
+           for c in cars):
+        raise ValueError("Invalid vehicle limits")
+    numbers = [horizon, output_dt, cfl, config.capture_radius, grid.extent]
+    numbers += [getattr(c, key) for c in cars for key in _LIMIT_KEYS]
+    if not (np.isfinite(numbers).all() and horizon > 0 and output_dt > 0 and 0 < cfl <= 1):
         raise ValueError("Invalid numeric solve options")
-    if not 0 < config.capture_radius < grid.extent or order not in (1, 2) or threads < 1:
+    if not (0 < config.capture_radius < grid.extent and order in (1, 2) and threads >= 1):
         raise ValueError("Invalid radius, order or thread count")
 
 
