@@ -90,15 +90,19 @@ def c1_config():
 
 
 def load_traffic():
-    """HBM bytes per launch of the dominant kernel, from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled), if present."""
+    """Memory-side bytes per launch of the dominant kernel from the committed
+    rocprofv3 --pmc summary: FETCH_SIZE and WRITE_SIZE scaled by a copy
+    kernel's reading (bytes the L2 requested from the memory fabric; hits
+    in the memory-side cache are not separable by any listed counter).
+    Returns (bytes, note) or (None, None)."""
     path = ROOT / "profiles" / "hji5d_blend_stage_c1_traffic.json"
     if not path.exists():
-        return None
+        return None, None
     try:
-        return float(json.loads(path.read_text())["hbm_bytes_per_launch"])
+        data = json.loads(path.read_text())
+        return float(data["hbm_bytes_per_launch"]), data.get("traffic_kind")
     except Exception:
-        return None
+        return None, None
 
 
 def cpu_baseline(cfg_native, sample_steps: int):
@@ -229,6 +233,44 @@ def run_reader_c1(cfg, n_queries: int = 1_000_000, cpu_queries: int = 20_000):
     return out
 
 
+def sl_algorithmic_bytes(pairs, cars):
+    """Algorithmic HBM bytes of one 8D sweep: each stored pair's value read
+    and written (8 B) plus both cars' successor tables read once (int32 ids
+    and float32 weights, 16 per state and command)."""
+    tables = sum(len(g.states) * g.allowed.shape[1] * 16 * 8 for g in cars)
+    return 8 * pairs + tables
+
+
+def load_counters(name):
+    """A committed rocprofv3 counter summary under profiles/ (JSON), or None."""
+    path = ROOT / "profiles" / name
+    if not path.exists():
+        return None
+    try:
+        return json.loads(path.read_text())
+    except Exception:
+        return None
+
+
+def sl_roofline(prof_stats, st0, sweep_seconds, cars, counters):
+    """Roofline object of an 8D sweep line: HBM against the algorithmic
+    bytes of the whole sweep (live timing), the E-side kernel's time, and
+    its VALU / LDS utilisation from the committed counter summary."""
+    nbytes = sl_algorithmic_bytes(st0["pairs"], cars)
+    achieved = nbytes / sweep_seconds / 1e9
+    out = {"bound": "valu",
+           "limiter": "dependent LDS gathers and FMAs of the E-side min-max (lane "
+                      "divergence from the early exit); HBM is not the bound",
+           "hbm": {"achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                   "frac": round(achieved / HBM_PEAK_GBS, 4),
+                   "algorithmic_bytes_per_sweep": nbytes},
+           "e_side_ms": prof_stats["e_ms"] / max(1, prof_stats["e_launches"]),
+           "p_side_ms": prof_stats["p_ms"] / max(1, prof_stats["p_launches"])}
+    if counters:
+        out["counters"] = counters
+    return out
+
+
 def c2_cpu_baseline(p, e, tables, v0, sweeps: int):
     """cpu_baseline leg of the C2 line: the oracle's FP64 OpenMP Jacobi
     sweep (the restatement of the reference's backup) timed for a few sweeps
@@ -254,6 +296,7 @@ def c2_cpu_baseline(p, e, tables, v0, sweeps: int):
 
 def run_c2(args):
     """8D contact game C2: sweeps of the compact min-max backup."""
+    import numpy as np
     from pe_amd import _native
     from pe_amd.local_game.grid_game import GameSolver
     from pe_amd.local_game.reference import benchmark_game
@@ -301,15 +344,54 @@ def run_c2(args):
                                          max(1, prof_r["e_launches"] - prof_w["e_launches"]),
             "residual_after": s.stats()["last_residual"],
         }
+        out["roofline"] = sl_roofline(prof, st0, wall / args.steps, (p, e),
+                                      load_counters("sl8d_e_side_c2_counters.json"))
         if args.to_residual:
             s.initialize()
             t2 = time.perf_counter()
             done, ok = s.run(args.to_residual, args.max_sweeps, 16, flags)
             out.update(to_residual=args.to_residual, sweeps_to_residual=done, converged=ok,
                        seconds_to_residual=time.perf_counter() - t2)
+            v32 = s.read()
+            # Accuracy at that point: the FP64 mode (the oracle's arithmetic)
+            # after the same sweeps.
+            with GameSolver(tables, precision=1) as s64:
+                s64.sweep(done, flags)
+                v64 = s64.read()
+            dv = np.abs(v32 - v64)
+            flips = int(((v32 <= 0) != (v64 <= 0)).sum())
+            out.update(max_abs_dv_vs_fp64=float(dv.max()), mean_abs_dv_vs_fp64=float(dv.mean()),
+                       capture_flips_vs_fp64=flips)
+        if getattr(args, "vstar_sweeps", 0):
+            out.update(time_to_vstar(s, flags, args.vstar_sweeps))
     if getattr(args, "cpu_sweeps", 0):
         out["cpu_baseline"] = c2_cpu_baseline(p, e, tables, v0, args.cpu_sweeps)
     return out
+
+
+def time_to_vstar(s, flags, cap, tol=1e-3, every=64):
+    """Sweeps and seconds from the payoff until max|V - V*| <= tol, where V*
+    is the same solver's field at residual 1e-6 (or after `cap` sweeps).  The
+    plain iterates decrease monotonically to V*, so the first check below
+    tol is the time to converge at that accuracy."""
+    import numpy as np
+
+    s.initialize()
+    done, ok = s.run(1e-6, cap, every, flags)
+    vstar = s.read()
+    res = s.stats()["last_residual"]
+    s.initialize()
+    t0 = s.stats()["update_seconds"]
+    k, hit = 0, None
+    while k < done:
+        s.sweep(every, flags)
+        k += every
+        if np.abs(s.read() - vstar).max() <= tol:
+            hit = (k, s.stats()["update_seconds"] - t0)
+            break
+    return {"vstar_sweeps": done, "vstar_residual": res, "vstar_converged_1e-6": ok,
+            "sweeps_to_1e-3_of_vstar": hit[0] if hit else None,
+            "gpu_seconds_to_1e-3_of_vstar": hit[1] if hit else None}
 
 
 def run_c4(args):
@@ -321,9 +403,9 @@ def run_c4(args):
     from pe_amd.local_game.reference import benchmark_games
 
     t0 = time.perf_counter()
-    # All maps' geometry in one launch per phase, then per-map host stencils.
-    tables = [g[4] for g in benchmark_games(range(1000, 1000 + args.maps), positions=9,
-                                            headings=8, speeds=4)]
+    # All maps' geometry in one launch per phase, then each map's stencils.
+    games = benchmark_games(range(1000, 1000 + args.maps), positions=9, headings=8, speeds=4)
+    tables = [g[4] for g in games]
     build_s = time.perf_counter() - t0
     print(f"[bench] C4 tables for {args.maps} maps ({build_s:.1f} s)", file=sys.stderr,
           flush=True)
@@ -359,6 +441,9 @@ def run_c4(args):
             "e_side_reference_order_ms": (prof_r["e_ms"] - prof["e_ms"]) /
                                          max(1, prof_r["e_launches"] - prof["e_launches"]),
         }
+        cars = [g[2] for g in games] + [g[3] for g in games]
+        out["roofline"] = sl_roofline(prof, st0, wall / args.steps, cars,
+                                      load_counters("sl8d_e_side_c4_counters.json"))
         if args.to_residual:
             import numpy as np
 
@@ -381,6 +466,48 @@ def run_c4(args):
                        sweeps_to_residual=done, converged=ok, seconds_to_residual=seconds,
                        map_sweeps_to_residual={"min": min(stops), "median": int(np.median(stops)),
                                                "max": max(stops), "total": int(sum(stops))})
+    if getattr(args, "cpu_sweeps", 0):
+        # cpu_baseline leg: the oracle's FP64 OpenMP sweep of the first two
+        # maps from the payoff, in legal-pair updates per second.
+        import numpy as np
+        import oracle
+        from pe_amd.local_game.grid_game import dense_inside
+
+        threads = host_cores()[0]
+        pairs = seconds = 0.0
+        for _, arena, p, e, _ in games[:2]:
+            inside = dense_inside(p, e, 3.0)
+            V = oracle.sl_init(p.states, e.states, inside, 3.0, arena.pursuer, arena.evader,
+                               threads=threads)
+            t = time.perf_counter()
+            for _ in range(args.cpu_sweeps):
+                V, _ = oracle.sl_sweep(V, p, e, inside, threads=threads)
+            seconds += time.perf_counter() - t
+            pairs += len(p.states) * len(e.states) * args.cpu_sweeps
+        out["cpu_baseline"] = {"value": pairs / seconds, "unit": "legal-pair updates/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"{args.cpu_sweeps} FP64 Jacobi sweeps of maps 1000 "
+                                         f"and 1001 from the payoff ({seconds:.2f} s, OpenMP)",
+                               "host": cores_note()}
+    return out
+
+
+def run_learned(args):
+    """The learned warm start against the plain DP on C4-grid maps: network
+    trained here on the GPU, held-out maps solved both ways."""
+    from pe_amd.learned.warm_start import compare_batched
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    grid = dict(positions=9, headings=8, speeds=4)
+    out = compare_batched(range(0, args.learn_train), range(100, 104),
+                          range(1000, 1000 + args.learn_test), grid=grid,
+                          steps=args.learn_steps, log=log)
+    out["workload"] = (f"learned warm start vs plain DP: network trained on maps "
+                       f"0-{args.learn_train - 1}, delta from maps 100-103, held-out maps "
+                       f"1000-{999 + args.learn_test}; C4 grid (9x9 positions x 8 headings x "
+                       "4 speeds per car), residual 1e-4 per map")
     return out
 
 
@@ -394,7 +521,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true",
                     help="skip the 8D contact-game line after the 5D headline")
-    ap.add_argument("--workload", choices=["c1", "c2", "c4"], default="c1")
+    ap.add_argument("--workload", choices=["c1", "c2", "c4", "learned"], default="c1")
     ap.add_argument("--maps", type=int, default=128)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--to-residual", type=float, default=None)
@@ -403,6 +530,13 @@ def main() -> int:
                     help="leave the 128-map C4 line out of the secondary object")
     ap.add_argument("--no-freeze", action="store_true",
                     help="C4: sweep every map until the last one converges")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="leave the learned warm-start comparison out")
+    ap.add_argument("--learn-train", type=int, default=16)
+    ap.add_argument("--learn-test", type=int, default=8)
+    ap.add_argument("--learn-steps", type=int, default=3000)
+    ap.add_argument("--vstar-sweeps", type=int, default=16384,
+                    help="C2: cap of the run to V* for the time to 1e-3 of V* (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="after the run, copy this many bytes once (counter calibration)")
     args = ap.parse_args()
@@ -414,6 +548,9 @@ def main() -> int:
     from pe_amd import _native
 
     info = _native.device_info()
+    if args.workload == "learned":
+        print(json.dumps(run_learned(args)), flush=True)
+        return 0
     if args.workload in ("c2", "c4"):
         args.steps = args.steps or 200
         out = run_c2(args) if args.workload == "c2" else run_c4(args)
@@ -445,9 +582,13 @@ def main() -> int:
         residual = st["last_residual"]
     avg_ms = [st["stage_ms"][k] / max(1, st["stage_launches"][k]) for k in range(2)]
     achieved = [STAGE_BYTES[k] * n / (avg_ms[k] * 1e-3) / 1e9 for k in range(2)]
-    traffic = load_traffic()
+    traffic, traffic_kind = load_traffic()
     roofline = {
         "bound": "hbm",
+        "limiter": "vector-memory instruction issue and load latency: the achieved "
+                   "rate is set by the stage's neighbour loads per node (16 for the "
+                   "two-plane blend stage), not by HBM bandwidth",
+        "traffic_kind": traffic_kind,
         "kernel": "hji5d_stage_fast<true,2,false,true> (RK2 blend stage, two psi planes per thread)",
         "achieved": round(achieved[1], 1),
         "peak": HBM_PEAK_GBS,
@@ -504,14 +645,20 @@ def main() -> int:
                 "last_residual": st["last_residual"]}
         print("[bench] secondary: 8D contact game (seed 0)", file=sys.stderr, flush=True)
         c2 = argparse.Namespace(seed=0, warmup=10, steps=200, to_residual=1e-4,
-                                max_sweeps=20000, cpu_sweeps=0 if args.no_cpu_baseline else 48)
+                                max_sweeps=20000, cpu_sweeps=0 if args.no_cpu_baseline else 48,
+                                vstar_sweeps=args.vstar_sweeps)
         secondary = {"hji5d_c1_to_residual": res5, "contact_game_c2": run_c2(c2)}
         if not args.no_c4:
             print("[bench] secondary: 8D contact game, 128 maps (C4)", file=sys.stderr,
                   flush=True)
             c4 = argparse.Namespace(maps=128, warmup=10, steps=100, to_residual=1e-4,
-                                    max_sweeps=20000, no_freeze=False)
+                                    max_sweeps=20000, no_freeze=False,
+                                    cpu_sweeps=0 if args.no_cpu_baseline else 4)
             secondary["contact_game_c4"] = run_c4(c4)
+        if not args.no_learned:
+            print("[bench] secondary: learned warm start vs plain DP", file=sys.stderr,
+                  flush=True)
+            secondary["learned_warm_start_c4_grid"] = run_learned(args)
         print("[bench] secondary: dense 8D square game", file=sys.stderr, flush=True)
         secondary["world8d_default"] = run_world8d(
             (9, 8, 7, 7), 40, 2, 0 if args.no_cpu_baseline else 2)

@@ -20,6 +20,12 @@
  * are kept in double, which the tests use to state the float32 error of the
  * GPU path.
  *
+ * Local exit (exit_radius > 0, the GPU variant gpu_relative.py:126-131 in its
+ * float64 mode): a node with x*x + y*y above the decimal literal "%.9e" of
+ * r^2 takes the literal "%.9e" of r and skips the update (and the step's
+ * maxima); the initial plane takes float(r) where x*x + y*y > r*r in float64
+ * (gpu_relative.py:167-169).
+ *
  * Parity pinned by: golden vectors of the compiled reference binary
  * (tests/golden), the explicit control-enumeration Euler step of
  * tests/test_reversible_free_dp.py:16-94 restated in numpy, and invariants
@@ -27,6 +33,7 @@
  */
 #include <math.h>
 #include <stdint.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <omp.h>
@@ -35,12 +42,20 @@
 
 typedef struct {
   double dx, dphi, dvp, dve, limit;
+  double exit_r2, exit_value; /* decimal literals of r^2 and r */
   size_t sy, sp, se, sv, n;
   double *sn, *cs;
 } grid_t;
 
 static void grid_setup(const pe_oracle_hji5d_cfg *c, grid_t *g) {
   const double pi = acos(-1.0);
+  if (c->exit_radius > 0) {
+    char text[64];
+    snprintf(text, sizeof text, "%.9e", c->exit_radius * c->exit_radius);
+    g->exit_r2 = strtod(text, NULL);
+    snprintf(text, sizeof text, "%.9e", c->exit_radius);
+    g->exit_value = strtod(text, NULL);
+  }
   g->dx = 2 * c->extent / (c->nx - 1);
   g->dphi = 2 * pi / c->nh;
   g->dvp = (c->vpmax - c->vpmin) / (c->nvp - 1);
@@ -155,6 +170,13 @@ static double stage(const pe_oracle_hji5d_cfg *c, const grid_t *g,
     for (int ix = 0; ix < c->nx; ix++) {
       const size_t j = row * c->nx + ix;
       const double x = -c->extent + ix * g->dx;
+      if (c->exit_radius > 0 && x * x + y * y > g->exit_r2) {
+        if (dst32)
+          dst32[j] = (float)g->exit_value;
+        else
+          dst64[j] = (double)(float)g->exit_value;
+        continue;
+      }
       double xm, xp, ym, yp, hm, hp, vm, vpl, em, ep;
       eno(src, j, 1, ix, c->nx, g->dx, 0, c->order, &xm, &xp);
       eno(src, j, g->sy, iy, c->nx, g->dx, 0, c->order, &ym, &yp);
@@ -212,7 +234,10 @@ static void initialize(const pe_oracle_hji5d_cfg *c, const grid_t *g,
     const double y = -c->extent + (row % c->nx) * g->dx;
     for (int ix = 0; ix < c->nx; ix++) {
       const size_t j = row * c->nx + ix;
-      const double phi = hypot(-c->extent + ix * g->dx, y) - c->radius;
+      const double x = -c->extent + ix * g->dx;
+      double phi = hypot(x, y) - c->radius;
+      if (c->exit_radius > 0 && x * x + y * y > c->exit_radius * c->exit_radius)
+        phi = (double)(float)c->exit_radius;
       double stored;
       if (a32) {
         a32[j] = (float)phi;

@@ -10,6 +10,9 @@ typedef struct {
   int nx, nh, nvp, nve, order, threads, store_f64;
   double extent, cfl, radius;
   double vpmin, vpmax, vemin, vemax, ap, ae, kp, ke;
+  /* > 0: the local exit clamp of the reference's GPU variant in its float64
+   * mode (reversible/gpu_relative.py:126-131, 167-169) */
+  double exit_radius;
 } pe_oracle_hji5d_cfg;
 
 /* Dense 8D world game options; the argv flags of

@@ -61,6 +61,7 @@ class HJI5DCfg(ctypes.Structure):
         ("ae", ctypes.c_double),
         ("kp", ctypes.c_double),
         ("ke", ctypes.c_double),
+        ("exit_radius", ctypes.c_double),
     ]
 
 
@@ -169,7 +170,8 @@ def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
 
 
-def hji5d_cfg(config, grid, *, order=2, cfl=0.8, threads=None, store_f64=False):
+def hji5d_cfg(config, grid, *, order=2, cfl=0.8, threads=None, store_f64=False,
+              exit_radius=0.0):
     """Options from a ReversibleConfig-like object and a RelativeGrid."""
     p, e = config.pursuer, config.evader
     return HJI5DCfg(
@@ -191,6 +193,7 @@ def hji5d_cfg(config, grid, *, order=2, cfl=0.8, threads=None, store_f64=False):
         e.acceleration,
         p.max_curvature,
         e.max_curvature,
+        exit_radius,
     )
 
 

@@ -474,10 +474,10 @@ __global__ __launch_bounds__(256) void hji5d_stage_fast(StageArgs<float> a, Hji5
       const float x = float(-g.extent) + float(ix) * dx;
       const float y = float(-g.extent) + float(iy) * dx;
       const unsigned j0 = (unsigned)(ie * (int)g.se + ih * g.nx2 + r);
-      if (g.exit_on && double(x * x + y * y) > g.exit_r2) {
+      if (g.exit_on && exit_outside_f32(g, ix, iy)) {
         // Outside the local exit radius every vP node takes the exit value
         // (gpu_relative.py:126-131).
-        const float ev = float(g.exit_value);
+        const float ev = g.exit_valuef;
         for (int iv = 0; iv < g.nvp; iv++) {
           a.dst[j0 + (unsigned)iv * (unsigned)g.sv] = ev;
           if (PAIR) a.dst[j0 + (unsigned)g.nx2 + (unsigned)iv * (unsigned)g.sv] = ev;

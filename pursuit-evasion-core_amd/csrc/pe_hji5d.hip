@@ -9,6 +9,8 @@
 // max |dst - previous| (the residual) into a per-step slot of a ring that the
 // host reads after each call, so stepping never waits on the device.
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <vector>
@@ -308,8 +310,18 @@ extern "C" int pe_hji5d_create(const pe_hji5d_config *cfg, pe_hji5d **out) {
   g.kp = c.p_curvature;
   g.ke = c.e_curvature;
   g.exit_on = c.exit_radius > 0;
-  g.exit_r2 = c.exit_radius * c.exit_radius;
-  g.exit_value = c.exit_radius;
+  {
+    // The reference formats r^2 and r with "%.9e" into its kernel source.
+    char r2[64], r1[64];
+    std::snprintf(r2, sizeof r2, "%.9e", c.exit_radius * c.exit_radius);
+    std::snprintf(r1, sizeof r1, "%.9e", c.exit_radius);
+    g.exit_r2 = std::strtod(r2, nullptr);
+    g.exit_value = std::strtod(r1, nullptr);
+    g.exit_r2f = std::strtof(r2, nullptr);
+    g.exit_valuef = std::strtof(r1, nullptr);
+    g.extentf = float(c.extent);
+    g.dxf = (2.0f * g.extentf) / float(c.positions - 1);
+  }
   // vP values per thread: by default the whole axis (fastest on C1 of the
   // chunk lengths 1, 2, 3, 5, 8 and 15).
   g.vpc = c.vp_chunk > 0 ? std::min(c.vp_chunk, g.nvp) : g.nvp;
@@ -431,7 +443,9 @@ extern "C" int pe_hji5d_create(const pe_hji5d_config *cfg, pe_hji5d **out) {
           else
             v = std::hypot(x, y) - c.radius;
           float f = float(v);
-          if (g.exit_on && x * x + y * y > g.exit_r2) f = float(c.exit_radius);
+          // numpy float64 coordinates and r * r (gpu_relative.py:167-169)
+          if (g.exit_on && x * x + y * y > c.exit_radius * c.exit_radius)
+            f = float(c.exit_radius);
           plane[((size_t)ih * nx + iy) * nx + ix] = f;
         }
     float *d_plane = nullptr;

@@ -7,6 +7,7 @@
 #include "pe_common.h"
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 namespace pe {
@@ -102,9 +103,10 @@ struct pe_mlp {
   // activations for batches of up to cap points
   long long cap = 0;
   float *raw = nullptr, *yt = nullptr, *yhat = nullptr, *dy = nullptr, *loss = nullptr;
+  float *partial = nullptr;  // split-K products and column-sum partials of a batch
   __bf16 *F = nullptr;
   std::vector<__bf16 *> Hs, dZ;  // per hidden layer
-  std::vector<float *> Zs;
+  std::vector<__bf16 *> Zs;  // pre-activations (bf16), kept for the SiLU derivative
   // dataset
   float *data_x = nullptr, *data_y = nullptr;
   long long data_n = 0;

@@ -32,9 +32,6 @@
 
 namespace pe {
 
-constexpr int kTile = 64;   // output tile edge per block
-constexpr int kStep = 32;   // K per LDS stage (one MFMA)
-constexpr int kPad = 8;     // LDS row padding (bf16) against bank conflicts
 constexpr int kRelative = 5;  // relative-frame features appended by mlp_features
 
 __device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }
@@ -58,100 +55,204 @@ struct GemmArgs {
   int M, N, K;
   int epilogue;
   float beta;          // kStoreF32: C = acc + beta * C
-  float *C;            // [M, N] float (kStoreF32 output, kBiasSilu Z out)
+  float *C;            // [M, N] float (kStoreF32 output)
   long long ldc;
   const float *bias;   // kBiasSilu
   __bf16 *H;           // kBiasSilu / kSiluGrad bf16 output [M, N]
   long long ldh;
-  const float *Z;      // kSiluGrad pre-activations [M, N]
+  __bf16 *Z;           // pre-activations [M, N] in bf16: kBiasSilu writes
+                       // them (if not null), kSiluGrad reads them
   long long ldz;
+  int kchunk;          // > 0: block z sums k in [z * kchunk, (z + 1) * kchunk)
+                       // and stores (kStoreF32, beta 0) at C + z * cstride
+  long long cstride;
 };
 
-// Loads the [64 rows x 32 k] tile X(row0 + r, k0 + c) into T[r][c].
-// X(row, k) = X[row * srow + k * sk].
-__device__ __forceinline__ void stage_tile(__bf16 (*T)[kStep + kPad], const __bf16 *X,
-                                           long long srow, long long sk, int row0, int rows,
-                                           int k0, int K, int tid) {
-  if (sk == 1) {
-    const int r = tid >> 2, c = (tid & 3) * 8;
-    const int row = row0 + r;
-    if (row < rows && k0 + c + 8 <= K && (srow % 8) == 0) {
-      *reinterpret_cast<bf16x8 *>(&T[r][c]) =
-          *reinterpret_cast<const bf16x8 *>(X + row * srow + (k0 + c));
-    } else {
-      for (int t = 0; t < 8; t++) {
-        const int k = k0 + c + t;
-        T[r][c + t] = (row < rows && k < K) ? X[row * srow + k] : to_bf16(0.f);
-      }
-    }
-  } else if (srow == 1) {
-    const int c = tid >> 3, r = (tid & 7) * 8;  // k, first row of 8
-    const int k = k0 + c;
-    if (k < K && row0 + r + 8 <= rows && (sk % 8) == 0) {
-      const bf16x8 v = *reinterpret_cast<const bf16x8 *>(X + k * sk + (row0 + r));
-      for (int t = 0; t < 8; t++) T[r + t][c] = v[t];
-    } else {
-      for (int t = 0; t < 8; t++) {
-        const int row = row0 + r + t;
-        T[r + t][c] = (row < rows && k < K) ? X[row + k * sk] : to_bf16(0.f);
-      }
-    }
-  } else {
-    const int r = tid >> 2, c = (tid & 3) * 8;
-    for (int t = 0; t < 8; t++) {
-      const int row = row0 + r, k = k0 + c + t;
-      T[r][c + t] = (row < rows && k < K) ? X[row * srow + k * sk] : to_bf16(0.f);
-    }
-  }
-}
+// One operand tile of ROWS rows x kK k staged through registers: each
+// thread loads ROWS / 32 vectors of 8 bf16 from global memory (issued before
+// the current tile's MFMAs) and writes them k-contiguous into LDS (after).
+// X(row, k) = X[row * srow + k * sk].  k-contiguous operands load 8
+// consecutive k of one row; row-contiguous ones 8 consecutive rows of one k,
+// transposed on the way into LDS.  Out-of-range rows and k read as zero.
+constexpr int kK = 64;      // K per LDS stage (two MFMA k-steps)
+constexpr int kLdsPad = 8;  // LDS row padding (bf16) against bank conflicts
 
-__global__ __launch_bounds__(256) void mfma_gemm(GemmArgs g) {
-  __shared__ __attribute__((aligned(16))) __bf16 As[kTile][kStep + kPad];
-  __shared__ __attribute__((aligned(16))) __bf16 Bs[kTile][kStep + kPad];
-  const int m0 = blockIdx.x * kTile, n0 = blockIdx.y * kTile;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;  // wave's quarter
-  f32x4 acc[2][2];
-  for (int i = 0; i < 2; i++)
-    for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < g.K; k0 += kStep) {
-    // Stage A(m0 + r, k0 + c) and B(k0 + c, n0 + r) into LDS, k-contiguous.
-    // 64 rows x 32 k = 2048 values per operand; 8 per thread: either 8
-    // consecutive k of one row (k-contiguous operand), or 8 consecutive rows
-    // of one k (row-contiguous operand, transposed on the way into LDS).
-    stage_tile(As, g.A, g.sam, g.sak, m0, g.M, k0, g.K, tid);
-    stage_tile(Bs, g.B, g.sbn, g.sbk, n0, g.N, k0, g.K, tid);
-    __syncthreads();
-    const int fr = lane & 15, fk = (lane >> 4) * 8;
-    bf16x8 a[2], b[2];
-    for (int i = 0; i < 2; i++)
-      a[i] = *reinterpret_cast<const bf16x8 *>(&As[wm + 16 * i + fr][fk]);
-    for (int j = 0; j < 2; j++)
-      b[j] = *reinterpret_cast<const bf16x8 *>(&Bs[wn + 16 * j + fr][fk]);
-    for (int i = 0; i < 2; i++)
-      for (int j = 0; j < 2; j++)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-    __syncthreads();
-  }
-  // Epilogue: lane holds D[4 * (lane / 16) + v][lane % 16] of each tile.
-  for (int i = 0; i < 2; i++)
-    for (int j = 0; j < 2; j++)
-      for (int v = 0; v < 4; v++) {
-        const int m = m0 + wm + 16 * i + 4 * (lane >> 4) + v;
-        const int n = n0 + wn + 16 * j + (lane & 15);
-        if (m >= g.M || n >= g.N) continue;
-        const float x = acc[i][j][v];
-        if (g.epilogue == kStoreF32) {
-          float *c = g.C + m * g.ldc + n;
-          *c = g.beta != 0.f ? x + g.beta * *c : x;
-        } else if (g.epilogue == kBiasSilu) {
-          const float z = x + g.bias[n];
-          if (g.C) g.C[m * g.ldc + n] = z;
-          g.H[m * g.ldh + n] = to_bf16(silu(z));
+template <int ROWS>
+struct TileLoad {
+  static constexpr int V = ROWS / 32;
+  bf16x8 v[V];
+
+  __device__ __forceinline__ void load(const __bf16 *__restrict__ X, long long srow,
+                                       long long sk, int row0, int rows, int k0, int kend,
+                                       int tid) {
+#pragma unroll
+    for (int i = 0; i < V; i++) {
+      const int q = tid + 256 * i;
+      if (sk == 1) {
+        const int row = row0 + q / (kK / 8), k = k0 + (q % (kK / 8)) * 8;
+        if (row < rows && k + 8 <= kend && (srow % 8) == 0) {
+          v[i] = *reinterpret_cast<const bf16x8 *>(X + row * srow + k);
         } else {
-          g.H[m * g.ldh + n] = to_bf16(x * silu_grad(g.Z[m * g.ldz + n]));
+#pragma unroll
+          for (int t = 0; t < 8; t++)
+            v[i][t] = (row < rows && k + t < kend) ? X[row * srow + k + t] : to_bf16(0.f);
+        }
+      } else if (srow == 1) {
+        const int k = k0 + q / (ROWS / 8), row = row0 + (q % (ROWS / 8)) * 8;
+        if (k < kend && row + 8 <= rows && (sk % 8) == 0) {
+          v[i] = *reinterpret_cast<const bf16x8 *>(X + k * sk + row);
+        } else {
+#pragma unroll
+          for (int t = 0; t < 8; t++)
+            v[i][t] = (k < kend && row + t < rows) ? X[k * sk + row + t] : to_bf16(0.f);
+        }
+      } else {
+        const int row = row0 + q / (kK / 8), k = k0 + (q % (kK / 8)) * 8;
+#pragma unroll
+        for (int t = 0; t < 8; t++)
+          v[i][t] = (row < rows && k + t < kend) ? X[row * srow + (k + t) * sk] : to_bf16(0.f);
+      }
+    }
+  }
+
+  __device__ __forceinline__ void store(__bf16 (*T)[kK + kLdsPad], long long srow,
+                                        long long sk, int tid) const {
+#pragma unroll
+    for (int i = 0; i < V; i++) {
+      const int q = tid + 256 * i;
+      if (sk != 1 && srow == 1) {
+        const int k = q / (ROWS / 8), r = (q % (ROWS / 8)) * 8;
+#pragma unroll
+        for (int t = 0; t < 8; t++) T[r + t][k] = v[i][t];
+      } else {
+        const int r = q / (kK / 8), c = (q % (kK / 8)) * 8;
+        *reinterpret_cast<bf16x8 *>(&T[r][c]) = v[i];
+      }
+    }
+  }
+};
+
+// C[m, n] = sum_k A(m, k) B(k, n), one BM x BN output tile per 256-thread
+// block; the four waves split it 2 x 2 and each owns (BM/2) x (BN/2) as
+// 16 x 16 tiles of v_mfma_f32_16x16x32_bf16.  K advances kK = 64 per stage
+// through double-buffered LDS (one barrier per stage); both operands sit
+// k-contiguous in LDS so each lane's MFMA operand (8 consecutive k) is one
+// 16-byte LDS read.
+template <int BM, int BN>
+__global__ __launch_bounds__(256) void mfma_gemm(GemmArgs g) {
+  constexpr int TM = BM / 32, TN = BN / 32;  // MFMA tiles per wave
+  __shared__ __attribute__((aligned(16))) __bf16 As[2][BM][kK + kLdsPad];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[2][BN][kK + kLdsPad];
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * (BM / 2), wn = (wave & 1) * (BN / 2);
+  int k_begin = 0, k_end = g.K;
+  float *C = g.C;
+  if (g.kchunk > 0) {
+    k_begin = blockIdx.z * g.kchunk;
+    k_end = min(g.K, k_begin + g.kchunk);
+    C += blockIdx.z * g.cstride;
+  }
+  f32x4 acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; i++)
+#pragma unroll
+    for (int j = 0; j < TN; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  TileLoad<BM> la;
+  TileLoad<BN> lb;
+  la.load(g.A, g.sam, g.sak, m0, g.M, k_begin, k_end, tid);
+  lb.load(g.B, g.sbn, g.sbk, n0, g.N, k_begin, k_end, tid);
+  la.store(As[0], g.sam, g.sak, tid);
+  lb.store(Bs[0], g.sbn, g.sbk, tid);
+  __syncthreads();
+  int buf = 0;
+  const int fr = lane & 15, fk = (lane >> 4) * 8;
+  for (int k0 = k_begin; k0 < k_end; k0 += kK) {
+    const bool more = k0 + kK < k_end;
+    if (more) {
+      la.load(g.A, g.sam, g.sak, m0, g.M, k0 + kK, k_end, tid);
+      lb.load(g.B, g.sbn, g.sbk, n0, g.N, k0 + kK, k_end, tid);
+    }
+#pragma unroll
+    for (int kk = 0; kk < kK; kk += 32) {
+      bf16x8 a[TM], b[TN];
+#pragma unroll
+      for (int i = 0; i < TM; i++)
+        a[i] = *reinterpret_cast<const bf16x8 *>(&As[buf][wm + 16 * i + fr][kk + fk]);
+#pragma unroll
+      for (int j = 0; j < TN; j++)
+        b[j] = *reinterpret_cast<const bf16x8 *>(&Bs[buf][wn + 16 * j + fr][kk + fk]);
+      // B first: the tile comes out transposed, so each lane holds four
+      // consecutive n of one m (vector stores in the epilogue).
+#pragma unroll
+      for (int i = 0; i < TM; i++)
+#pragma unroll
+        for (int j = 0; j < TN; j++)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[j], a[i], acc[i][j], 0, 0, 0);
+    }
+    if (more) {
+      la.store(As[buf ^ 1], g.sam, g.sak, tid);
+      lb.store(Bs[buf ^ 1], g.sbn, g.sbk, tid);
+    }
+    __syncthreads();
+    buf ^= 1;
+  }
+  // Epilogue: lane holds C[m][n .. n + 3] of each tile, m = lane % 16 and
+  // n = 4 * (lane / 16) within the tile.
+#pragma unroll
+  for (int i = 0; i < TM; i++)
+#pragma unroll
+    for (int j = 0; j < TN; j++) {
+      const int m = m0 + wm + 16 * i + (lane & 15);
+      const int n = n0 + wn + 16 * j + 4 * (lane >> 4);
+      if (m >= g.M) continue;
+      const f32x4 x = acc[i][j];
+      const bool whole = n + 3 < g.N;
+      if (g.epilogue == kStoreF32) {
+        float *c = C + m * g.ldc + n;
+        if (whole && (g.ldc % 4) == 0) {
+          f32x4 y = x;
+          if (g.beta != 0.f) y += g.beta * *reinterpret_cast<const f32x4 *>(c);
+          *reinterpret_cast<f32x4 *>(c) = y;
+        } else {
+#pragma unroll
+          for (int v = 0; v < 4 && n + v < g.N; v++)
+            c[v] = g.beta != 0.f ? x[v] + g.beta * c[v] : x[v];
+        }
+      } else if (g.epilogue == kBiasSilu) {
+        if (whole && (g.ldz % 4) == 0 && (g.ldh % 4) == 0) {
+          const f32x4 z = x + *reinterpret_cast<const f32x4 *>(g.bias + n);
+          bf16x4 hv, zv;
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            hv[v] = to_bf16(silu(z[v]));
+            zv[v] = to_bf16(z[v]);
+          }
+          if (g.Z) *reinterpret_cast<bf16x4 *>(g.Z + m * g.ldz + n) = zv;
+          *reinterpret_cast<bf16x4 *>(g.H + m * g.ldh + n) = hv;
+        } else {
+#pragma unroll
+          for (int v = 0; v < 4 && n + v < g.N; v++) {
+            const float z = x[v] + g.bias[n + v];
+            if (g.Z) g.Z[m * g.ldz + n + v] = to_bf16(z);
+            g.H[m * g.ldh + n + v] = to_bf16(silu(z));
+          }
+        }
+      } else {
+        if (whole && (g.ldz % 4) == 0 && (g.ldh % 4) == 0) {
+          const bf16x4 z = *reinterpret_cast<const bf16x4 *>(g.Z + m * g.ldz + n);
+          bf16x4 hv;
+#pragma unroll
+          for (int v = 0; v < 4; v++) hv[v] = to_bf16(x[v] * silu_grad((float)z[v]));
+          *reinterpret_cast<bf16x4 *>(g.H + m * g.ldh + n) = hv;
+        } else {
+#pragma unroll
+          for (int v = 0; v < 4 && n + v < g.N; v++)
+            g.H[m * g.ldh + n + v] =
+                to_bf16(x[v] * silu_grad((float)g.Z[m * g.ldz + n + v]));
         }
       }
+    }
 }
 
 // ---------------------------------------------------------------- features
@@ -228,72 +329,111 @@ __global__ __launch_bounds__(256) void mlp_loss(const float *__restrict__ yhat,
   if ((threadIdx.x & 63) == 0 && l != 0.f) atomicAdd(loss, l);
 }
 
-// Head backward: dw4 = sum_i dy_i h_i, db4 = sum dy, and
-// dZ3[i, k] = dy_i w4[k] silu'(Z3[i, k]) (bf16).
-__global__ __launch_bounds__(256) void mlp_head_back(
-    const __bf16 *__restrict__ H, const float *__restrict__ Z, long long n,
-    int width, const float *__restrict__ w, const float *__restrict__ dy,
-    float *__restrict__ dw, float *__restrict__ db, __bf16 *__restrict__ dZ) {
-  const int k = blockIdx.x;  // one block per hidden unit
-  float t = 0.f, tb = 0.f;
-  for (long long i = threadIdx.x; i < n; i += blockDim.x) {
-    t = fmaf(dy[i], (float)H[i * width + k], t);
-    tb += dy[i];
-    dZ[i * width + k] = to_bf16(dy[i] * w[k] * silu_grad(Z[i * width + k]));
+// Head backward, one thread per (row, unit): dZ3[i, k] = dy_i w4[k]
+// silu'(Z3[i, k]) (bf16).  The head's weight and bias gradients are column
+// sums (column_partials with weights dy).
+__global__ __launch_bounds__(256) void mlp_head_back(const __bf16 *__restrict__ Z, long long n,
+                                                     int width, const float *__restrict__ w,
+                                                     const float *__restrict__ dy,
+                                                     __bf16 *__restrict__ dZ) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * width) return;
+  const long long i = t / width;
+  const int k = (int)(t - i * width);
+  dZ[t] = to_bf16(dy[i] * w[k] * silu_grad((float)Z[t]));
+}
+
+// Column sums of a row-major [n, width] matrix over chunks of `rows` rows:
+// partial[c, k] = sum over rows i of chunk c of a_i X[i, k] (a_i = wt[i], or
+// 1 without weights); with X null, partial[c, 0] = sum of wt over the
+// chunk.  Threads run along k, so every row read is coalesced.
+__global__ __launch_bounds__(256) void column_partials(const __bf16 *__restrict__ X,
+                                                       const float *__restrict__ wt,
+                                                       long long n, int width, int rows,
+                                                       float *__restrict__ partial) {
+  const long long r0 = (long long)blockIdx.x * rows;
+  const long long r1 = min(n, r0 + rows);
+  if (!X) {
+    if (threadIdx.x == 0) {
+      float t = 0.f;
+      for (long long i = r0; i < r1; i++) t += wt[i];
+      partial[blockIdx.x] = t;
+    }
+    return;
   }
-  __shared__ float red[2][4];
-  for (int off = 32; off > 0; off >>= 1) {
-    t += __shfl_xor(t, off, 64);
-    tb += __shfl_xor(tb, off, 64);
-  }
-  if ((threadIdx.x & 63) == 0) {
-    red[0][threadIdx.x >> 6] = t;
-    red[1][threadIdx.x >> 6] = tb;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    dw[k] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
-    if (k == 0) db[0] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+  for (int k = threadIdx.x; k < width; k += blockDim.x) {
+    float t = 0.f;
+#pragma unroll 16
+    for (long long i = r0; i < r1; i++)
+      t = fmaf(wt ? wt[i] : 1.f, (float)X[i * width + k], t);
+    partial[(long long)blockIdx.x * width + k] = t;
   }
 }
 
-// db[n] = sum_m dZ[m, n] (bf16 dZ), one block per column.
-__global__ __launch_bounds__(256) void column_sum(const __bf16 *__restrict__ X,
-                                                  long long m, int width,
-                                                  float *__restrict__ out) {
-  const int n = blockIdx.x;
+// out[j] = sum over c of partial[c * count + j], in c order: one thread per
+// output, reads coalesced across threads (many outputs, few chunks).
+__global__ __launch_bounds__(256) void sum_partials(const float *__restrict__ partial,
+                                                    int chunks, long long count,
+                                                    float *__restrict__ out) {
+  const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count) return;
   float t = 0.f;
-  for (long long i = threadIdx.x; i < m; i += blockDim.x) t += (float)X[i * width + n];
-  __shared__ float red[4];
-  for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = t;
-  __syncthreads();
-  if (threadIdx.x == 0) out[n] = red[0] + red[1] + red[2] + red[3];
+#pragma unroll 8
+  for (int c = 0; c < chunks; c++) t += partial[(long long)c * count + j];
+  out[j] = t;
 }
 
-// AdamW on the FP32 masters; refreshes the BF16 copy and its transpose.
-__global__ __launch_bounds__(256) void adamw(float *__restrict__ p,
-                                             const float *__restrict__ g,
-                                             float *__restrict__ m1,
-                                             float *__restrict__ m2, long long n,
-                                             float lr, float b1, float b2,
-                                             float eps, float wd, float c1,
-                                             float c2, float clip,
-                                             __bf16 *__restrict__ copy,
-                                             __bf16 *__restrict__ copy_t,
-                                             int rows, int cols) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float gi = g[i] * clip;
-  m1[i] = b1 * m1[i] + (1.f - b1) * gi;
-  m2[i] = b2 * m2[i] + (1.f - b2) * gi * gi;
-  const float mh = m1[i] / c1, vh = m2[i] / c2;
-  const float x = p[i] * (1.f - lr * wd) - lr * mh / (sqrtf(vh) + eps);
-  p[i] = x;
-  if (copy) copy[i] = to_bf16(x);
-  if (copy_t) {
-    const int r = (int)(i / cols), c = (int)(i % cols);
-    copy_t[(long long)c * rows + r] = to_bf16(x);
+// The same sum with one wave per output: lane l adds chunks l, l + 64, ...
+// and the wave combines its lanes in a fixed butterfly order (few outputs,
+// many chunks).
+__global__ __launch_bounds__(256) void sum_partials_wide(const float *__restrict__ partial,
+                                                         int chunks, long long count,
+                                                         float *__restrict__ out) {
+  const long long j = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (j >= count) return;  // uniform per wave
+  const int lane = threadIdx.x & 63;
+  float t = 0.f;
+  for (int c = lane; c < chunks; c += 64) t += partial[(long long)c * count + j];
+  for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+  if (lane == 0) out[j] = t;
+}
+
+// AdamW over every parameter tensor of the network in one launch: segment
+// k covers elements [first[k], first[k + 1]) of the concatenation.
+constexpr int kMaxSegments = 2 * 8 + 2;
+struct AdamSegment {
+  float *p;
+  const float *g;
+  float *m1, *m2;
+  __bf16 *copy, *copy_t;
+  int rows, cols;
+  float wd;
+};
+struct AdamSegments {
+  AdamSegment s[kMaxSegments];
+  long long first[kMaxSegments + 1];
+  int count;
+};
+
+__global__ __launch_bounds__(256) void adamw_all(AdamSegments segs, float lr, float b1,
+                                                 float b2, float eps, float c1, float c2) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= segs.first[segs.count]) return;
+  int k = 0;
+  while (t >= segs.first[k + 1]) k++;
+  const AdamSegment &a = segs.s[k];
+  const long long i = t - segs.first[k];
+  const float gi = a.g[i];
+  const float m1 = b1 * a.m1[i] + (1.f - b1) * gi;
+  const float m2 = b2 * a.m2[i] + (1.f - b2) * gi * gi;
+  a.m1[i] = m1;
+  a.m2[i] = m2;
+  const float x = a.p[i] * (1.f - lr * a.wd) - lr * (m1 / c1) / (sqrtf(m2 / c2) + eps);
+  a.p[i] = x;
+  if (a.copy) a.copy[i] = to_bf16(x);
+  if (a.copy_t) {
+    const int r = (int)(i / a.cols), c = (int)(i % a.cols);
+    a.copy_t[(long long)c * a.rows + r] = to_bf16(x);
   }
 }
 
@@ -329,10 +469,59 @@ using namespace pe;
 
 
 
-static int launch_gemm(pe_mlp *h, const GemmArgs &g) {
-  const dim3 grid((unsigned)((g.M + kTile - 1) / kTile), (unsigned)((g.N + kTile - 1) / kTile));
-  hipLaunchKernelGGL(mfma_gemm, grid, dim3(256), 0, h->stream, g);
+// Launch with the largest output tile that still gives at least 256 blocks
+// (one per CU), else 64 x 64.
+static int launch_gemm_on(hipStream_t stream, const GemmArgs &g, int splits) {
+  auto blocks = [&](int bm, int bn) {
+    return (long long)((g.M + bm - 1) / bm) * ((g.N + bn - 1) / bn) * splits;
+  };
+  auto go = [&](auto kernel, int bm, int bn) {
+    const dim3 grid((unsigned)((g.M + bm - 1) / bm), (unsigned)((g.N + bn - 1) / bn),
+                    (unsigned)splits);
+    hipLaunchKernelGGL(kernel, grid, dim3(256), 0, stream, g);
+  };
+  if (blocks(128, 128) >= 256) go(mfma_gemm<128, 128>, 128, 128);
+  else if (blocks(128, 64) >= 256) go(mfma_gemm<128, 64>, 128, 64);
+  else go(mfma_gemm<64, 64>, 64, 64);
   PE_LAUNCHED(mfma_gemm);
+  return PE_OK;
+}
+
+static int launch_gemm(pe_mlp *h, const GemmArgs &g, int splits = 1) {
+  return launch_gemm_on(h->stream, g, splits);
+}
+
+// Column sums: at least kSumRows rows per chunk and at most kSumChunks
+// chunks.  Weight-gradient GEMMs: the batch is split into at most
+// kMaxSplits chunks of at least kSplitK rows (multiples of the LDS stage),
+// enough blocks to fill the chip for a batch of 8192 without moving more
+// partial products than needed for larger ones.
+constexpr int kSumRows = 16;
+constexpr int kSumChunks = 512;
+constexpr int kSplitK = 256;
+constexpr int kMaxSplits = 64;
+
+static int sum_rows(long long n) {
+  return (int)std::max<long long>(kSumRows, (n + kSumChunks - 1) / kSumChunks);
+}
+static int split_chunk(long long n) {
+  long long chunk = std::max<long long>(kSplitK, (n + kMaxSplits - 1) / kMaxSplits);
+  return (int)((chunk + kK - 1) / kK * kK);
+}
+
+// out[width] = column sums of X [n, width] weighted by wt (see
+// column_partials), through the handle's partial buffer.
+static int column_sums(pe_mlp *h, const __bf16 *X, const float *wt, long long n, int width,
+                       float *out) {
+  const int rows = sum_rows(n);
+  const int chunks = (int)((n + rows - 1) / rows);
+  hipLaunchKernelGGL(column_partials, dim3((unsigned)chunks), dim3(256), 0, h->stream, X, wt,
+                     n, width, rows, h->partial);
+  PE_LAUNCHED(column_partials);
+  const long long count = X ? width : 1;
+  hipLaunchKernelGGL(sum_partials_wide, dim3((unsigned)((count + 3) / 4)), dim3(256), 0,
+                     h->stream, h->partial, chunks, count, out);
+  PE_LAUNCHED(sum_partials_wide);
   return PE_OK;
 }
 
@@ -356,8 +545,8 @@ static int forward(pe_mlp *h, long long n, bool keep) {
     g.N = h->fan_out[l];
     g.K = h->fan_in[l];
     g.epilogue = kBiasSilu;
-    g.C = keep ? h->Zs[l] : nullptr;
-    g.ldc = h->fan_out[l];
+    g.Z = keep ? h->Zs[l] : nullptr;
+    g.ldz = h->fan_out[l];
     g.bias = h->b[l];
     g.H = h->Hs[l];
     g.ldh = h->fan_out[l];
@@ -372,19 +561,20 @@ static int forward(pe_mlp *h, long long n, bool keep) {
   return PE_OK;
 }
 
-// Gradients of the weighted MSE of n points in raw / yt into gW, gb, ...
-static int backward(pe_mlp *h, long long n, float *loss_host) {
+// Gradients of the weighted MSE of n points in raw / yt into gW, gb, ...;
+// the batch loss is added to *loss (device memory, zeroed by the caller).
+static int backward(pe_mlp *h, long long n, float *loss) {
   int rc = forward(h, n, true);
   if (rc) return rc;
-  hipLaunchKernelGGL(fill_f32, dim3(1), dim3(64), 0, h->stream, h->loss, 1ll, 0.f);
   hipLaunchKernelGGL(mlp_loss, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream,
-                     h->yhat, h->yt, n, h->cfg.under_weight, h->dy, h->loss);
+                     h->yhat, h->yt, n, h->cfg.under_weight, h->dy, loss);
   PE_LAUNCHED(mlp_loss);
   const int L = h->layers;
   const int top = h->fan_out[L - 1];
-  hipLaunchKernelGGL(mlp_head_back, dim3((unsigned)top), dim3(256), 0, h->stream,
-                     h->Hs[L - 1], h->Zs[L - 1], n, top, h->w_head, h->dy, h->gw_head,
-                     h->gb_head, h->dZ[L - 1]);
+  if ((rc = column_sums(h, h->Hs[L - 1], h->dy, n, top, h->gw_head))) return rc;
+  if ((rc = column_sums(h, nullptr, h->dy, n, 1, h->gb_head))) return rc;
+  hipLaunchKernelGGL(mlp_head_back, dim3((unsigned)((n * top + 255) / 256)), dim3(256), 0,
+                     h->stream, h->Zs[L - 1], n, top, h->w_head, h->dy, h->dZ[L - 1]);
   PE_LAUNCHED(mlp_head_back);
   for (int l = L - 1; l >= 0; l--) {
     const __bf16 *input = l == 0 ? h->F : h->Hs[l - 1];
@@ -401,12 +591,19 @@ static int backward(pe_mlp *h, long long n, float *loss_host) {
     g.N = h->fan_in[l];
     g.K = (int)n;
     g.epilogue = kStoreF32;
-    g.C = h->gW[l];
+    // split over the batch: partial products in h->partial, then summed
+    const int kchunk = split_chunk(n);
+    const int splits = (int)((n + kchunk - 1) / kchunk);
+    const long long count = (long long)h->fan_out[l] * h->fan_in[l];
+    g.C = h->partial;
     g.ldc = h->fan_in[l];
-    if ((rc = launch_gemm(h, g))) return rc;
-    hipLaunchKernelGGL(column_sum, dim3((unsigned)h->fan_out[l]), dim3(256), 0, h->stream,
-                       h->dZ[l], n, h->fan_out[l], h->gb[l]);
-    PE_LAUNCHED(column_sum);
+    g.kchunk = kchunk;
+    g.cstride = count;
+    if ((rc = launch_gemm(h, g, splits))) return rc;
+    hipLaunchKernelGGL(sum_partials, dim3((unsigned)((count + 255) / 256)), dim3(256), 0,
+                       h->stream, h->partial, splits, count, h->gW[l]);
+    PE_LAUNCHED(sum_partials);
+    if ((rc = column_sums(h, h->dZ[l], nullptr, n, h->fan_out[l], h->gb[l]))) return rc;
     if (l > 0) {
       // dZ_{l-1}[m, i] = (sum_o dZ_l[m, o] W_l[o, i]) * silu'(Z_{l-1}[m, i])
       GemmArgs d{};
@@ -427,9 +624,6 @@ static int backward(pe_mlp *h, long long n, float *loss_host) {
       if ((rc = launch_gemm(h, d))) return rc;
     }
   }
-  if (loss_host) {
-    PE_HIP(hipMemcpyAsync(loss_host, h->loss, 4, hipMemcpyDeviceToHost, h->stream));
-  }
   return PE_OK;
 }
 
@@ -438,29 +632,28 @@ static int optimizer_step(pe_mlp *h) {
   const pe_mlp_config &c = h->cfg;
   const float c1 = 1.f - std::pow(c.beta1, (float)h->step);
   const float c2 = 1.f - std::pow(c.beta2, (float)h->step);
-  auto run = [&](float *p, float *g, float *m, float *v, long long n, __bf16 *cp,
-                 __bf16 *ct, int rows, int cols, float wd) -> int {
-    hipLaunchKernelGGL(adamw, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream,
-                       p, g, m, v, n, c.learning_rate, c.beta1, c.beta2, c.eps, wd, c1, c2,
-                       1.f, cp, ct, rows, cols);
-    PE_LAUNCHED(adamw);
-    return PE_OK;
+  AdamSegments segs{};
+  auto add = [&](float *p, float *g, float *m, float *v, long long n, __bf16 *cp, __bf16 *ct,
+                 int rows, int cols, float wd) {
+    segs.s[segs.count] = AdamSegment{p, g, m, v, cp, ct, rows, cols, wd};
+    segs.first[segs.count + 1] = segs.first[segs.count] + n;
+    segs.count++;
   };
-  int rc;
   for (int l = 0; l < h->layers; l++) {
-    const long long n = (long long)h->fan_out[l] * h->fan_in[l];
-    if ((rc = run(h->W[l], h->gW[l], h->mW[l], h->vW[l], n, h->Wh[l], h->Wt[l], h->fan_out[l],
-                  h->fan_in[l], c.weight_decay)))
-      return rc;
-    if ((rc = run(h->b[l], h->gb[l], h->mb[l], h->vb[l], h->fan_out[l], nullptr, nullptr, 1,
-                  h->fan_out[l], 0.f)))
-      return rc;
+    add(h->W[l], h->gW[l], h->mW[l], h->vW[l], (long long)h->fan_out[l] * h->fan_in[l],
+        h->Wh[l], h->Wt[l], h->fan_out[l], h->fan_in[l], c.weight_decay);
+    add(h->b[l], h->gb[l], h->mb[l], h->vb[l], h->fan_out[l], nullptr, nullptr, 1,
+        h->fan_out[l], 0.f);
   }
   const int top = h->fan_out[h->layers - 1];
-  if ((rc = run(h->w_head, h->gw_head, h->mw_head, h->vw_head, top, nullptr, nullptr, 1, top,
-                c.weight_decay)))
-    return rc;
-  return run(h->b_head, h->gb_head, h->mb_head, h->vb_head, 1, nullptr, nullptr, 1, 1, 0.f);
+  add(h->w_head, h->gw_head, h->mw_head, h->vw_head, top, nullptr, nullptr, 1, top,
+      c.weight_decay);
+  add(h->b_head, h->gb_head, h->mb_head, h->vb_head, 1, nullptr, nullptr, 1, 1, 0.f);
+  const long long total = segs.first[segs.count];
+  hipLaunchKernelGGL(adamw_all, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                     h->stream, segs, c.learning_rate, c.beta1, c.beta2, c.eps, c1, c2);
+  PE_LAUNCHED(adamw_all);
+  return PE_OK;
 }
 
 static int refresh_copies(pe_mlp *h) {
@@ -508,7 +701,8 @@ static int for_each_param(pe_mlp *h, F f) {
 static int ensure_capacity(pe_mlp *h, long long n) {
   if (n <= h->cap) return PE_OK;
   auto free_all = [&]() {
-    for (void *p : {(void *)h->raw, (void *)h->yt, (void *)h->yhat, (void *)h->dy, (void *)h->F})
+    for (void *p : {(void *)h->raw, (void *)h->yt, (void *)h->yhat, (void *)h->dy, (void *)h->F,
+                    (void *)h->partial})
       if (p) (void)hipFree(p);
     for (auto *p : h->Hs) if (p) (void)hipFree(p);
     for (auto *p : h->dZ) if (p) (void)hipFree(p);
@@ -522,10 +716,21 @@ static int ensure_capacity(pe_mlp *h, long long n) {
   PE_HIP(hipMalloc(&h->yhat, sizeof(float) * cap));
   PE_HIP(hipMalloc(&h->dy, sizeof(float) * cap));
   PE_HIP(hipMalloc(&h->F, sizeof(__bf16) * cap * h->shape.width));
+  {
+    long long widest = 1, largest = 1;
+    for (int l = 0; l < h->layers; l++) {
+      widest = std::max<long long>(widest, h->fan_out[l]);
+      largest = std::max<long long>(largest, (long long)h->fan_out[l] * h->fan_in[l]);
+    }
+    const long long splits = (cap + split_chunk(cap) - 1) / split_chunk(cap);
+    const long long chunks = (cap + sum_rows(cap) - 1) / sum_rows(cap);
+    PE_HIP(hipMalloc(&h->partial,
+                     sizeof(float) * std::max(splits * largest, chunks * widest)));
+  }
   for (int l = 0; l < h->layers; l++) {
     PE_HIP(hipMalloc(&h->Hs[l], sizeof(__bf16) * cap * h->fan_out[l]));
     PE_HIP(hipMalloc(&h->dZ[l], sizeof(__bf16) * cap * h->fan_out[l]));
-    PE_HIP(hipMalloc(&h->Zs[l], sizeof(float) * cap * h->fan_out[l]));
+    PE_HIP(hipMalloc(&h->Zs[l], sizeof(__bf16) * cap * h->fan_out[l]));
   }
   h->cap = cap;
   return PE_OK;
@@ -708,8 +913,10 @@ extern "C" int pe_mlp_gradients(pe_mlp *h, const float *raw, const float *target
   PE_HIP(hipMemcpyAsync(h->raw, raw, sizeof(float) * n * h->shape.raw, hipMemcpyHostToDevice,
                         h->stream));
   PE_HIP(hipMemcpyAsync(h->yt, target, sizeof(float) * n, hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(fill_f32, dim3(1), dim3(64), 0, h->stream, h->loss, 1ll, 0.f);
+  if ((rc = backward(h, n, h->loss))) return rc;
   float l = 0;
-  if ((rc = backward(h, n, &l))) return rc;
+  PE_HIP(hipMemcpyAsync(&l, h->loss, 4, hipMemcpyDeviceToHost, h->stream));
   PE_HIP(hipStreamSynchronize(h->stream));
   if (loss) *loss = l;
   return for_each_param(h, [&](float *, float *g, long long at, long long m) -> int {
@@ -741,19 +948,33 @@ extern "C" int pe_mlp_train(pe_mlp *h, int32_t steps, int32_t batch, uint32_t se
   PE_HIP(hipSetDevice(h->device));
   int rc = ensure_capacity(h, batch);
   if (rc) return rc;
-  std::vector<float> host(steps > 0 ? steps : 1);
-  for (int s = 0; s < steps; s++) {
-    hipLaunchKernelGGL(gather_rows, dim3((unsigned)((batch + 255) / 256)), dim3(256), 0,
-                       h->stream, h->data_x, h->data_y, h->shape.raw, h->data_n,
-                       (long long)batch, seed, (unsigned)(h->step + s), h->raw, h->yt);
-    PE_LAUNCHED(gather_rows);
-    if ((rc = backward(h, batch, losses ? &host[s] : nullptr))) return rc;
-    if ((rc = optimizer_step(h))) return rc;
-  }
-  PE_HIP(hipStreamSynchronize(h->stream));
-  if (losses)
-    for (int s = 0; s < steps; s++) losses[s] = host[s];
-  return PE_OK;
+  // Per-step losses stay on the device until the end: no step waits on the
+  // host.
+  float *history = nullptr;
+  PE_HIP(hipMalloc(&history, sizeof(float) * (steps > 0 ? steps : 1)));
+  auto body = [&]() -> int {
+    hipLaunchKernelGGL(fill_f32, dim3(64), dim3(256), 0, h->stream, history,
+                       (long long)(steps > 0 ? steps : 1), 0.f);
+    PE_LAUNCHED(fill_f32);
+    const unsigned first = (unsigned)h->step;
+    for (int s = 0; s < steps; s++) {
+      hipLaunchKernelGGL(gather_rows, dim3((unsigned)((batch + 255) / 256)), dim3(256), 0,
+                         h->stream, h->data_x, h->data_y, h->shape.raw, h->data_n,
+                         (long long)batch, seed, first + (unsigned)s, h->raw, h->yt);
+      PE_LAUNCHED(gather_rows);
+      int r = backward(h, batch, history + s);
+      if (r) return r;
+      if ((r = optimizer_step(h))) return r;
+    }
+    if (losses && steps > 0)
+      PE_HIP(hipMemcpyAsync(losses, history, sizeof(float) * steps, hipMemcpyDeviceToHost,
+                            h->stream));
+    PE_HIP(hipStreamSynchronize(h->stream));
+    return PE_OK;
+  };
+  rc = body();
+  (void)hipFree(history);
+  return rc;
 }
 
 extern "C" int pe_mlp_gemm_check(const uint16_t *a, const uint16_t *b, int32_t M,
@@ -789,9 +1010,8 @@ extern "C" int pe_mlp_gemm_check(const uint16_t *a, const uint16_t *b, int32_t M
     g.epilogue = kStoreF32;
     g.C = (float *)dc;
     g.ldc = N;
-    const dim3 grid((unsigned)((M + kTile - 1) / kTile), (unsigned)((N + kTile - 1) / kTile));
-    hipLaunchKernelGGL(mfma_gemm, grid, dim3(256), 0, 0, g);
-    PE_LAUNCHED(mfma_gemm);
+    int r = launch_gemm_on(0, g, 1);
+    if (r) return r;
     PE_HIP(hipMemcpy(c, dc, 4ll * M * N, hipMemcpyDeviceToHost));
     return PE_OK;
   };
@@ -819,6 +1039,7 @@ extern "C" void pe_mlp_destroy(pe_mlp *h) {
                   (void *)h->mw_head, (void *)h->vw_head, (void *)h->mb_head, (void *)h->vb_head,
                   (void *)h->shift, (void *)h->scale, (void *)h->proj, (void *)h->raw,
                   (void *)h->yt, (void *)h->yhat, (void *)h->dy, (void *)h->loss, (void *)h->F,
+                  (void *)h->partial,
                   (void *)h->data_x, (void *)h->data_y})
     f(p);
   if (h->stream) (void)hipStreamDestroy(h->stream);

@@ -155,3 +155,49 @@ def to_bf16_bits(x):
     u = np.ascontiguousarray(x, np.float32).view(np.uint32)
     r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
     return r, (r.astype(np.uint32) << 16).view(np.float32)
+
+
+def training_flops(net, batch):
+    """Algorithmic FLOPs of one training step of `batch` points: forward
+    (2 per multiply-add of every layer, head included) plus backward (the
+    weight gradients of every layer and the input gradients of all but the
+    first)."""
+    c = net.config
+    width = ((8 + c.rff + c.extra + 5 + 15) // 16) * 16
+    dims = [width] + [c.hidden] * c.layers
+    layer = [dims[i] * dims[i + 1] for i in range(c.layers)]
+    forward = 2 * batch * (sum(layer) + c.hidden)
+    backward = 2 * batch * (sum(layer) + sum(layer[1:]) + 2 * c.hidden)
+    return forward + backward
+
+
+def main(argv=None):
+    """Training throughput on synthetic data: `steps` AdamW steps at
+    `batch` points, printed as one JSON line with the algorithmic TFLOP/s."""
+    import argparse
+    import json
+    import time
+
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--batch", type=int, default=8192)
+    ap.add_argument("--points", type=int, default=1 << 20)
+    a = ap.parse_args(argv)
+    rng = np.random.default_rng(0)
+    x = rng.uniform(-1, 1, (a.points, 8 + EXTRA)).astype(np.float32)
+    y = (np.sin(3 * x[:, 0]) * np.cos(2 * x[:, 4]) + 0.1 * x[:, 8]).astype(np.float32)
+    with ValueMLP(shift=np.zeros(8), scale=np.ones(8)) as net:
+        net.fit(x, y, steps=5, batch=a.batch)
+        t0 = time.perf_counter()
+        losses = net.fit(x, y, steps=a.steps, batch=a.batch, seed=1)
+        seconds = time.perf_counter() - t0
+        flops = training_flops(net, a.batch) * a.steps
+    print(json.dumps(dict(steps=a.steps, batch=a.batch, seconds=seconds,
+                          ms_per_step=seconds / a.steps * 1e3,
+                          tflops=flops / seconds / 1e12, loss_first=float(losses[0]),
+                          loss_last=float(losses[-1]))), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -195,6 +195,128 @@ def evaluate(net, delta, seed, *, vstar_tol=1e-6, max_sweeps=60_000, log=print, 
     return out
 
 
+def _batch(seeds, grid):
+    """Tables of several seeded maps (geometry of all maps in one launch per
+    phase), with each car's clearance as the network's map features."""
+    from ..local_game.reference import benchmark_games
+
+    out = []
+    for _, arena, p, e, tables in benchmark_games(seeds, **grid):
+        for side, g in (("p", p), ("e", e)):
+            tables[side]["clearance"] = g.geometry.clearance(g.states)
+        out.append((p, e, tables))
+    return out
+
+
+def _run_batch(solver, tol, max_sweeps, every=16):
+    """Plain sweeps of every map of a batched solver until each map's
+    residual is <= tol (converged maps frozen); GPU seconds, sweeps, and
+    each map's stopping sweep."""
+    t0 = solver.stats()["update_seconds"]
+    done, ok = solver.run(tol, max_sweeps, every, FLAGS | _native.PE_SL_FREEZE)
+    checks = solver.trace()[every - 1::every]
+    stops = [int(every * (np.argmax(checks[:, k] <= tol) + 1)) if (checks[:, k] <= tol).any()
+             else None for k in range(checks.shape[1])]
+    return solver.stats()["update_seconds"] - t0, done, ok, stops
+
+
+def compare_batched(train_seeds, validate_seeds, test_seeds, *, grid, tol=1e-4,
+                    max_sweeps=20_000, vstar_tol=1e-6, vstar_cap=30_000,
+                    steps=3000, batch=8192, samples=32768, seed=0, log=print):
+    """The learned warm start against the plain DP on batches of maps.
+
+    Labels: the plain DP to residual `tol` on the training maps (all maps in
+    one handle).  The network (8D state, phi, both clearances) is trained
+    on the GPU by the library's MFMA kernels; delta is the largest
+    underestimate over the inside pairs of the validation maps.  On the
+    held-out test maps: the plain DP from phi and the warm start
+    min(phi, net + delta), each run to residual `tol` with converged maps
+    frozen; GPU seconds of both.  V* is the plain DP continued to residual
+    vstar_tol (or vstar_cap sweeps).  The judge: the warm answer is as close
+    to V* as the plain answer (or within 1e-3), and the warm start is
+    nowhere more than 1e-3 below V* (a value below V* stays below it)."""
+    out = dict(train_maps=len(train_seeds), validate_maps=len(validate_seeds),
+               test_maps=len(test_seeds), tolerance=tol)
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    games = _batch(list(train_seeds) + list(validate_seeds), grid)
+    n_train = len(train_seeds)
+    with GameSolver([g[2] for g in games]) as s:
+        label_s, _, ok, _ = _run_batch(s, tol, max_sweeps)
+        fields = [s.read(k) for k in range(len(games))]
+        inputs = [s.pair_inputs(0, f.size, k) for k, f in enumerate(fields)]
+    out.update(label_gpu_seconds=label_s, labels_converged=bool(ok))
+    xs, ys = [], []
+    for (p, e, _), f, x in zip(games[:n_train], fields, inputs):
+        inside = np.flatnonzero(dense_inside(p, e, EXIT_RADIUS).ravel())
+        pick = np.sort(rng.choice(inside, size=min(samples, len(inside)), replace=False))
+        xs.append(x[pick])
+        ys.append(f.ravel()[pick])
+    p0, e0, _ = games[0]
+    t1 = time.perf_counter()
+    net, info = train(np.concatenate(xs), np.concatenate(ys), p0, e0, steps=steps, batch=batch,
+                      seed=seed, under_weight=4.0)
+    out.update(training=info, train_wall_seconds=time.perf_counter() - t1)
+    worst, rmse = 0.0, []
+    for (p, e, _), f, x in zip(games[n_train:], fields[n_train:], inputs[n_train:]):
+        keep = dense_inside(p, e, EXIT_RADIUS).ravel()
+        miss = f.ravel()[keep] - net(x[keep])
+        worst = max(worst, float(miss.max()))
+        rmse.append(float(np.sqrt(np.mean(miss ** 2))))
+    delta = max(0.0, worst)
+    out.update(validation_rmse=rmse, delta=delta)
+    log(f"[learned] labels {label_s:.1f} s GPU, training {out['train_wall_seconds']:.1f} s, "
+        f"validation RMSE {np.round(rmse, 3).tolist()}, delta {delta:.3f}")
+    held = _batch(list(test_seeds), grid)
+    tables = [g[2] for g in held]
+    masks = [dense_inside(p, e, EXIT_RADIUS) for p, e, _ in held]
+    with GameSolver(tables) as ref:
+        # V*: the plain DP continued to residual vstar_tol (or the cap)
+        _, vstar_done, vstar_ok, _ = _run_batch(ref, vstar_tol, vstar_cap)
+        vstar = [ref.read(k) for k in range(len(held))]
+    with GameSolver(tables) as cold:
+        cold_s, cold_done, cold_ok, cold_stops = _run_batch(cold, tol, max_sweeps)
+        cold_fields = [cold.read(k) for k in range(len(held))]
+    with GameSolver(tables) as warm:
+        w0 = time.perf_counter()
+        warm.warm_start(net, delta)
+        start_s = time.perf_counter() - w0
+        v0 = [warm.read(k) for k in range(len(held))]
+        warm_s, warm_done, warm_ok, warm_stops = _run_batch(warm, tol, max_sweeps)
+        warm_fields = [warm.read(k) for k in range(len(held))]
+
+    def worst(fields, sign=None):
+        out = []
+        for f, v, keep in zip(fields, vstar, masks):
+            d = (f - v)[keep] if sign is None else sign * (f - v)[keep]
+            out.append(float(np.abs(d).max() if sign is None else max(d.max(), 0.0)))
+        return out
+
+    cold_err, warm_err = worst(cold_fields), worst(warm_fields)
+    below = worst(v0, -1.0)
+    out.update(
+        vstar=dict(tolerance=vstar_tol, sweeps=vstar_done, converged=vstar_ok),
+        cold=dict(gpu_seconds=cold_s, sweeps=cold_done, converged=cold_ok,
+                  map_sweeps=cold_stops, max_abs_error_vs_vstar=cold_err),
+        warm=dict(gpu_seconds=warm_s + start_s, start_seconds=start_s, sweeps=warm_done,
+                  converged=warm_ok, map_sweeps=warm_stops, max_abs_error_vs_vstar=warm_err,
+                  start_below_vstar_max=below,
+                  start_mean_gap_to_vstar=[float((z - v)[keep].mean())
+                                           for z, v, keep in zip(v0, vstar, masks)]),
+        # The plain DP judges the warm answer: it must be as close to V* as
+        # the plain answer at the same residual (or within 1e-3), and the
+        # warm start must not lock in an underestimate beyond 1e-3.
+        judge_ok=bool(all(w <= max(c, 1e-3) for w, c in zip(warm_err, cold_err))
+                      and max(below) <= 1e-3),
+        speedup=cold_s / (warm_s + start_s) if warm_s + start_s > 0 else None,
+        total_wall_seconds=time.perf_counter() - t0)
+    log(f"[learned] held-out: cold {cold_s:.2f} s / {cold_done} sweeps, warm "
+        f"{warm_s + start_s:.2f} s / {warm_done} sweeps; error vs V*: cold "
+        f"{max(cold_err):.2e}, warm {max(warm_err):.2e}; judge {out['judge_ok']}")
+    net.close()
+    return out
+
+
 def seeds(text):
     lo, _, hi = text.partition("-")
     return list(range(int(lo), int(hi or lo) + 1))

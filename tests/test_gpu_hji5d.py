@@ -295,3 +295,100 @@ def test_fp32_kernel_variants_are_bit_identical(gpu, grid, steps, order):
         np.testing.assert_array_equal(np.isnan(arr), np.isnan(a0), err_msg=name)
         np.testing.assert_array_equal(np.nan_to_num(arr).view(np.uint32),
                                       np.nan_to_num(a0).view(np.uint32), err_msg=name)
+
+
+@pytest.mark.parametrize("radius", [2.3, float(np.nextafter(2.5, 0.0))])
+def test_exit_radius_float64_bit_exact_with_oracle(gpu, oracle_lib, radius):
+    """The local exit clamp (gpu_relative.py:126-131, 167-169) in FP64 local
+    arithmetic: bit for bit with the oracle's exit mode over 8 RK2 steps on
+    a 33-point grid.  2.3: the kernel test uses the literal 5.29, not
+    r * r = 5.289999999999999.  nextafter(2.5, 0): r * r < 6.25 = the
+    literal, so the nodes with x^2 + y^2 = 6.25 start at the exit value
+    (the initial plane compares with r * r) and are then updated (the kernel
+    compares with the literal)."""
+    grid = RelativeGrid(33, 12, 7, 7, 4.0)
+    ocfg = oracle_lib.HJI5DCfg(33, 12, 7, 7, 2, 16, 0, 4.0, 0.8, 0.3, -1.0, 2.0, -0.5, 1.0,
+                               3.0, 3.0, 1.0, 2.0, radius)
+    field, arrival = oracle_lib.hji5d_init(ocfg)
+    dt = oracle_lib.hji5d_cfl_limit(ocfg)
+    cfg = native_config(a3(0.3), grid, order=2, cfl=0.8, local_precision="float64",
+                        exit_radius=radius)
+    print(f"[gpu] exit radius {radius!r} float64 vs oracle", flush=True)
+    with _native.HJI5D(cfg) as h:
+        first, _ = h.read()
+        np.testing.assert_array_equal(u32(first), u32(field))
+        h.step(dt, 8)
+        gpu_field, gpu_arrival = h.read()
+    oracle_lib.hji5d_time_steps(ocfg, field, arrival, dt, 8)
+    np.testing.assert_array_equal(u32(gpu_field), u32(field))
+    np.testing.assert_array_equal(np.isnan(gpu_arrival), np.isnan(arrival))
+    x = -4.0 + np.arange(33) * (8.0 / 32)
+    d = (x[None, :] ** 2 + x[:, None] ** 2)
+    on_circle = d == 6.25
+    if 2.4 < radius < 2.5:
+        assert on_circle.any() and (first.reshape(-1, 33, 33)[:, on_circle] == 2.5).all()
+
+
+def test_exit_radius_float32_uses_float_arithmetic(gpu, oracle_lib):
+    """FP32 mode: the exit test is the reference's float one (x = -W +
+    ix * dx with W and dx = 2 W / (n - 1) in float, x*x + y*y > float
+    literal of r^2), checked against numpy float32 arithmetic on a grid
+    whose spacing is not a binary fraction; exit nodes hold the float
+    literal of r; elsewhere the field is within the FP32 tolerance of the
+    oracle's float64 exit mode (where the two exit sets agree)."""
+    n, extent, radius = 37, 3.7, 2.9
+    grid = RelativeGrid(n, 12, 7, 7, extent)
+    cfg = native_config(a3(0.3), grid, order=2, cfl=0.8, exit_radius=radius)
+    ocfg = oracle_lib.HJI5DCfg(n, 12, 7, 7, 2, 16, 0, extent, 0.8, 0.3, -1.0, 2.0, -0.5, 1.0,
+                               3.0, 3.0, 1.0, 2.0, radius)
+    field, arrival = oracle_lib.hji5d_init(ocfg)
+    dt = oracle_lib.hji5d_cfl_limit(ocfg)
+    print("[gpu] exit radius float32", flush=True)
+    with _native.HJI5D(cfg) as h:
+        h.step(dt, 6)
+        got, _ = h.read()
+    oracle_lib.hji5d_time_steps(ocfg, field, arrival, dt, 6)
+    W = np.float32(extent)
+    dx = (np.float32(2.0) * W) / np.float32(n - 1)
+    xf = -W + np.arange(n, dtype=np.float32) * dx
+    out32 = (xf[None, :] * xf[None, :] + xf[:, None] * xf[:, None]) > np.float32(
+        f"{radius * radius:.9e}")
+    x64 = -extent + np.arange(n) * (2 * extent / (n - 1))
+    out64 = (x64[None, :] ** 2 + x64[:, None] ** 2) > float(f"{radius * radius:.9e}")
+    g = got.reshape(-1, n, n)
+    assert (g[:, out32] == np.float32(f"{radius:.9e}")).all()
+    agree = out32 == out64
+    print(f"exit sets: float32 {out32.sum()}, float64 {out64.sum()}, differ at "
+          f"{np.argwhere(~agree).tolist()}")
+    inside = agree & ~out32
+    o = field.reshape(-1, n, n)
+    d = np.abs(g[:, inside].astype(np.float64) - o[:, inside])
+    assert d.max() <= 1e-3 and d.mean() <= 1e-5
+
+
+@pytest.mark.parametrize("feet", [None, "rect"])
+def test_two_plane_kernel_with_exit_radius(gpu, feet):
+    """The two-plane FP32 kernel (default for even heading counts) against
+    the one-plane kernel with the exit clamp, with and without rectangle
+    capture: fields and arrival times bit for bit."""
+    from pe_amd.local_game.geometry import Footprint
+
+    grid = RelativeGrid(17, 12, 7, 7, 4.0)
+    if feet:
+        config = ReversibleConfig(point_vehicles=False, capture_rule="rectangle_contact",
+                                  boundary_rule="collision_constrained")
+        fp = (Footprint(0.5, 0.3), Footprint(0.5, 0.3))
+    else:
+        config, fp = a3(0.3), None
+    out = {}
+    for name, flags in (("pair", 0), ("single", _native.PE_HJI5D_PSI_SINGLE)):
+        cfg = native_config(config, grid, exit_radius=3.0, capture_footprints=fp)
+        cfg.flags = flags
+        print(f"[gpu] exit radius variants {name} {feet}", flush=True)
+        with _native.HJI5D(cfg) as h:
+            h.step(h.stats()["cfl_dt_limit"], 18)
+            out[name] = h.read()
+    (f0, a0), (f1, a1) = out["pair"], out["single"]
+    np.testing.assert_array_equal(u32(f0), u32(f1))
+    np.testing.assert_array_equal(np.isnan(a0), np.isnan(a1))
+    np.testing.assert_array_equal(u32(np.nan_to_num(a0)), u32(np.nan_to_num(a1)))

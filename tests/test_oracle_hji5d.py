@@ -248,3 +248,48 @@ def test_float64_storage_close_to_float32_storage(oracle_lib):
                                  case["horizon"], case["output_dt"], history=False)
     d = np.abs(r64["final"] - z["final"].astype(np.float64))
     assert d.max() < 1e-3 and d.mean() < 1e-6
+
+
+def _exit_cfg(oracle, exit_radius, nx=33, extent=4.0, threads=4):
+    return oracle.HJI5DCfg(nx, 12, 7, 7, 2, threads, 0, extent, 0.8, 0.3,
+                           -1.0, 2.0, -0.5, 1.0, 3.0, 3.0, 1.0, 2.0, exit_radius)
+
+
+def test_exit_radius_mode_known_answers_and_locality(oracle_lib):
+    """The oracle's local-exit mode (gpu_relative.py:126-131, float64):
+    nodes beyond the decimal literal of r^2 hold the literal of r at every
+    step (count from numpy, float64); the test compares with the literal,
+    not r * r (2.3: r * r = 5.289999999999999, literal 5.29); nodes whose
+    +-2 stencil touches no exit node evolve exactly as without the exit
+    clamp."""
+    r = 2.3
+    lit_r2, lit_r = float(f"{r * r:.9e}"), float(f"{r:.9e}")
+    assert lit_r2 != r * r
+    cfg = _exit_cfg(oracle_lib, r)
+    base = _exit_cfg(oracle_lib, 0.0)
+    out = oracle_lib.hji5d_solve(cfg, 0.1, 0.05)
+    ref = oracle_lib.hji5d_solve(base, 0.1, 0.05)
+    n = 33
+    x = -4.0 + np.arange(n) * (8.0 / (n - 1))
+    xx, yy = np.meshgrid(x, x)
+    outside = xx * xx + yy * yy > lit_r2
+    init_outside = xx * xx + yy * yy > r * r
+    f = out["final"].reshape(7, 7, 12, n, n)
+    assert (f[..., outside] == np.float32(lit_r)).all()
+    frame0 = out["fields"][0].reshape(7, 7, 12, n, n)
+    assert (frame0[..., init_outside] == np.float32(r)).all()
+    # one RK2 step (two stages reach 4 nodes in x and y) with and without
+    # the clamp: equal wherever no exit node lies within 4 nodes
+    f1, a1 = oracle_lib.hji5d_init(cfg)
+    f2, a2 = oracle_lib.hji5d_init(base)
+    dt = oracle_lib.hji5d_cfl_limit(cfg)
+    oracle_lib.hji5d_time_steps(cfg, f1, a1, dt, 1)
+    oracle_lib.hji5d_time_steps(base, f2, a2, dt, 1)
+    from scipy.ndimage import binary_dilation
+
+    far = ~binary_dilation(outside, np.ones((9, 9), bool))
+    assert far.sum() > 30
+    f1, f2 = f1.reshape(7, 7, 12, n, n), f2.reshape(7, 7, 12, n, n)
+    np.testing.assert_array_equal(f1[..., far].view(np.uint32), f2[..., far].view(np.uint32))
+    assert (f1[..., outside] == np.float32(lit_r)).all()
+    assert (f <= frame0 + 1e-6).all()
