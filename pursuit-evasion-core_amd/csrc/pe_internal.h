@@ -107,6 +107,12 @@ struct pe_mlp {
   __bf16 *F = nullptr;
   std::vector<__bf16 *> Hs, dZ;  // per hidden layer
   std::vector<__bf16 *> Zs;  // pre-activations (bf16), kept for the SiLU derivative
+  // transposed copies [width, ldt] of F, Hs and dZ, written by the kernels
+  // that produce them during training: the operands of the weight-gradient
+  // GEMMs, which sum over the batch, then sit k-contiguous
+  __bf16 *FT = nullptr;
+  std::vector<__bf16 *> HT, dZT;
+  long long ldt = 0;         // cap rounded up to a multiple of 8
   // dataset
   float *data_x = nullptr, *data_y = nullptr;
   long long data_n = 0;

@@ -13,15 +13,25 @@
 // transposed BF16 copies [in, out] for the MFMA operands.
 //
 // GEMM: C[m, n] = sum_k A(m, k) B(k, n) with arbitrary strides, one
-// 64 x 64 output tile per 256-thread block; each wave owns a 32 x 32 quarter
-// as 2 x 2 tiles of v_mfma_f32_16x16x32_bf16.  K advances 32 at a time
-// through LDS, where both operands are stored k-contiguous so that each
-// lane's MFMA operand (8 consecutive k) is one 16-byte LDS read.
+// BM x BN output tile per 256-thread block (128 x 128, 128 x 64 or 64 x 64:
+// the largest that still gives 256 blocks); the four waves split it 2 x 2
+// and each owns its quarter as 16 x 16 tiles of v_mfma_f32_16x16x32_bf16.
+// K advances 64 at a time through double-buffered LDS: the next stage's
+// global loads are issued before the current stage's MFMAs and written to
+// the other buffer after them, one barrier per stage.  Both operands sit
+// k-contiguous in LDS, so each lane's MFMA operand (8 consecutive k) is one
+// 16-byte LDS read.  The epilogues (bias + SiLU, SiLU derivative, plain or
+// accumulated store) are fused and store 4 consecutive outputs per lane.
+// Weight gradients are split over k (at most 64 chunks of at least 256
+// rows of the batch, each storing a partial tile) and the partials summed in
+// a fixed order, so the training step is deterministic.
 //
 // MFMA 16x16x32 bf16 operand layout (wave64), lane l:
 //   A fragment: row (l % 16), k = 8 * (l / 16) .. +7
 //   B fragment: col (l % 16), k = 8 * (l / 16) .. +7
 //   D fragment: col (l % 16), rows 4 * (l / 16) .. +3
+// The kernel passes the B fragment as the first MFMA operand, which
+// transposes D: lane l then holds row (l % 16), cols 4 * (l / 16) .. +3.
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -63,6 +73,9 @@ struct GemmArgs {
   __bf16 *Z;           // pre-activations [M, N] in bf16: kBiasSilu writes
                        // them (if not null), kSiluGrad reads them
   long long ldz;
+  __bf16 *HT;          // kBiasSilu / kSiluGrad: the bf16 output also stored
+                       // transposed, HT[n * ldht + m] (if not null)
+  long long ldht;
   int kchunk;          // > 0: block z sums k in [z * kchunk, (z + 1) * kchunk)
                        // and stores (kStoreF32, beta 0) at C + z * cstride
   long long cstride;
@@ -197,6 +210,12 @@ __global__ __launch_bounds__(256) void mfma_gemm(GemmArgs g) {
     __syncthreads();
     buf ^= 1;
   }
+  // The transposed output goes through LDS, which the K loop has finished
+  // with (its last barrier): each wave's (BN/2) x (BM/2) block of HT is
+  // staged m-contiguous in its own region, then stored 16 bytes per lane.
+  constexpr int kTPad = BM / 2 + 8;
+  static_assert(4 * (BN / 2) * kTPad <= 2 * BM * (kK + kLdsPad), "staging fits in As");
+  __bf16(*Tt)[kTPad] = reinterpret_cast<__bf16(*)[kTPad]>(&As[0][0][0]) + wave * (BN / 2);
   // Epilogue: lane holds C[m][n .. n + 3] of each tile, m = lane % 16 and
   // n = 4 * (lane / 16) within the tile.
 #pragma unroll
@@ -219,51 +238,84 @@ __global__ __launch_bounds__(256) void mfma_gemm(GemmArgs g) {
           for (int v = 0; v < 4 && n + v < g.N; v++)
             c[v] = g.beta != 0.f ? x[v] + g.beta * c[v] : x[v];
         }
-      } else if (g.epilogue == kBiasSilu) {
-        if (whole && (g.ldz % 4) == 0 && (g.ldh % 4) == 0) {
-          const f32x4 z = x + *reinterpret_cast<const f32x4 *>(g.bias + n);
-          bf16x4 hv, zv;
+      } else {
+        // bias + SiLU (Z kept if asked) or the SiLU derivative (Z read)
+        const bool vec = whole && (g.ldz % 4) == 0 && (g.ldh % 4) == 0;
+        bf16x4 hv, zv;
+        if (g.epilogue == kBiasSilu) {
 #pragma unroll
           for (int v = 0; v < 4; v++) {
-            hv[v] = to_bf16(silu(z[v]));
-            zv[v] = to_bf16(z[v]);
+            const float z = n + v < g.N ? x[v] + g.bias[n + v] : 0.f;
+            hv[v] = to_bf16(silu(z));
+            zv[v] = to_bf16(z);
           }
-          if (g.Z) *reinterpret_cast<bf16x4 *>(g.Z + m * g.ldz + n) = zv;
-          *reinterpret_cast<bf16x4 *>(g.H + m * g.ldh + n) = hv;
-        } else {
+          if (g.Z) {
+            if (vec) {
+              *reinterpret_cast<bf16x4 *>(g.Z + m * g.ldz + n) = zv;
+            } else {
 #pragma unroll
-          for (int v = 0; v < 4 && n + v < g.N; v++) {
-            const float z = x[v] + g.bias[n + v];
-            if (g.Z) g.Z[m * g.ldz + n + v] = to_bf16(z);
-            g.H[m * g.ldh + n + v] = to_bf16(silu(z));
+              for (int v = 0; v < 4; v++)
+                if (n + v < g.N) g.Z[m * g.ldz + n + v] = zv[v];
+            }
           }
+        } else {
+          if (vec) {
+            zv = *reinterpret_cast<const bf16x4 *>(g.Z + m * g.ldz + n);
+          } else {
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+              zv[v] = n + v < g.N ? g.Z[m * g.ldz + n + v] : to_bf16(0.f);
+          }
+#pragma unroll
+          for (int v = 0; v < 4; v++) hv[v] = to_bf16(x[v] * silu_grad((float)zv[v]));
         }
-      } else {
-        if (whole && (g.ldz % 4) == 0 && (g.ldh % 4) == 0) {
-          const bf16x4 z = *reinterpret_cast<const bf16x4 *>(g.Z + m * g.ldz + n);
-          bf16x4 hv;
-#pragma unroll
-          for (int v = 0; v < 4; v++) hv[v] = to_bf16(x[v] * silu_grad((float)z[v]));
+        if (vec) {
           *reinterpret_cast<bf16x4 *>(g.H + m * g.ldh + n) = hv;
         } else {
 #pragma unroll
-          for (int v = 0; v < 4 && n + v < g.N; v++)
-            g.H[m * g.ldh + n + v] =
-                to_bf16(x[v] * silu_grad((float)g.Z[m * g.ldz + n + v]));
+          for (int v = 0; v < 4; v++)
+            if (n + v < g.N) g.H[m * g.ldh + n + v] = hv[v];
+        }
+        if (g.HT) {
+#pragma unroll
+          for (int v = 0; v < 4; v++)
+            Tt[16 * j + 4 * (lane >> 4) + v][16 * i + (lane & 15)] = hv[v];
         }
       }
     }
+  if (g.HT && g.epilogue != kStoreF32) {  // uniform over the block
+    __syncthreads();
+    // 8 lanes per row of BM/2 values (8 each), 64 / (BM/16) rows per pass
+    constexpr int kLanesPerRow = BM / 16, kRowsPerPass = 64 / kLanesPerRow;
+    const int r0 = lane / kLanesPerRow, c = (lane % kLanesPerRow) * 8;
+    const int m = m0 + wm + c;
+#pragma unroll
+    for (int r = r0; r < BN / 2; r += kRowsPerPass) {
+      const int n = n0 + wn + r;
+      if (n >= g.N) continue;
+      __bf16 *dst = g.HT + (long long)n * g.ldht + m;
+      if (m + 8 <= g.M && (g.ldht % 8) == 0) {
+        *reinterpret_cast<bf16x8 *>(dst) = *reinterpret_cast<const bf16x8 *>(&Tt[r][c]);
+      } else {
+#pragma unroll
+        for (int t = 0; t < 8; t++)
+          if (m + t < g.M) dst[t] = Tt[r][c + t];
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------------- features
 
 
 
-// xn = (x - shift) * scale for the 8 state coordinates; features into F.
+// xn = (x - shift) * scale for the 8 state coordinates; features into F
+// and, if FT is not null, transposed into FT[c * ldt + i].
 __global__ __launch_bounds__(256) void mlp_features(
     const float *__restrict__ raw, long long n, MlpShape s,
     const float *__restrict__ shift, const float *__restrict__ scale,
-    const float *__restrict__ proj /* [rff/2, 8] */, __bf16 *__restrict__ F) {
+    const float *__restrict__ proj /* [rff/2, 8] */, __bf16 *__restrict__ F,
+    __bf16 *__restrict__ FT, long long ldt) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float *x = raw + i * s.raw;
@@ -290,6 +342,8 @@ __global__ __launch_bounds__(256) void mlp_features(
   f[c++] = to_bf16(__sinf(x[6] - x[2]));
   f[c++] = to_bf16(sqrtf(dx * dx + dy * dy));
   for (; c < s.width; c++) f[c] = to_bf16(0.f);
+  if (FT)  // consecutive threads write consecutive i of each feature row
+    for (c = 0; c < s.width; c++) FT[c * ldt + i] = f[c];
 }
 
 // y[i] = w . h[i] + b  (final 256 -> 1 layer); one wave per point.
@@ -329,18 +383,38 @@ __global__ __launch_bounds__(256) void mlp_loss(const float *__restrict__ yhat,
   if ((threadIdx.x & 63) == 0 && l != 0.f) atomicAdd(loss, l);
 }
 
-// Head backward, one thread per (row, unit): dZ3[i, k] = dy_i w4[k]
-// silu'(Z3[i, k]) (bf16).  The head's weight and bias gradients are column
-// sums (column_partials with weights dy).
+// Head backward: dZ3[i, k] = dy_i w4[k] silu'(Z3[i, k]) (bf16), stored row
+// by row into dZ and transposed into dZT[k * ldt + i].  One 256-thread block
+// per 64 x 64 tile: rows of the tile are written from registers, the
+// transposed rows through a padded LDS tile, both coalesced.  The head's
+// weight and bias gradients are column sums (column_partials with weights
+// dy).
 __global__ __launch_bounds__(256) void mlp_head_back(const __bf16 *__restrict__ Z, long long n,
                                                      int width, const float *__restrict__ w,
                                                      const float *__restrict__ dy,
-                                                     __bf16 *__restrict__ dZ) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n * width) return;
-  const long long i = t / width;
-  const int k = (int)(t - i * width);
-  dZ[t] = to_bf16(dy[i] * w[k] * silu_grad((float)Z[t]));
+                                                     __bf16 *__restrict__ dZ,
+                                                     __bf16 *__restrict__ dZT, long long ldt) {
+  __shared__ __bf16 tile[64][65];
+  const long long i0 = (long long)blockIdx.x * 64;
+  const int k0 = blockIdx.y * 64;
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    const long long i = i0 + r;
+    const int k = k0 + c;
+    __bf16 v = to_bf16(0.f);
+    if (i < n && k < width) {
+      v = to_bf16(dy[i] * w[k] * silu_grad((float)Z[i * width + k]));
+      dZ[i * width + k] = v;
+    }
+    tile[c][r] = v;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int c = e >> 6, r = e & 63;
+    const long long i = i0 + r;
+    const int k = k0 + c;
+    if (i < n && k < width) dZT[k * ldt + i] = tile[c][r];
+  }
 }
 
 // Column sums of a row-major [n, width] matrix over chunks of `rows` rows:
@@ -529,7 +603,8 @@ static int column_sums(pe_mlp *h, const __bf16 *X, const float *wt, long long n,
 static int forward(pe_mlp *h, long long n, bool keep) {
   const MlpShape &s = h->shape;
   hipLaunchKernelGGL(mlp_features, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                     h->stream, h->raw, n, s, h->shift, h->scale, h->proj, h->F);
+                     h->stream, h->raw, n, s, h->shift, h->scale, h->proj, h->F,
+                     keep ? h->FT : nullptr, h->ldt);
   PE_LAUNCHED(mlp_features);
   const __bf16 *x = h->F;
   int width = s.width;
@@ -550,6 +625,9 @@ static int forward(pe_mlp *h, long long n, bool keep) {
     g.bias = h->b[l];
     g.H = h->Hs[l];
     g.ldh = h->fan_out[l];
+    // the next layer's input, transposed for its weight gradient
+    g.HT = keep && l + 1 < h->layers ? h->HT[l] : nullptr;
+    g.ldht = h->ldt;
     int rc = launch_gemm(h, g);
     if (rc) return rc;
     x = h->Hs[l];
@@ -573,20 +651,20 @@ static int backward(pe_mlp *h, long long n, float *loss) {
   const int top = h->fan_out[L - 1];
   if ((rc = column_sums(h, h->Hs[L - 1], h->dy, n, top, h->gw_head))) return rc;
   if ((rc = column_sums(h, nullptr, h->dy, n, 1, h->gb_head))) return rc;
-  hipLaunchKernelGGL(mlp_head_back, dim3((unsigned)((n * top + 255) / 256)), dim3(256), 0,
-                     h->stream, h->Zs[L - 1], n, top, h->w_head, h->dy, h->dZ[L - 1]);
+  hipLaunchKernelGGL(mlp_head_back, dim3((unsigned)((n + 63) / 64), (unsigned)((top + 63) / 64)),
+                     dim3(256), 0, h->stream, h->Zs[L - 1], n, top, h->w_head, h->dy,
+                     h->dZ[L - 1], h->dZT[L - 1], h->ldt);
   PE_LAUNCHED(mlp_head_back);
   for (int l = L - 1; l >= 0; l--) {
-    const __bf16 *input = l == 0 ? h->F : h->Hs[l - 1];
-    const int in_w = l == 0 ? h->shape.width : h->fan_out[l - 1];
-    // gW[o, i] = sum_m dZ[m, o] X[m, i]: A(o, m) = dZ[m*out + o], B(m, i) = X[m*in_w + i]
+    // gW[o, i] = sum_m dZ[m, o] X[m, i] over the transposed copies:
+    // A(o, m) = dZT[o * ldt + m], B(m, i) = XT[i * ldt + m] (both k-contiguous)
     GemmArgs g{};
-    g.A = h->dZ[l];
-    g.sam = 1;
-    g.sak = h->fan_out[l];
-    g.B = input;
-    g.sbk = in_w;
-    g.sbn = 1;
+    g.A = h->dZT[l];
+    g.sam = h->ldt;
+    g.sak = 1;
+    g.B = l == 0 ? h->FT : h->HT[l - 1];
+    g.sbk = 1;
+    g.sbn = h->ldt;
     g.M = h->fan_out[l];
     g.N = h->fan_in[l];
     g.K = (int)n;
@@ -621,6 +699,8 @@ static int backward(pe_mlp *h, long long n, float *loss) {
       d.ldh = h->fan_in[l];
       d.Z = h->Zs[l - 1];
       d.ldz = h->fan_in[l];
+      d.HT = h->dZT[l - 1];
+      d.ldht = h->ldt;
       if ((rc = launch_gemm(h, d))) return rc;
     }
   }
@@ -702,11 +782,15 @@ static int ensure_capacity(pe_mlp *h, long long n) {
   if (n <= h->cap) return PE_OK;
   auto free_all = [&]() {
     for (void *p : {(void *)h->raw, (void *)h->yt, (void *)h->yhat, (void *)h->dy, (void *)h->F,
-                    (void *)h->partial})
+                    (void *)h->partial, (void *)h->FT})
       if (p) (void)hipFree(p);
-    for (auto *p : h->Hs) if (p) (void)hipFree(p);
-    for (auto *p : h->dZ) if (p) (void)hipFree(p);
-    for (auto *p : h->Zs) if (p) (void)hipFree(p);
+    for (auto *v : {&h->Hs, &h->dZ, &h->Zs, &h->HT, &h->dZT})
+      for (auto *&p : *v) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+      }
+    h->raw = h->yt = h->yhat = h->dy = h->partial = nullptr;
+    h->F = h->FT = nullptr;
   };
   free_all();
   h->cap = 0;
@@ -716,6 +800,8 @@ static int ensure_capacity(pe_mlp *h, long long n) {
   PE_HIP(hipMalloc(&h->yhat, sizeof(float) * cap));
   PE_HIP(hipMalloc(&h->dy, sizeof(float) * cap));
   PE_HIP(hipMalloc(&h->F, sizeof(__bf16) * cap * h->shape.width));
+  const long long ldt = (cap + 7) / 8 * 8;
+  PE_HIP(hipMalloc(&h->FT, sizeof(__bf16) * ldt * h->shape.width));
   {
     long long widest = 1, largest = 1;
     for (int l = 0; l < h->layers; l++) {
@@ -731,7 +817,10 @@ static int ensure_capacity(pe_mlp *h, long long n) {
     PE_HIP(hipMalloc(&h->Hs[l], sizeof(__bf16) * cap * h->fan_out[l]));
     PE_HIP(hipMalloc(&h->dZ[l], sizeof(__bf16) * cap * h->fan_out[l]));
     PE_HIP(hipMalloc(&h->Zs[l], sizeof(__bf16) * cap * h->fan_out[l]));
+    PE_HIP(hipMalloc(&h->HT[l], sizeof(__bf16) * ldt * h->fan_out[l]));
+    PE_HIP(hipMalloc(&h->dZT[l], sizeof(__bf16) * ldt * h->fan_out[l]));
   }
+  h->ldt = ldt;
   h->cap = cap;
   return PE_OK;
 }
@@ -770,6 +859,7 @@ extern "C" int pe_mlp_create(const pe_mlp_config *cfg, pe_mlp **out) {
   h->mW.resize(c.layers); h->vW.resize(c.layers); h->mb.resize(c.layers); h->vb.resize(c.layers);
   h->Wh.resize(c.layers); h->Wt.resize(c.layers);
   h->Hs.resize(c.layers); h->dZ.resize(c.layers); h->Zs.resize(c.layers);
+  h->HT.resize(c.layers); h->dZT.resize(c.layers);
   auto body = [&]() -> int {
     PE_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     std::mt19937_64 rng((unsigned long long)c.seed);
@@ -894,7 +984,8 @@ extern "C" int pe_mlp_features(pe_mlp *h, const float *raw, int64_t n, float *fe
   PE_HIP(hipMemcpyAsync(h->raw, raw, sizeof(float) * n * h->shape.raw, hipMemcpyHostToDevice,
                         h->stream));
   hipLaunchKernelGGL(mlp_features, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream,
-                     h->raw, (long long)n, h->shape, h->shift, h->scale, h->proj, h->F);
+                     h->raw, (long long)n, h->shape, h->shift, h->scale, h->proj, h->F,
+                     nullptr, 0ll);
   PE_LAUNCHED(mlp_features);
   std::vector<__bf16> f((size_t)n * h->shape.width);
   PE_HIP(hipMemcpyAsync(f.data(), h->F, f.size() * 2, hipMemcpyDeviceToHost, h->stream));
@@ -1032,14 +1123,14 @@ extern "C" void pe_mlp_destroy(pe_mlp *h) {
     for (void *p : {(void *)h->W[l], (void *)h->b[l], (void *)h->gW[l], (void *)h->gb[l],
                     (void *)h->mW[l], (void *)h->vW[l], (void *)h->mb[l], (void *)h->vb[l],
                     (void *)h->Wh[l], (void *)h->Wt[l], (void *)h->Hs[l], (void *)h->dZ[l],
-                    (void *)h->Zs[l]})
+                    (void *)h->Zs[l], (void *)h->HT[l], (void *)h->dZT[l]})
       f(p);
   }
   for (void *p : {(void *)h->w_head, (void *)h->b_head, (void *)h->gw_head, (void *)h->gb_head,
                   (void *)h->mw_head, (void *)h->vw_head, (void *)h->mb_head, (void *)h->vb_head,
                   (void *)h->shift, (void *)h->scale, (void *)h->proj, (void *)h->raw,
                   (void *)h->yt, (void *)h->yhat, (void *)h->dy, (void *)h->loss, (void *)h->F,
-                  (void *)h->partial,
+                  (void *)h->FT, (void *)h->partial,
                   (void *)h->data_x, (void *)h->data_y})
     f(p);
   if (h->stream) (void)hipStreamDestroy(h->stream);

@@ -100,6 +100,49 @@ def test_reader_matches_solver_labels(gpu, solved):
     ref.close()
 
 
+def test_reader_continuous_queries_match_product_formula(gpu, oracle_lib, solved):
+    """ContactReference.fields (reference.py:405-431) at the solve's
+    continuous query states, against a restatement in numpy: both cars'
+    stencils from the oracle, the 16 x 16 product weights, the stored
+    float16 value of each kept pair and the exit value for the others,
+    summed in float64; then the payoff clip, the exit value outside the
+    radius and the payoff at budget 0.  The reader sums in float32 on the
+    GPU: agreement to 1e-6 at every budget."""
+    out, _ = solved
+    ref = ContactReference(out)
+    q = np.load(out / "queries.npz")["states"].reshape(-1, 8)
+    got, ok, _ = ref.fields(q)
+
+    def stencil(g, s):
+        geo = g.geometry
+        ids, w, valid, _, _, _ = oracle_lib.car_stencil(g.axes, g.dense_to_compact, s,
+                                                        geo.obstacles, geo.arena.half_width,
+                                                        geo.params)
+        return np.maximum(ids, 0), w.astype(np.float64), valid
+
+    pi, pw, pv = stencil(ref.p, q[:, :4])
+    ei, ew, ev = stencil(ref.e, q[:, 4:])
+    d = ref.domain
+    rank = d.lookup[d.p_position[pi][:, :, None], ei[:, None, :]]
+    packed = d.offsets[pi][:, :, None] + np.maximum(rank, 0)
+    history = np.asarray(ref.values, np.float64)
+    terms = np.where(rank[None] >= 0, history[:, packed], d.exit_radius)
+    want = np.einsum("na,nb,tnab->tn", pw, ew, terms)
+    phi = ref.arena.capture_value(q)
+    want = np.minimum(want, phi[None])
+    inside = np.linalg.norm(q[:, :2] - q[:, 4:6], axis=1) <= d.exit_radius
+    want = np.where(inside[None], want, d.exit_radius)
+    valid = pv & ev
+    want[0, valid & inside] = phi[valid & inside]
+    np.testing.assert_array_equal(ok, valid & inside)
+    assert np.isnan(got[:, ~valid]).all()
+    diff = np.abs(got[:, valid] - want[:, valid])
+    print(f"reader vs product formula at {valid.sum()} continuous queries, "
+          f"{len(ref.times)} budgets: max|dV|={diff.max():.2e}")
+    assert valid.sum() > 0 and diff.max() <= 1e-6
+    ref.close()
+
+
 def test_resume_after_interruption_equals_uninterrupted_solve(gpu, tmp_path, monkeypatch):
     """(reference.py:289-348) A solve interrupted right after the checkpoint
     of budget 0.5 s (host-side exception in the record step), then re-run

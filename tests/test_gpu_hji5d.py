@@ -170,6 +170,11 @@ def test_c1_full_size_properties(gpu, oracle_lib):
     print(f"C1 float32 vs float64 local arithmetic after 810 steps: max|dV|={d.max():.3e} "
           f"mean|dV|={d.mean():.3e}; capture-set flips {len(flips)} "
           f"(|V64| max {np.abs(fields['float64'][flips]).max() if len(flips) else 0:.2e})")
+    # every flip by node: storage index (vP, vE, psi, y, x), both values
+    for flat in flips:
+        node = np.unravel_index(flat, C1.shape)
+        print(f"  flip {tuple(int(i) for i in node)}: V32={fields['float32'][flat]:+.3e} "
+              f"V64={fields['float64'][flat]:+.3e}")
     # The FP32 error is bounded by the scheme's own tie-breaking ambiguity
     # (the FP64 field's mirror asymmetry) and is small on average.
     assert d.max() <= 2 * asym_max["float64"] + 1e-3 and d.mean() <= 1e-4
@@ -364,6 +369,40 @@ def test_exit_radius_float32_uses_float_arithmetic(gpu, oracle_lib):
     o = field.reshape(-1, n, n)
     d = np.abs(g[:, inside].astype(np.float64) - o[:, inside])
     assert d.max() <= 1e-3 and d.mean() <= 1e-5
+
+
+def test_rectangle_capture_float64_bit_exact_with_oracle(gpu, oracle_lib):
+    """Rectangle-contact capture (gpu_relative.py:156-160) changes only the
+    initial plane, whose closed form the initialisation test checks; the
+    time steps do not read it again.  From the GPU's own initial field,
+    8 FP64 RK2 steps with the exit clamp at r = 2.9 equal the oracle's
+    steps bit for bit, fields and arrival times."""
+    from pe_amd.local_game.geometry import Footprint
+
+    grid = RelativeGrid(33, 12, 7, 7, 4.0)
+    rect = ReversibleConfig(point_vehicles=False, capture_rule="rectangle_contact",
+                            boundary_rule="collision_constrained")
+    feet = (Footprint(0.8, 0.5), Footprint(0.4, 0.25))
+    cfg = native_config(rect, grid, order=2, cfl=0.8, local_precision="float64",
+                        exit_radius=2.9, capture_footprints=feet)
+    ocfg = oracle_lib.HJI5DCfg(
+        cfg.positions, cfg.headings, cfg.p_speeds, cfg.e_speeds, cfg.order, 16, 0,
+        cfg.extent, cfg.cfl, cfg.radius, cfg.p_min, cfg.p_max, cfg.e_min, cfg.e_max,
+        cfg.p_acceleration, cfg.e_acceleration, cfg.p_curvature, cfg.e_curvature, 2.9)
+    dt = oracle_lib.hji5d_cfl_limit(ocfg)
+    print("[gpu] rectangle capture + exit radius 2.9, float64 vs oracle", flush=True)
+    with _native.HJI5D(cfg) as h:
+        assert h.stats()["cfl_dt_limit"] == pytest.approx(dt, rel=1e-14)
+        phi, arrival0 = h.read()
+        h.step(dt, 8)
+        gpu_field, gpu_arrival = h.read()
+    field, arrival = phi.copy(), arrival0.copy()
+    np.testing.assert_array_equal(np.isnan(arrival), phi > 0)
+    oracle_lib.hji5d_time_steps(ocfg, field, arrival, dt, 8)
+    assert (field != phi).any()
+    np.testing.assert_array_equal(u32(gpu_field), u32(field))
+    np.testing.assert_array_equal(np.isnan(gpu_arrival), np.isnan(arrival))
+    np.testing.assert_array_equal(u32(np.nan_to_num(gpu_arrival)), u32(np.nan_to_num(arrival)))
 
 
 @pytest.mark.parametrize("feet", [None, "rect"])

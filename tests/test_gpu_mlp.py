@@ -70,11 +70,13 @@ def torch_reference(net, feats, y, under_weight=1.0):
     return out.detach().numpy(), loss.item(), [t.grad.numpy().ravel() for t in ts]
 
 
-@pytest.mark.parametrize("under_weight", [1.0, 4.0])
-def test_forward_and_gradients_match_float64_autograd(gpu, under_weight):
+@pytest.mark.parametrize("under_weight,n", [(1.0, 512), (4.0, 512), (1.0, 333)])
+def test_forward_and_gradients_match_float64_autograd(gpu, under_weight, n):
+    """n = 333: a batch that is no multiple of 8 or of a GEMM tile, so the
+    transposed copies of the weight-gradient GEMMs end in partial rows."""
     net = make_net(under_weight=under_weight)
-    x, y = random_inputs(512, 1)
-    print(f"[gpu] mlp forward/backward under_weight={under_weight}", flush=True)
+    x, y = random_inputs(n, 1)
+    print(f"[gpu] mlp forward/backward under_weight={under_weight} n={n}", flush=True)
     feats = net.features(x)
     pred = net(x)
     ref_out, ref_loss, ref_grads = torch_reference(net, feats, y, under_weight)

