@@ -1,6 +1,7 @@
 // FP32 RK stage of the 5D relative HJI update, written for instruction
-// economy on gfx950.  The stage is VALU-issue bound (packed FP32 is not
-// used), so the cost of a node is its VALU instruction count.
+// economy on gfx950: the cost of a node is the vector instructions it
+// issues, its neighbour loads first, then its VALU (packed FP32 is not
+// used).
 //
 // Measured on C1 (34.2M nodes), blend / stage-1 launch:
 //   generic template hji5d_stage<float, false, *>            232 / 195 us
