@@ -237,7 +237,9 @@ def run_reader_c1(cfg, n_queries: int = 1_000_000, cpu_queries: int = 20_000):
         import rtable_oracle
 
         t0 = time.perf_counter()
-        rtable_oracle.field_np(hist, times, axes, states[:cpu_queries], budgets[:cpu_queries])
+        for k in range(0, cpu_queries, 250_000):  # bounded host memory
+            end = min(cpu_queries, k + 250_000)
+            rtable_oracle.field_np(hist, times, axes, states[k:end], budgets[k:end])
         cpu_s = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": cpu_queries / cpu_s, "unit": "field queries/s",
                                "cores": 1, "kind": "port",
@@ -479,7 +481,7 @@ def run_c4(args):
                        map_sweeps_to_residual={"min": min(stops), "median": int(np.median(stops)),
                                                "max": max(stops), "total": int(sum(stops))})
     if getattr(args, "cpu_sweeps", 0):
-        # cpu_baseline leg: the oracle's FP64 OpenMP sweep of the first two
+        # cpu_baseline leg: the oracle's FP64 OpenMP sweep of the first eight
         # maps from the payoff, in legal-pair updates per second.
         import numpy as np
         import oracle
@@ -487,7 +489,7 @@ def run_c4(args):
 
         threads = host_cores()[0]
         pairs = seconds = 0.0
-        for _, arena, p, e, _ in games[:2]:
+        for _, arena, p, e, _ in games[:8]:
             inside = dense_inside(p, e, 3.0)
             V = oracle.sl_init(p.states, e.states, inside, 3.0, arena.pursuer, arena.evader,
                                threads=threads)
@@ -498,8 +500,8 @@ def run_c4(args):
             pairs += len(p.states) * len(e.states) * args.cpu_sweeps
         out["cpu_baseline"] = {"value": pairs / seconds, "unit": "legal-pair updates/s",
                                "cores": threads, "kind": "port",
-                               "sample": f"{args.cpu_sweeps} FP64 Jacobi sweeps of maps 1000 "
-                                         f"and 1001 from the payoff ({seconds:.2f} s, OpenMP)",
+                               "sample": f"{args.cpu_sweeps} FP64 Jacobi sweeps of each of maps "
+                                         f"1000-1007 from the payoff ({seconds:.2f} s, OpenMP)",
                                "host": cores_note()}
     return out
 
@@ -530,7 +532,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--profile-steps", type=int, default=100)
-    ap.add_argument("--cpu-sample-steps", type=int, default=16)
+    ap.add_argument("--cpu-sample-steps", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true",
                     help="skip the 8D contact-game line after the 5D headline")
@@ -667,7 +669,7 @@ def main() -> int:
                   flush=True)
             c4 = argparse.Namespace(maps=128, warmup=10, steps=100, to_residual=1e-4,
                                     max_sweeps=20000, no_freeze=False,
-                                    cpu_sweeps=0 if args.no_cpu_baseline else 4)
+                                    cpu_sweeps=0 if args.no_cpu_baseline else 100)
             secondary["contact_game_c4"] = run_c4(c4)
         if not args.no_learned:
             print("[bench] secondary: learned warm start vs plain DP", file=sys.stderr,
@@ -683,11 +685,11 @@ def main() -> int:
                     "sweeps": cold["to_vstar"], "to_residual_1e-4": cold["to_residual"]}
         print("[bench] secondary: dense 8D square game", file=sys.stderr, flush=True)
         secondary["world8d_default"] = run_world8d(
-            (9, 8, 7, 7), 40, 2, 0 if args.no_cpu_baseline else 2)
+            (9, 8, 7, 7), 40, 2, 0 if args.no_cpu_baseline else 40)
         secondary["world8d_13x12x7x7"] = run_world8d((13, 12, 7, 7), 20, 2)
         print("[bench] secondary: 5D table reader", file=sys.stderr, flush=True)
         secondary["reader_c1"] = run_reader_c1(cfg, cpu_queries=0 if args.no_cpu_baseline
-                                               else 20_000)
+                                               else 1_000_000)
         out["secondary"] = secondary
     print(json.dumps(out), flush=True)
     return 0

@@ -180,3 +180,28 @@ def test_c4_converged_fp32_vs_fp64(gpu, c4_games):
               flush=True)
         assert d.max() <= 1e-3
         assert (np.abs(v64.ravel()[flip]) <= 1e-3).all()
+
+
+def test_c4_full_batch_equals_maps_swept_alone(gpu):
+    """The C4 configuration as batched: all 128 maps (seeds 1000-1127) in one
+    handle, 24 graph-replayed sweeps, then 24 more with converged maps
+    frozen at a loose tolerance.  Three maps (first, middle, last) swept
+    alone the same way give the same fields bit for bit, and the same
+    residual trace while every map is still swept."""
+    seeds = list(range(1000, 1128))
+    print("[gpu] C4 128-map batch: tables", flush=True)
+    games = benchmark_games(seeds, **C4)
+    flags = EARLY | ORDERED | GRAPH
+    with GameSolver([g[4] for g in games]) as batch:
+        assert batch.stats()["maps"] == 128
+        batch.sweep(24, flags)
+        trace = batch.trace()
+        done, _ = batch.run(2e-2, 24, 8, flags | _native.PE_SL_FREEZE)
+        fields = {k: batch.read(k) for k in (0, 63, 127)}
+    print(f"[gpu] C4 128-map batch: {done} frozen-run sweeps", flush=True)
+    for k, f in fields.items():
+        with GameSolver(games[k][4]) as alone:
+            alone.sweep(24, flags)
+            np.testing.assert_array_equal(alone.trace()[:, 0], trace[:, k])
+            alone.run(2e-2, 24, 8, flags | _native.PE_SL_FREEZE)
+            np.testing.assert_array_equal(alone.read().view(np.uint32), f.view(np.uint32))
