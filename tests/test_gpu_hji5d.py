@@ -170,11 +170,30 @@ def test_c1_full_size_properties(gpu, oracle_lib):
     print(f"C1 float32 vs float64 local arithmetic after 810 steps: max|dV|={d.max():.3e} "
           f"mean|dV|={d.mean():.3e}; capture-set flips {len(flips)} "
           f"(|V64| max {np.abs(fields['float64'][flips]).max() if len(flips) else 0:.2e})")
-    # every flip by node: storage index (vP, vE, psi, y, x), both values
+    # every flip by node: storage index (vP, vE, psi, y, x), both values;
+    # each lies within one grid cell of the FP64 capture boundary (a grid
+    # neighbour along some axis, psi periodic, is on the other side)
+    f64 = fields["float64"].reshape(C1.shape)
+
+    def on_boundary(node):
+        inside = f64[node] <= 0
+        for axis in range(5):
+            for step in (-1, 1):
+                nb = list(node)
+                nb[axis] += step
+                if axis == 2:
+                    nb[2] %= C1.headings
+                elif not 0 <= nb[axis] < C1.shape[axis]:
+                    continue
+                if (f64[tuple(nb)] <= 0) != inside:
+                    return True
+        return False
+
     for flat in flips:
-        node = np.unravel_index(flat, C1.shape)
-        print(f"  flip {tuple(int(i) for i in node)}: V32={fields['float32'][flat]:+.3e} "
-              f"V64={fields['float64'][flat]:+.3e}")
+        node = tuple(int(i) for i in np.unravel_index(flat, C1.shape))
+        print(f"  flip {node}: V32={fields['float32'][flat]:+.3e} "
+              f"V64={fields['float64'][flat]:+.3e}, on the FP64 boundary: {on_boundary(node)}")
+        assert on_boundary(node)
     # The FP32 error is bounded by the scheme's own tie-breaking ambiguity
     # (the FP64 field's mirror asymmetry) and is small on average.
     assert d.max() <= 2 * asym_max["float64"] + 1e-3 and d.mean() <= 1e-4
