@@ -572,6 +572,8 @@ class SL:
         arrays = [np.ascontiguousarray(p_ids, np.int32), np.ascontiguousarray(p_weights, np.float32),
                   np.ascontiguousarray(e_ids, np.int32), np.ascontiguousarray(e_weights, np.float32)]
         n = len(arrays[0])
+        if any(a.size != n * 16 for a in arrays):
+            raise ValueError("query stencils [n, 16] required for both cars")
         out = np.empty(n, np.float32)
         check(lib().pe_sl_query(self._h, int(index), n, *[ptr(a) for a in arrays], ptr(out)))
         return out
@@ -761,6 +763,8 @@ class RelTable:
     def field(self, states, budgets):
         s = np.ascontiguousarray(states, np.float64).reshape(-1, 5)
         b = np.ascontiguousarray(budgets, np.float64).reshape(-1)
+        if len(b) != len(s):
+            raise ValueError("one budget per state required")
         out = np.empty(len(s), np.float64)
         check(lib().pe_rtable_field(self._h, ptr(s), ptr(b), len(s), ptr(out)))
         return out
@@ -770,6 +774,8 @@ class RelTable:
         at most PE_RTABLE_NOALLOC_MAX queries per call)."""
         s = np.ascontiguousarray(states, np.float64).reshape(-1, 5)
         b = np.ascontiguousarray(budgets, np.float64).reshape(-1)
+        if len(b) != len(s):
+            raise ValueError("one budget per state required")
         out = np.empty(len(s), np.float64)
         check(lib().pe_rtable_field_noalloc(self._h, ptr(s), ptr(b), len(s), ptr(out)))
         return out
