@@ -531,7 +531,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--profile-steps", type=int, default=100)
+    ap.add_argument("--profile-steps", type=int, default=None,
+                    help="steps of the per-launch event pass (default: --steps)")
     ap.add_argument("--cpu-sample-steps", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true",
@@ -590,12 +591,19 @@ def main() -> int:
         device_seconds = h.stats()["update_seconds"] - before
         print(f"[bench] timed {args.steps} steps: {wall:.4f} s wall, "
               f"{device_seconds:.4f} s device", file=sys.stderr, flush=True)
-        # Roofline pass: per-launch hipEvents on the library's stream.
+        residual = h.stats()["last_residual"]
+    # Roofline pass: the same steps again from phi on a fresh handle, with
+    # per-launch hipEvents on the library's stream, so that the per-launch
+    # average covers the same part of the solve as the timed run (the step
+    # cost drifts as the field evolves).
+    profile_steps = args.steps if args.profile_steps is None else max(
+        1, min(args.profile_steps, args.steps))
+    with _native.HJI5D(cfg) as h:
+        if args.warmup:
+            h.step(dt, args.warmup)
         h.set_profiling(True)
-        h.step(dt, max(1, min(args.profile_steps, args.steps)))
+        h.step(dt, profile_steps)
         st = h.stats()
-        h.set_profiling(False)
-        residual = st["last_residual"]
     avg_ms = [st["stage_ms"][k] / max(1, st["stage_launches"][k]) for k in range(2)]
     achieved = [STAGE_BYTES[k] * n / (avg_ms[k] * 1e-3) / 1e9 for k in range(2)]
     traffic, traffic_kind = load_traffic()
