@@ -457,6 +457,14 @@ void pe_mlp_destroy(pe_mlp *h);
    order is reset so that the first sweep records it from V0.  No
    counterpart in the reference. */
 int pe_sl_warm_start(pe_sl *sl, pe_mlp *mlp, float delta);
+/* Command order of every inside pair from the network: the argmin P command
+   and the argmax E command of one backup evaluated on min(phi, mlp) are
+   stored as the pairs' control order, which the next sweep with
+   PE_SL_ORDERED tries first (the values are those of any order; the
+   handle's field is not changed).  Call after pe_sl_initialize or
+   pe_sl_warm_start, which reset the order.  No counterpart in the
+   reference. */
+int pe_sl_order_from_mlp(pe_sl *sl, pe_mlp *mlp);
 /* Raw network inputs [count, 11] of pairs [first, first + count) of a map
    (row-major pair index ip * e.n + ie), for building training sets. */
 int pe_sl_pair_inputs(pe_sl *sl, int32_t map, int64_t first, int64_t count,

@@ -39,6 +39,9 @@ struct MlpShape {
 int mlp_forward_loaded(struct ::pe_mlp *h, long long n);
 int mlp_reserve(struct ::pe_mlp *h, long long n);
 
+// One order probe of a solver handle (pe_sl8d.hip).
+int sl_probe_orders(pe_sl *h, const float *field, float *scratch);
+
 }  // namespace pe
 
 struct pe_sl {
@@ -69,6 +72,10 @@ struct pe_sl {
   double *pstates = nullptr, *estates = nullptr;
   float *pclear = nullptr, *eclear = nullptr;  // per-state clearance (may be zero)
   float *va = nullptr, *vb = nullptr, *G = nullptr;
+  // while set, the E side starts every pair from +3e38 instead of its value,
+  // so each inside pair records its argmin / argmax commands (an order
+  // probe into scratch buffers; the handle's fields are not touched)
+  bool probe_orders = false;
   // residual ring [slots][B] and graph slots [graph_sweeps][B]
   unsigned long long *ring = nullptr, *gslots = nullptr;
   int ring_slots = 256, ring_used = 0;

@@ -102,6 +102,23 @@ static void enqueue_sweep(pe_sl *h, const float *src, float *dst, int flags,
     enqueue_sweep_t<float>(h, src, dst, flags, slot, record);
 }
 
+namespace pe {
+// Records in h->order, for every inside pair of every active map, the
+// argmin P command and the argmax E command of one backup evaluated on
+// `field` (device memory in the handle's dense layout, FP32 handles): the
+// E side starts each pair from +3e38, so every pair records its commands.
+// The backup's values go to `scratch` and are dropped; the handle's value
+// buffers, trace and sweep count are not touched.
+int sl_probe_orders(pe_sl *h, const float *field, float *scratch) {
+  h->probe_orders = true;
+  enqueue_sweep(h, field, scratch, PE_SL_EARLY_EXIT | PE_SL_ORDERED, h->gslots, false);
+  h->probe_orders = false;
+  PE_HIP(hipGetLastError());
+  PE_HIP(hipStreamSynchronize(h->stream));
+  return PE_OK;
+}
+}  // namespace pe
+
 static int sl_flush(pe_sl *h) {
   if (h->ring_used == 0) return PE_OK;
   std::vector<unsigned long long> host((size_t)h->ring_used * h->B);

@@ -93,6 +93,47 @@ extern "C" int pe_sl_pair_inputs(pe_sl *sl, int32_t map, int64_t first, int64_t 
   return PE_OK;
 }
 
+extern "C" int pe_sl_order_from_mlp(pe_sl *sl, pe_mlp *mlp) {
+  clear_error();
+  if (!sl || !mlp) return fail(PE_ERR_INVALID, "invalid argument");
+  if (mlp->shape.raw != 11) return fail(PE_ERR_INVALID, "the network needs extra = 3 inputs");
+  if (sl->f64) return fail(PE_ERR_INVALID, "command orders need an FP32 solver handle");
+  if (sl->device != mlp->device) return fail(PE_ERR_INVALID, "handles on different devices");
+  PE_HIP(hipSetDevice(sl->device));
+  PE_HIP(hipStreamSynchronize(sl->stream));
+  const long long chunk = 1 << 18;
+  int rc = mlp_reserve(mlp, chunk);
+  if (rc) return rc;
+  float *field = nullptr;  // [pairs] network field, then [pairs] probe output
+  PE_HIP(hipMalloc(&field, sizeof(float) * 2 * (size_t)sl->pairs));
+  auto body = [&]() -> int {
+    for (int b = 0; b < sl->B; b++) {
+      const SlMap &m = sl->maps[b];
+      const long long n = (long long)m.np * m.ne;
+      for (long long first = 0; first < n; first += chunk) {
+        const long long count = std::min(chunk, n - first);
+        const dim3 grid((unsigned)((count + 255) / 256));
+        hipLaunchKernelGGL(pair_inputs, grid, dim3(256), 0, mlp->stream, sl->d_maps, b,
+                           first, count, sl->pstates, sl->estates, sl->pclear, sl->eclear,
+                           mlp->raw);
+        PE_LAUNCHED(pair_inputs);
+        int r = mlp_forward_loaded(mlp, count);
+        if (r) return r;
+        // min(phi, network) into the scratch field; orders reset to none
+        hipLaunchKernelGGL(warm_combine, grid, dim3(256), 0, mlp->stream, sl->d_maps, b,
+                           first, count, mlp->raw, mlp->yhat, 0.f, sl->ppos, sl->epos,
+                           sl->inside, field, sl->order);
+        PE_LAUNCHED(warm_combine);
+      }
+    }
+    PE_HIP(hipStreamSynchronize(mlp->stream));
+    return sl_probe_orders(sl, field, field + sl->pairs);
+  };
+  rc = body();
+  (void)hipFree(field);
+  return rc;
+}
+
 extern "C" int pe_sl_warm_start(pe_sl *sl, pe_mlp *mlp, float delta) {
   clear_error();
   if (!sl || !mlp || !std::isfinite(delta)) return fail(PE_ERR_INVALID, "invalid argument");

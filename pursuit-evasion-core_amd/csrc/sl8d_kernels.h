@@ -388,7 +388,7 @@ __global__ __launch_bounds__(256) void sl_e_side(
     const uint8_t *__restrict__ eallowed, const uint8_t *__restrict__ pallowed,
     const int *__restrict__ ppos, const int *__restrict__ epos,
     const uint8_t *__restrict__ inside, uint8_t *__restrict__ order,
-    unsigned long long *__restrict__ slot) {
+    unsigned long long *__restrict__ slot, int probe) {
   const SlMap m = maps[blockIdx.z];
   if (!m.active) return;  // the whole block: no residual for a frozen map
   const int ie = blockIdx.x * 256 + threadIdx.x;
@@ -398,7 +398,7 @@ __global__ __launch_bounds__(256) void sl_e_side(
     const int ep = epos[erow];
     for (int ip = blockIdx.y; ip < m.np; ip += gridDim.y) {
       const long long pair = m.v_off + (long long)ip * m.ne + ie;
-      const T start = V[pair];
+      const T start = probe ? T(3.0e38f) : V[pair];
       T best = start;
       const int prow = m.p_row + ip;
       if (inside[m.tab_off + (long long)ppos[prow] * m.npos_e + ep]) {
@@ -434,7 +434,7 @@ __global__ __launch_bounds__(kLdsMaxThreads) void sl_e_side_lds(
     const uint8_t *__restrict__ eallowed, const uint8_t *__restrict__ pallowed,
     const int *__restrict__ ppos, const int *__restrict__ ilist,
     const int *__restrict__ ioff, uint8_t *__restrict__ order,
-    unsigned long long *__restrict__ slot) {
+    unsigned long long *__restrict__ slot, int probe) {
   extern __shared__ __align__(16) unsigned char tile_bytes[];
   T *tile = reinterpret_cast<T *>(tile_bytes);
   const SlMap m = maps[blockIdx.z];
@@ -480,7 +480,7 @@ __global__ __launch_bounds__(kLdsMaxThreads) void sl_e_side_lds(
       const int ie = ilist[k0 + k];
       const int erow = m.e_row + ie;
       const long long pair = m.v_off + (long long)ip * m.ne + ie;
-      const T start = V[pair];
+      const T start = probe ? T(3.0e38f) : V[pair];
       const T best = pair_backup<EARLY, ORDER>(m, prow, erow, start, pair, row, eids, ew,
                                                  eallowed, pallowed, order);
       Vn[pair] = best;
@@ -526,6 +526,7 @@ void enqueue_sweep_t(pe_sl *h, const void *src_, void *dst_, int flags,
   if (record) sl_record(h, 0, false);
   if (record) sl_record(h, 1, true);
   const bool early = flags & PE_SL_EARLY_EXIT, ordered = flags & PE_SL_ORDERED;
+  const int probe = h->probe_orders ? 1 : 0;
   const bool lds = h->lds_bytes > 0 && !(flags & PE_SL_GLOBAL_GATHER);
   const dim3 lds_grid((unsigned)h->np_max, 1, (unsigned)h->B), lds_block(h->lds_threads);
   const bool packed = h->erec && !(flags & PE_SL_WIDE_IDS);
@@ -535,11 +536,11 @@ void enqueue_sweep_t(pe_sl *h, const void *src_, void *dst_, int flags,
   if (lds)                                                                             \
     hipLaunchKernelGGL((sl_e_side_lds<T, E, O, ID>), lds_grid, lds_block, h->lds_bytes,  \
                        h->stream, h->d_maps, src, dst, G, IDS, h->ew, h->eallowed,       \
-                       h->pallowed, h->ppos, h->ilist, h->ioff, h->order, slot);         \
+                       h->pallowed, h->ppos, h->ilist, h->ioff, h->order, slot, probe);  \
   else                                                                                   \
     hipLaunchKernelGGL((sl_e_side<T, E, O, ID>), grid, block, 0, h->stream, h->d_maps,   \
                        src, dst, G, IDS, h->ew, h->eallowed, h->pallowed, h->ppos,       \
-                       h->epos, h->inside, h->order, slot)
+                       h->epos, h->inside, h->order, slot, probe)
 #define PE_E_SIDE(E, O)                                            \
   do {                                                             \
     if (packed) {                                                  \

@@ -282,6 +282,7 @@ SIGNATURES = {
     "pe_sl_set_lds_threads": (ctypes.c_int, [P, I32]),
     "pe_sl_destroy": (None, [P]),
     "pe_sl_warm_start": (ctypes.c_int, [P, P, ctypes.c_float]),
+    "pe_sl_order_from_mlp": (ctypes.c_int, [P, P]),
     "pe_sl_pair_inputs": (ctypes.c_int, [P, I32, I64, I64, P]),
     "pe_history_create": (ctypes.c_int, [P, I64, I64, P]),
     "pe_history_query": (ctypes.c_int, [P, I64, I32, P, P, P, ctypes.c_float, P]),
@@ -603,6 +604,11 @@ class SL:
 
     def warm_start(self, mlp, delta):
         check(lib().pe_sl_warm_start(self._h, mlp._h, float(delta)))
+
+    def order_from(self, mlp):
+        """Stored command orders of every inside pair from the network's
+        field (pe_sl_order_from_mlp); values are unchanged."""
+        check(lib().pe_sl_order_from_mlp(self._h, mlp._h))
 
 
 class History:

@@ -379,6 +379,34 @@ def compare_batched(train_seeds, validate_seeds, test_seeds, *, grid, label_tol=
                   start_mean_gap_to_vstar=[float((z - v)[keep].mean())
                                            for z, v, keep in zip(v0, vstar, masks)])
 
+    # The network's command order for a cold run: the first sweep's E side
+    # in the index order (nothing stored yet) and in the network's order,
+    # and the cold run to residual `tol_res` from the network's order.
+    order = {}
+    with GameSolver(tables) as s:
+        s.set_profiling(True)
+        s.sweep(1, FLAGS & ~_native.PE_SL_GRAPH)
+        st = s.stats()
+        order["first_sweep_e_ms_index_order"] = st["e_ms"] / max(1, st["e_launches"])
+        s.set_profiling(False)
+        p_s, p_done, _, p_stops = _run_batch(s, tol_res, vstar_cap)
+        order.update(plain_sweeps_to_residual=p_done + 1, plain_gpu_seconds_to_residual=p_s,
+                     plain_map_sweeps=[None if k is None else k + 1 for k in p_stops])
+    with GameSolver(tables) as s:
+        o0 = time.perf_counter()
+        s.order_from(net)
+        order["order_seconds"] = time.perf_counter() - o0
+        s.set_profiling(True)
+        s.sweep(1, FLAGS & ~_native.PE_SL_GRAPH)
+        st = s.stats()
+        order["first_sweep_e_ms_network_order"] = st["e_ms"] / max(1, st["e_launches"])
+        s.set_profiling(False)
+        o_s, o_done, o_ok, o_stops = _run_batch(s, tol_res, vstar_cap)
+        order.update(sweeps_to_residual=o_done + 1, gpu_seconds_to_residual=o_s,
+                     map_sweeps=[None if k is None else k + 1 for k in o_stops])
+    log(f"[learned] command order: first E side {order['first_sweep_e_ms_index_order']:.3f} ms "
+        f"(index order) / {order['first_sweep_e_ms_network_order']:.3f} ms (network order)")
+
     def ratio(key):
         c, w = cold_t[key]["batch_gpu_seconds"], warm_t[key]["batch_gpu_seconds"]
         return c / (w + start_s) if c is not None and w is not None else None
@@ -387,7 +415,7 @@ def compare_batched(train_seeds, validate_seeds, test_seeds, *, grid, label_tol=
         vstar=dict(tolerance=vstar_tol, cap=vstar_cap, sweeps=vstar_done, converged=vstar_ok,
                    gpu_seconds=vstar_s, map_sweeps=vstar_stops, maps_with_vstar=has_vstar,
                    residual=[float(r) for r in vstar_res]),
-        cold=cold_t, warm=warm_t,
+        cold=cold_t, warm=warm_t, network_command_order=order,
         # The plain DP judges the warm start: it must not start more than
         # tol_err below V* anywhere (such a value is never raised again),
         # and the warm run must come within tol_err of V* on every map.

@@ -24,11 +24,16 @@ What holds whatever the network learned, and is asserted:
 What the network gains is measured and printed, not asserted: on these
 maps it is no sweep (the calibrated delta is of the order of the values'
 range, so min(phi, net + delta) is phi on the slow pairs).
+
+The network also orders each pair's commands (pe_sl_order_from_mlp) for
+the first sweep of a cold run: values and residuals are asserted equal bit
+for bit with and without it, and the first sweep's E side is timed.
 """
 
 import numpy as np
 import pytest
 
+from pe_amd import _native
 from pe_amd.learned import warm_start as ws
 from pe_amd.local_game.grid_game import GameSolver
 
@@ -111,3 +116,29 @@ def test_warm_start_is_bounded_by_cold_and_judged_by_the_plain_dp(case):
         # a pair that started more than 1e-3 below V* never comes within
         # 1e-3 of it: the plain DP's judgement stands
         assert first["warm"] is None
+
+
+def test_network_command_order_leaves_the_values(case):
+    """pe_sl_order_from_mlp stores, for every inside pair, the commands of
+    one backup evaluated on min(phi, network); the sweeps then try them
+    first.  Early exit and the visiting order never change a value, so a
+    cold run from phi gives the same fields and residuals bit for bit with
+    the network's order as with none; the first sweep's E side is timed
+    both ways."""
+    flags = ws.FLAGS & ~_native.PE_SL_GRAPH
+    with GameSolver(case["tables"]) as plain, GameSolver(case["tables"]) as ordered:
+        ordered.order_from(case["net"])
+        first = {}
+        for name, s in (("index order", plain), ("network order", ordered)):
+            s.set_profiling(True)
+            s.sweep(1, flags)
+            st = s.stats()
+            first[name] = st["e_ms"] / max(1, st["e_launches"])
+            s.set_profiling(False)
+        plain.sweep(63, ws.FLAGS)
+        ordered.sweep(63, ws.FLAGS)
+        np.testing.assert_array_equal(plain.read().view(np.uint32),
+                                      ordered.read().view(np.uint32))
+        np.testing.assert_array_equal(plain.trace(), ordered.trace())
+    print(f"[gpu] {case['name']}: first sweep's E side {first['index order']:.3f} ms in index "
+          f"order, {first['network order']:.3f} ms in the network's order", flush=True)
