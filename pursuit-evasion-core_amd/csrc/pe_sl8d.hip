@@ -229,6 +229,12 @@ static int do_sweeps(pe_sl *h, int count, int flags) {
   return PE_OK;
 }
 
+// Packed E records (16-bit byte offsets into a G row) when every map's row
+// of E values spans at most 64 KiB.
+static bool packed_records(const pe_sl *h) {
+  return (long long)(h->ne_max - 1) * (long long)h->esize <= 65535;
+}
+
 extern "C" int pe_sl_create(const pe_sl_game *games, int32_t n_maps,
                             pe_sl **out) {
   return pe_sl_create_ex(games, n_maps, 0, out);
@@ -353,7 +359,7 @@ extern "C" int pe_sl_create_ex(const pe_sl_game *games, int32_t n_maps, int32_t 
           double fill[3];
           for (int k = 0; k < 3; k++)
             fill[k] = lds_slots[k] ? (double)h->active / (double)lds_slots[k] : 1.0;
-          h->lds_threads = sl_lds_threads(bytes, fill, h->ne_max < 65536);
+          h->lds_threads = sl_lds_threads(bytes, fill, packed_records(h));
         }
       }
     }
@@ -430,11 +436,11 @@ extern "C" int pe_sl_create_ex(const pe_sl_game *games, int32_t n_maps, int32_t 
       PE_HIP(hipMalloc((void **)&h->pcount, std::max<size_t>(n, 1)));
       PE_HIP(hipMemcpy(h->pcount, cnt.data(), n, hipMemcpyHostToDevice));
     }
-    if (h->ne_max < 65536) {
+    if (packed_records(h)) {
       const long long n = erow * (long long)na;  // records
       PE_HIP(hipMalloc(&h->erec, sizeof(ERec) * std::max(n, 1ll)));
       hipLaunchKernelGGL(sl_pack_erec, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0,
-                         h->eids, h->ew, (ERec *)h->erec, n);
+                         h->eids, h->ew, (ERec *)h->erec, n, (int)h->esize);
       PE_LAUNCHED(sl_pack_erec);
       PE_HIP(hipDeviceSynchronize());
     }

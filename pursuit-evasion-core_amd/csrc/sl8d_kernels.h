@@ -122,8 +122,10 @@ __device__ __forceinline__ int nth_command(int i, int first) {
   return i == 0 ? first : (i <= first ? i - 1 : i);
 }
 
-// E stencil ids of one (state, command): 16 values of type ID (int, or
-// 16-bit when every map has fewer than 65,536 E states), in vertex order.
+// E stencil ids of one (state, command): 16 values of type ID, in vertex
+// order: int ids, or, in the packed records, 16-bit byte offsets (id times
+// the value size, so every map's E states must span at most 64 KiB of a G
+// row).
 template <typename ID>
 struct EIds;
 template <>
@@ -161,8 +163,18 @@ struct ERec {
 // the separate 32-bit id and float weight tables (ID = int).
 template <typename ID>
 struct EStencil {
-  EIds<typename std::conditional<std::is_same<ID, ERec>::value, unsigned short, int>::type> id;
+  static constexpr bool kBytes = std::is_same<ID, ERec>::value;
+  EIds<typename std::conditional<kBytes, unsigned short, int>::type> id;
   float4 w[4];
+  // Value of vertex k in the row Gu: the packed records hold byte offsets
+  // (an add to the row address), the tables ids (scaled by the value size).
+  template <typename T>
+  __device__ __forceinline__ T at(const T *Gu, int k) const {
+    if constexpr (kBytes)
+      return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(Gu) + id[k]);
+    else
+      return Gu[id[k]];
+  }
   __device__ __forceinline__ void load(const ID *__restrict__ eids,
                                        const float *__restrict__ ew, long long cmd) {
     if constexpr (std::is_same<ID, ERec>::value) {
@@ -186,14 +198,14 @@ struct EStencil {
     qb = 0;
 #pragma unroll
     for (int c = 0; c < kStencil / 4; c++) {
-      qa = madd(a.w[c].x, Gu[a.id[4 * c]], qa);
-      qb = madd(b.w[c].x, Gu[b.id[4 * c]], qb);
-      qa = madd(a.w[c].y, Gu[a.id[4 * c + 1]], qa);
-      qb = madd(b.w[c].y, Gu[b.id[4 * c + 1]], qb);
-      qa = madd(a.w[c].z, Gu[a.id[4 * c + 2]], qa);
-      qb = madd(b.w[c].z, Gu[b.id[4 * c + 2]], qb);
-      qa = madd(a.w[c].w, Gu[a.id[4 * c + 3]], qa);
-      qb = madd(b.w[c].w, Gu[b.id[4 * c + 3]], qb);
+      qa = madd(a.w[c].x, a.at(Gu, 4 * c), qa);
+      qb = madd(b.w[c].x, b.at(Gu, 4 * c), qb);
+      qa = madd(a.w[c].y, a.at(Gu, 4 * c + 1), qa);
+      qb = madd(b.w[c].y, b.at(Gu, 4 * c + 1), qb);
+      qa = madd(a.w[c].z, a.at(Gu, 4 * c + 2), qa);
+      qb = madd(b.w[c].z, b.at(Gu, 4 * c + 2), qb);
+      qa = madd(a.w[c].w, a.at(Gu, 4 * c + 3), qa);
+      qb = madd(b.w[c].w, b.at(Gu, 4 * c + 3), qb);
     }
   }
   // sum_k w_k G[id_k], vertex order (grid_game.py:406-408)
@@ -202,10 +214,10 @@ struct EStencil {
     T q = 0;
 #pragma unroll
     for (int c = 0; c < kStencil / 4; c++) {
-      q = madd(w[c].x, Gu[id[4 * c]], q);
-      q = madd(w[c].y, Gu[id[4 * c + 1]], q);
-      q = madd(w[c].z, Gu[id[4 * c + 2]], q);
-      q = madd(w[c].w, Gu[id[4 * c + 3]], q);
+      q = madd(w[c].x, at(Gu, 4 * c), q);
+      q = madd(w[c].y, at(Gu, 4 * c + 1), q);
+      q = madd(w[c].z, at(Gu, 4 * c + 2), q);
+      q = madd(w[c].w, at(Gu, 4 * c + 3), q);
     }
     return q;
   }
@@ -560,18 +572,19 @@ void enqueue_sweep_t(pe_sl *h, const void *src_, void *dst_, int flags,
   if (record) sl_record(h, 1, false);
 }
 
-// Interleaved records of the E stencils: 16-bit ids (every id below 65,536)
-// then the 16 weights, one record per (state, command).
+// Interleaved records of the E stencils: 16-bit byte offsets (id times the
+// value size `scale`, every offset below 65,536) then the 16 weights, one
+// record per (state, command).
 __global__ __launch_bounds__(256) inline void sl_pack_erec(const int *__restrict__ ids,
                                                            const float *__restrict__ w,
                                                            ERec *__restrict__ out,
-                                                           long long n) {
+                                                           long long n, int scale) {
   const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
   if (r >= n) return;
   unsigned short id[16];
   float wt[16];
   for (int k = 0; k < 16; k++) {
-    id[k] = (unsigned short)ids[r * 16 + k];
+    id[k] = (unsigned short)(ids[r * 16 + k] * scale);
     wt[k] = w[r * 16 + k];
   }
   ERec rec;
