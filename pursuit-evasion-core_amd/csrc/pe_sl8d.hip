@@ -359,7 +359,7 @@ extern "C" int pe_sl_create_ex(const pe_sl_game *games, int32_t n_maps, int32_t 
           double fill[3];
           for (int k = 0; k < 3; k++)
             fill[k] = lds_slots[k] ? (double)h->active / (double)lds_slots[k] : 1.0;
-          h->lds_threads = sl_lds_threads(bytes, fill, packed_records(h));
+          h->lds_threads = sl_lds_threads(bytes, fill, packed_records(h), h->na <= 9);
         }
       }
     }

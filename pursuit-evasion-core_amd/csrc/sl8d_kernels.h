@@ -236,13 +236,16 @@ struct EStencil {
 //   kOnePass      u* first, u-major; then every other allowed P command is
 //                 carried along one pass over the E commands, so each E
 //                 stencil is read once for all of them instead of once per P
-//                 command (more registers: ~96 VGPRs against ~48).
+//                 command (more registers: ~96 VGPRs against ~48);
+//   kOnePass9     the same with room for 9 commands per state instead of
+//                 15 (the {-A, 0, A} x {-K, 0, K} set): the running maxima
+//                 take 6 registers fewer, ~73 VGPRs in all.
 // The value and the stored commands are the same in all three: max and min
 // are exact, each (u, b) combination is summed in the same order, ties go
 // to the first command in visiting order, and a command cut against the
 // bound of u* could not have been chosen later (its max already reaches a
 // value at or above the final min).
-constexpr int kPlainOrder = 0, kUMajor = 1, kOnePass = 2;
+constexpr int kPlainOrder = 0, kUMajor = 1, kOnePass = 2, kOnePass9 = 3;
 
 template <bool EARLY, int ORDER, typename T, typename Row, typename ID>
 __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T start,
@@ -253,6 +256,8 @@ __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T s
                                          const uint8_t *__restrict__ pallowed,
                                          uint8_t *__restrict__ order) {
   constexpr bool ORDERED = ORDER != kPlainOrder;
+  constexpr bool ONE_PASS = ORDER == kOnePass || ORDER == kOnePass9;
+  constexpr int MAXA = ORDER == kOnePass9 ? 9 : kMaxActions;  // commands per state
   T best = start;
   int ufirst = 0, bfirst = 0;
   if (ORDERED) {
@@ -265,7 +270,7 @@ __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T s
   const long long cmd0 = (long long)erow * m.na;
   const uint8_t *pa = pallowed + (long long)prow * m.na;
   int best_u = -1, best_b = -1;
-  if constexpr (ORDER == kOnePass) {
+  if constexpr (ONE_PASS) {
     // u* over its E commands two at a time: two stencil reads in flight and
     // two independent sums; the second result is dropped when the first
     // already cuts, so the visited sequence is the sequential one.
@@ -309,7 +314,7 @@ __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T s
       }
     }
   }
-  for (int i = 0; i < (ORDER == kOnePass ? 0 : m.na); i++) {
+  for (int i = 0; i < (ONE_PASS ? 0 : m.na); i++) {
     const int u = ORDERED ? nth_command(i, ufirst) : i;
     if (!pa[u]) continue;
     const T *Gu = row(u);
@@ -337,22 +342,22 @@ __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T s
       best_b = arg;
     }
   }
-  if constexpr (ORDER == kOnePass) {
+  if constexpr (ONE_PASS) {
     unsigned alive = 0;
 #pragma unroll
-    for (int u = 0; u < kMaxActions; u++)
+    for (int u = 0; u < MAXA; u++)
       if (u < m.na && u != ufirst && pa[u]) alive |= 1u << u;
-    T worst[kMaxActions];
+    T worst[MAXA];
     unsigned long long args = 0;  // 4 bits per P command
 #pragma unroll
-    for (int u = 0; u < kMaxActions; u++) worst[u] = T(-1e30);
+    for (int u = 0; u < MAXA; u++) worst[u] = T(-1e30);
     for (int j = 0; j < m.na && alive; j++) {
       const int b = nth_command(j, bfirst);
       if (!eallowed[cmd0 + b]) continue;
       EStencil<ID> st;
       st.load(eids, ew, cmd0 + b);
 #pragma unroll
-      for (int u = 0; u < kMaxActions; u++) {
+      for (int u = 0; u < MAXA; u++) {
         if (!((alive >> u) & 1u)) continue;
         const T q = st.eval(row(u));
         if (q > worst[u]) {
@@ -364,7 +369,7 @@ __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T s
     }
     // Survivors in visiting order (ascending u after u*).
 #pragma unroll
-    for (int u = 0; u < kMaxActions; u++) {
+    for (int u = 0; u < MAXA; u++) {
       if (((alive >> u) & 1u) && worst[u] < best) {
         best = worst[u];
         best_u = u;
@@ -561,9 +566,12 @@ void enqueue_sweep_t(pe_sl *h, const void *src_, void *dst_, int flags,
       PE_E_SIDE_ID(E, O, int, h->eids);                            \
     }                                                              \
   } while (0)
-  if (early && ordered && one_pass) PE_E_SIDE(true, kOnePass);
+  const bool nine = h->na <= 9;
+  if (early && ordered && one_pass && nine) PE_E_SIDE(true, kOnePass9);
+  else if (early && ordered && one_pass) PE_E_SIDE(true, kOnePass);
   else if (early && ordered) PE_E_SIDE(true, kUMajor);
   else if (early) PE_E_SIDE(true, kPlainOrder);
+  else if (ordered && one_pass && nine) PE_E_SIDE(false, kOnePass9);
   else if (ordered && one_pass) PE_E_SIDE(false, kOnePass);
   else if (ordered) PE_E_SIDE(false, kUMajor);
   else PE_E_SIDE(false, kPlainOrder);
@@ -601,6 +609,10 @@ template <typename T>
 bool sl_allow_lds(size_t bytes) {
   bool ok = true;
   for (const void *f : {(const void *)sl_e_side_lds<T, true, kOnePass, int>,
+                        (const void *)sl_e_side_lds<T, true, kOnePass9, int>,
+                        (const void *)sl_e_side_lds<T, false, kOnePass9, int>,
+                        (const void *)sl_e_side_lds<T, true, kOnePass9, ERec>,
+                        (const void *)sl_e_side_lds<T, false, kOnePass9, ERec>,
                         (const void *)sl_e_side_lds<T, true, kUMajor, int>,
                         (const void *)sl_e_side_lds<T, true, kPlainOrder, int>,
                         (const void *)sl_e_side_lds<T, false, kOnePass, int>,
@@ -619,16 +631,20 @@ bool sl_allow_lds(size_t bytes) {
 }
 
 // Threads per block of the LDS E side for FP32 values in the default
-// (one-pass) order: of 1024, 512 and 256, the size that keeps the most
-// lanes busy, counting the blocks one CU holds at this kernel's registers
-// and `bytes` of LDS, times the fraction of lane slots that the inside
-// pairs of the P states fill (fill[k] for blocks of 256 << k threads); ties
-// go to the smaller block.
-inline int sl_lds_threads(size_t bytes, const double fill[3], bool packed) {
-  const void *f = packed ? (const void *)sl_e_side_lds<float, true, kOnePass, ERec>
-                         : (const void *)sl_e_side_lds<float, true, kOnePass, int>;
-  int best = kLdsMaxThreads;
-  double best_score = -1;
+// (one-pass) order, of 256, 512 and 1024.  Score: the blocks one CU holds
+// at this kernel's registers and `bytes` of LDS, times the threads per
+// block, times the fraction of lane slots that the inside pairs of the P
+// states fill (fill[k] for blocks of 256 << k threads): the busy lanes per
+// CU.  The smallest size within 80 % of the best score is taken: more,
+// smaller blocks measured faster than a few percent more busy lanes in
+// larger ones (C4 maps: 3.85 ms with 256 threads against 4.17 ms with 512,
+// whose score is 20 % higher).
+inline int sl_lds_threads(size_t bytes, const double fill[3], bool packed, bool nine) {
+  const void *f = nine ? (packed ? (const void *)sl_e_side_lds<float, true, kOnePass9, ERec>
+                                 : (const void *)sl_e_side_lds<float, true, kOnePass9, int>)
+                       : (packed ? (const void *)sl_e_side_lds<float, true, kOnePass, ERec>
+                                 : (const void *)sl_e_side_lds<float, true, kOnePass, int>);
+  double score[3] = {-1, -1, -1}, best = -1;
   for (int k = 0; k < 3; k++) {
     const int t = 256 << k;
     int blocks = 0;
@@ -636,13 +652,12 @@ inline int sl_lds_threads(size_t bytes, const double fill[3], bool packed) {
       (void)hipGetLastError();
       continue;
     }
-    const double score = blocks * t * fill[k];
-    if (score > best_score * 1.0001) {
-      best_score = score;
-      best = t;
-    }
+    score[k] = blocks * t * fill[k];
+    best = std::max(best, score[k]);
   }
-  return best;
+  for (int k = 0; k < 3; k++)
+    if (best > 0 && score[k] >= 0.8 * best) return 256 << k;
+  return kLdsMaxThreads;
 }
 
 // FP64 instances (pe_sl8d_f64.hip, built without contraction).

@@ -138,11 +138,14 @@ def run_variants(tables, sweeps):
     return out
 
 
-def test_early_exit_ordering_and_graph_are_exact(gpu):
+# 9 commands per state take the one-pass kernel with 9-slot running maxima,
+# 12 the one with room for 15.
+@pytest.mark.parametrize("na", [9, 12])
+def test_early_exit_ordering_and_graph_are_exact(gpu, na):
     rng = np.random.default_rng(21)
-    p, e = fake_grid(400, 9, rng), fake_grid(300, 9, rng)
+    p, e = fake_grid(400, na, rng), fake_grid(300, na, rng)
     tables, _ = fake_tables(p, e, 1.0)
-    print("[gpu] flag variants", flush=True)
+    print(f"[gpu] flag variants, {na} commands per state", flush=True)
     res = run_variants(tables, 6)
     base, trace = res["plain"]
     for name in ("early", "ordered", "graph", "global", "global_plain", "wide_ids",
