@@ -270,15 +270,20 @@ __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T s
   const long long cmd0 = (long long)erow * m.na;
   const uint8_t *pa = pallowed + (long long)prow * m.na;
   int best_u = -1, best_b = -1;
+  // allowed E commands of the pair's E state, bit j = j-th in visiting order
+  // (read once for both passes of the one-pass order)
+  unsigned ea_visit = 0;
+  if constexpr (ONE_PASS) {
+    for (int j = 0; j < m.na; j++)
+      if (eallowed[cmd0 + nth_command(j, bfirst)]) ea_visit |= 1u << j;
+  }
   if constexpr (ONE_PASS) {
     // u* over its E commands two at a time: two stencil reads in flight and
     // two independent sums; the second result is dropped when the first
     // already cuts, so the visited sequence is the sequential one.
     if (pa[ufirst]) {
       const T *Gu = row(ufirst);
-      unsigned ea = 0;  // allowed E commands, bit j = j-th in visiting order
-      for (int j = 0; j < m.na; j++)
-        if (eallowed[cmd0 + nth_command(j, bfirst)]) ea |= 1u << j;
+      unsigned ea = ea_visit;
       T worst = T(-1e30);
       int arg = -1;
       bool cut = false;
@@ -351,9 +356,8 @@ __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T s
     unsigned long long args = 0;  // 4 bits per P command
 #pragma unroll
     for (int u = 0; u < MAXA; u++) worst[u] = T(-1e30);
-    for (int j = 0; j < m.na && alive; j++) {
-      const int b = nth_command(j, bfirst);
-      if (!eallowed[cmd0 + b]) continue;
+    for (unsigned todo = ea_visit; todo && alive; todo &= todo - 1) {
+      const int b = nth_command(__builtin_ctz(todo), bfirst);
       EStencil<ID> st;
       st.load(eids, ew, cmd0 + b);
 #pragma unroll
