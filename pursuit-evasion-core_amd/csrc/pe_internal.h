@@ -67,6 +67,8 @@ struct pe_sl {
   // count of non-zero weights per (state, command); pids / pw hold them.
   uint8_t *pcount = nullptr;
   float *pw = nullptr, *ew = nullptr;
+  // allowed commands of each state as a bit mask (bit u), for the E side
+  uint16_t *pmask = nullptr, *emask = nullptr;
   uint8_t *pallowed = nullptr, *eallowed = nullptr, *inside = nullptr,
           *near = nullptr, *order = nullptr;
   double *pstates = nullptr, *estates = nullptr;

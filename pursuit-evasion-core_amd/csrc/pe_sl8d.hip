@@ -395,6 +395,21 @@ extern "C" int pe_sl_create_ex(const pe_sl_game *games, int32_t n_maps, int32_t 
           (r = cat(P ? &h->ppos : &h->epos, [](const pe_sl_car &c) { return c.position; },
                    1, P)))
         return r;
+      // Allowed commands of each state as one bit mask.
+      std::vector<uint16_t> mask;
+      for (int b = 0; b < n_maps; b++) {
+        const pe_sl_car &c = P ? games[b].p : games[b].e;
+        for (int i = 0; i < c.n; i++) {
+          unsigned bits = 0;
+          for (int u = 0; u < na; u++)
+            if (c.allowed[(long long)i * na + u]) bits |= 1u << u;
+          mask.push_back((uint16_t)bits);
+        }
+      }
+      uint16_t **mdst = P ? &h->pmask : &h->emask;
+      PE_HIP(hipMalloc((void **)mdst, sizeof(uint16_t) * std::max<size_t>(mask.size(), 1)));
+      PE_HIP(hipMemcpy(*mdst, mask.data(), sizeof(uint16_t) * mask.size(),
+                       hipMemcpyHostToDevice));
       // Per-state clearance as float (zeros where not given).
       std::vector<float> clear;
       for (int b = 0; b < n_maps; b++) {
@@ -740,7 +755,8 @@ extern "C" void pe_sl_destroy(pe_sl *h) {
                   (void *)h->ilist, (void *)h->ioff, (void *)h->nlist, (void *)h->noff,
                   (void *)h->pcount,
                   (void *)h->ppos, (void *)h->epos, (void *)h->pw, (void *)h->ew,
-                  (void *)h->pallowed, (void *)h->eallowed, (void *)h->inside,
+                  (void *)h->pallowed, (void *)h->eallowed, (void *)h->pmask,
+                  (void *)h->emask, (void *)h->inside,
                   (void *)h->near, (void *)h->order, (void *)h->pstates,
                   (void *)h->estates, (void *)h->pclear, (void *)h->eclear,
                   (void *)h->va, (void *)h->vb, (void *)h->G,

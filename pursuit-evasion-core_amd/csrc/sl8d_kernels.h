@@ -252,8 +252,8 @@ __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T s
                                          long long pair, Row row,
                                          const ID *__restrict__ eids,
                                          const float *__restrict__ ew,
-                                         const uint8_t *__restrict__ eallowed,
-                                         const uint8_t *__restrict__ pallowed,
+                                         const uint16_t *__restrict__ emask,
+                                         const uint16_t *__restrict__ pmask,
                                          uint8_t *__restrict__ order) {
   constexpr bool ORDERED = ORDER != kPlainOrder;
   constexpr bool ONE_PASS = ORDER == kOnePass || ORDER == kOnePass9;
@@ -268,20 +268,18 @@ __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T s
     }
   }
   const long long cmd0 = (long long)erow * m.na;
-  const uint8_t *pa = pallowed + (long long)prow * m.na;
+  // allowed commands, bit u: of the P state and of the E state
+  const unsigned pm = pmask[prow], em = emask[erow];
   int best_u = -1, best_b = -1;
-  // allowed E commands of the pair's E state, bit j = j-th in visiting order
-  // (read once for both passes of the one-pass order)
-  unsigned ea_visit = 0;
-  if constexpr (ONE_PASS) {
-    for (int j = 0; j < m.na; j++)
-      if (eallowed[cmd0 + nth_command(j, bfirst)]) ea_visit |= 1u << j;
-  }
+  // the E state's allowed commands in visiting order (bit j = j-th visited:
+  // bfirst, then the others ascending)
+  const unsigned ea_visit = ((em >> bfirst) & 1u) | ((em & ((1u << bfirst) - 1u)) << 1) |
+                            (em & ~((2u << bfirst) - 1u));
   if constexpr (ONE_PASS) {
     // u* over its E commands two at a time: two stencil reads in flight and
     // two independent sums; the second result is dropped when the first
     // already cuts, so the visited sequence is the sequential one.
-    if (pa[ufirst]) {
+    if ((pm >> ufirst) & 1u) {
       const T *Gu = row(ufirst);
       unsigned ea = ea_visit;
       T worst = T(-1e30);
@@ -321,14 +319,14 @@ __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T s
   }
   for (int i = 0; i < (ONE_PASS ? 0 : m.na); i++) {
     const int u = ORDERED ? nth_command(i, ufirst) : i;
-    if (!pa[u]) continue;
+    if (!((pm >> u) & 1u)) continue;
     const T *Gu = row(u);
     T worst = T(-1e30);
     int arg = -1;
     bool cut = false;
     for (int j = 0; j < m.na; j++) {
       const int b = ORDERED ? nth_command(j, bfirst) : j;
-      if (!eallowed[cmd0 + b]) continue;
+      if (!((em >> b) & 1u)) continue;
       EStencil<ID> st;
       st.load(eids, ew, cmd0 + b);
       const T q = st.eval(Gu);
@@ -348,10 +346,7 @@ __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T s
     }
   }
   if constexpr (ONE_PASS) {
-    unsigned alive = 0;
-#pragma unroll
-    for (int u = 0; u < MAXA; u++)
-      if (u < m.na && u != ufirst && pa[u]) alive |= 1u << u;
+    unsigned alive = pm & ~(1u << ufirst);
     T worst[MAXA];
     unsigned long long args = 0;  // 4 bits per P command
 #pragma unroll
@@ -406,7 +401,7 @@ __global__ __launch_bounds__(256) void sl_e_side(
     const SlMap *__restrict__ maps, const T *__restrict__ V,
     T *__restrict__ Vn, const T *__restrict__ G,
     const ID *__restrict__ eids, const float *__restrict__ ew,
-    const uint8_t *__restrict__ eallowed, const uint8_t *__restrict__ pallowed,
+    const uint16_t *__restrict__ emask, const uint16_t *__restrict__ pmask,
     const int *__restrict__ ppos, const int *__restrict__ epos,
     const uint8_t *__restrict__ inside, uint8_t *__restrict__ order,
     unsigned long long *__restrict__ slot, int probe) {
@@ -425,7 +420,7 @@ __global__ __launch_bounds__(256) void sl_e_side(
       if (inside[m.tab_off + (long long)ppos[prow] * m.npos_e + ep]) {
         auto row = [&](int u) { return G + m.g_off + ((long long)u * m.np + ip) * m.ne; };
         best = pair_backup<EARLY, ORDER>(m, prow, erow, start, pair, row, eids, ew,
-                                           eallowed, pallowed, order);
+                                           emask, pmask, order);
       }
       Vn[pair] = best;
       const T d = best > start ? best - start : start - best;
@@ -452,7 +447,7 @@ __global__ __launch_bounds__(kLdsMaxThreads) void sl_e_side_lds(
     const SlMap *__restrict__ maps, const T *__restrict__ V,
     T *__restrict__ Vn, const T *__restrict__ G,
     const ID *__restrict__ eids, const float *__restrict__ ew,
-    const uint8_t *__restrict__ eallowed, const uint8_t *__restrict__ pallowed,
+    const uint16_t *__restrict__ emask, const uint16_t *__restrict__ pmask,
     const int *__restrict__ ppos, const int *__restrict__ ilist,
     const int *__restrict__ ioff, uint8_t *__restrict__ order,
     unsigned long long *__restrict__ slot, int probe) {
@@ -503,7 +498,7 @@ __global__ __launch_bounds__(kLdsMaxThreads) void sl_e_side_lds(
       const long long pair = m.v_off + (long long)ip * m.ne + ie;
       const T start = probe ? T(3.0e38f) : V[pair];
       const T best = pair_backup<EARLY, ORDER>(m, prow, erow, start, pair, row, eids, ew,
-                                                 eallowed, pallowed, order);
+                                                 emask, pmask, order);
       Vn[pair] = best;
       const T d = best > start ? best - start : start - best;
       change = change > d ? change : d;
@@ -556,11 +551,11 @@ void enqueue_sweep_t(pe_sl *h, const void *src_, void *dst_, int flags,
 #define PE_E_SIDE_ID(E, O, ID, IDS)                                                    \
   if (lds)                                                                             \
     hipLaunchKernelGGL((sl_e_side_lds<T, E, O, ID>), lds_grid, lds_block, h->lds_bytes,  \
-                       h->stream, h->d_maps, src, dst, G, IDS, h->ew, h->eallowed,       \
-                       h->pallowed, h->ppos, h->ilist, h->ioff, h->order, slot, probe);  \
+                       h->stream, h->d_maps, src, dst, G, IDS, h->ew, h->emask,          \
+                       h->pmask, h->ppos, h->ilist, h->ioff, h->order, slot, probe);     \
   else                                                                                   \
     hipLaunchKernelGGL((sl_e_side<T, E, O, ID>), grid, block, 0, h->stream, h->d_maps,   \
-                       src, dst, G, IDS, h->ew, h->eallowed, h->pallowed, h->ppos,       \
+                       src, dst, G, IDS, h->ew, h->emask, h->pmask, h->ppos,             \
                        h->epos, h->inside, h->order, slot, probe)
 #define PE_E_SIDE(E, O)                                            \
   do {                                                             \
