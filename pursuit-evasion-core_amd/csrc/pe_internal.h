@@ -58,7 +58,9 @@ struct pe_sl {
   long long pairs = 0, g_size = 0, active = 0;
   // car tables
   int *pids = nullptr, *eids = nullptr, *ppos = nullptr, *epos = nullptr;
-  void *erec = nullptr;  // E stencils as interleaved 16-bit ids + weights (every e.n < 65,536)
+  // E stencils as records of 16-bit relative LDS-tile offsets + weights
+  // (LDS E side, when every offset fits)
+  void *erec = nullptr;
   // E indices of the inside pairs of each (map, P position), ascending:
   // ilist[ioff[k] .. ioff[k + 1]) for k = SlMap::ipos_off + position.
   int *ilist = nullptr, *ioff = nullptr;

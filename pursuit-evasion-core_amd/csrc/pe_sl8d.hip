@@ -231,8 +231,43 @@ static int do_sweeps(pe_sl *h, int count, int flags) {
 
 // Packed E records (16-bit byte offsets into a G row) when every map's row
 // of E values spans at most 64 KiB.
-static bool packed_records(const pe_sl *h) {
-  return (long long)(h->ne_max - 1) * (long long)h->esize <= 65535;
+// Packed E stencil records of every map, for the LDS E side: per (state,
+// command) the 16 vertices as signed 16-bit byte offsets in the interleaved
+// tile ([e][na] values of `esize` bytes) relative to the state's own
+// values, then the 16 weights.  A vertex of zero weight, and every vertex of
+// a command the state does not allow (never evaluated), gets offset 0.
+// False when some offset does not fit 16 bits: then the E side reads the
+// 32-bit id tables.
+static bool pack_erecs(const pe_sl_game *games, int n_maps, int na, size_t esize,
+                       std::vector<ERec> &out) {
+  out.clear();
+  for (int b = 0; b < n_maps; b++) {
+    const pe_sl_car &c = games[b].e;
+    for (long long r = 0; r < (long long)c.n * na; r++) {
+      const long long self = r / na;
+      const bool allowed = c.allowed[r] != 0;
+      ERec rec;
+      unsigned short off[kStencil];
+      float *w = reinterpret_cast<float *>(&rec.w[0]);
+      for (int k = 0; k < kStencil; k++) {
+        const float wk = c.weights[r * kStencil + k];
+        w[k] = wk;
+        long long d = 0;
+        if (allowed && wk != 0.f) d = ((long long)c.successors[r * kStencil + k] - self) * na *
+                                      (long long)esize;
+        if (d < -32768 || d > 32767) {
+          out.clear();
+          return false;
+        }
+        off[k] = (unsigned short)(short)d;
+      }
+      unsigned *ids = reinterpret_cast<unsigned *>(&rec.ids[0]);
+      for (int k = 0; k < kStencil / 2; k++)
+        ids[k] = (unsigned)off[2 * k] | ((unsigned)off[2 * k + 1] << 16);
+      out.push_back(rec);
+    }
+  }
+  return true;
 }
 
 extern "C" int pe_sl_create(const pe_sl_game *games, int32_t n_maps,
@@ -344,6 +379,8 @@ extern "C" int pe_sl_create_ex(const pe_sl_game *games, int32_t n_maps, int32_t 
     pe_sl_destroy(h);
     return code;
   };
+  std::vector<ERec> erecs;
+  const bool packed = pack_erecs(games, n_maps, na, h->esize, erecs);
   auto body = [&]() -> int {
     PE_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     // LDS E side when the G rows of one P state fit a workgroup's LDS.
@@ -359,7 +396,7 @@ extern "C" int pe_sl_create_ex(const pe_sl_game *games, int32_t n_maps, int32_t 
           double fill[3];
           for (int k = 0; k < 3; k++)
             fill[k] = lds_slots[k] ? (double)h->active / (double)lds_slots[k] : 1.0;
-          h->lds_threads = sl_lds_threads(bytes, fill, packed_records(h), h->na <= 9);
+          h->lds_threads = sl_lds_threads(bytes, fill, packed, h->na <= 9);
         }
       }
     }
@@ -451,13 +488,10 @@ extern "C" int pe_sl_create_ex(const pe_sl_game *games, int32_t n_maps, int32_t 
       PE_HIP(hipMalloc((void **)&h->pcount, std::max<size_t>(n, 1)));
       PE_HIP(hipMemcpy(h->pcount, cnt.data(), n, hipMemcpyHostToDevice));
     }
-    if (packed_records(h)) {
-      const long long n = erow * (long long)na;  // records
-      PE_HIP(hipMalloc(&h->erec, sizeof(ERec) * std::max(n, 1ll)));
-      hipLaunchKernelGGL(sl_pack_erec, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0,
-                         h->eids, h->ew, (ERec *)h->erec, n, (int)h->esize);
-      PE_LAUNCHED(sl_pack_erec);
-      PE_HIP(hipDeviceSynchronize());
+    if (h->lds_bytes && packed) {  // only the LDS E side reads them
+      PE_HIP(hipMalloc(&h->erec, sizeof(ERec) * std::max<size_t>(erecs.size(), 1)));
+      PE_HIP(hipMemcpy(h->erec, erecs.data(), sizeof(ERec) * erecs.size(),
+                       hipMemcpyHostToDevice));
     }
     PE_HIP(hipMalloc((void **)&h->ilist, sizeof(int) * std::max<size_t>(ilist.size(), 1)));
     PE_HIP(hipMalloc((void **)&h->ioff, sizeof(int) * ioff.size()));

@@ -123,9 +123,9 @@ __device__ __forceinline__ int nth_command(int i, int first) {
 }
 
 // E stencil ids of one (state, command): 16 values of type ID, in vertex
-// order: int ids, or, in the packed records, 16-bit byte offsets (id times
-// the value size, so every map's E states must span at most 64 KiB of a G
-// row).
+// order: int E-state ids, or, in the packed records, signed 16-bit byte
+// offsets in the interleaved LDS tile relative to the E state's own values
+// (zero for a vertex of zero weight).
 template <typename ID>
 struct EIds;
 template <>
@@ -145,15 +145,16 @@ struct EIds<int> {
 template <>
 struct EIds<unsigned short> {
   uint4 v[2];
+  // offset of vertex k, sign-extended (one SDWA add with the base)
   __device__ __forceinline__ int operator[](int k) const {
     const uint4 &a = v[k >> 3];
     const int r = (k >> 1) & 3;
     const unsigned w = r == 0 ? a.x : r == 1 ? a.y : r == 2 ? a.z : a.w;
-    return (int)((k & 1) ? (w >> 16) : (w & 0xFFFFu));
+    return (int)(short)((k & 1) ? (w >> 16) : w);
   }
 };
 
-// Interleaved 16-bit ids and weights of one (state, command): 96 bytes.
+// Interleaved 16-bit offsets and weights of one (state, command): 96 bytes.
 struct ERec {
   uint4 ids[2];
   float4 w[4];
@@ -163,17 +164,12 @@ struct ERec {
 // the separate 32-bit id and float weight tables (ID = int).
 template <typename ID>
 struct EStencil {
-  static constexpr bool kBytes = std::is_same<ID, ERec>::value;
-  EIds<typename std::conditional<kBytes, unsigned short, int>::type> id;
+  EIds<typename std::conditional<std::is_same<ID, ERec>::value, unsigned short, int>::type> id;
   float4 w[4];
-  // Value of vertex k in the row Gu: the packed records hold byte offsets
-  // (an add to the row address), the tables ids (scaled by the value size).
-  template <typename T>
-  __device__ __forceinline__ T at(const T *Gu, int k) const {
-    if constexpr (kBytes)
-      return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(Gu) + id[k]);
-    else
-      return Gu[id[k]];
+  __device__ __forceinline__ float weight(int k) const {
+    const float4 &a = w[k >> 2];
+    const int r = k & 3;
+    return r == 0 ? a.x : r == 1 ? a.y : r == 2 ? a.z : a.w;
   }
   __device__ __forceinline__ void load(const ID *__restrict__ eids,
                                        const float *__restrict__ ew, long long cmd) {
@@ -190,44 +186,116 @@ struct EStencil {
       for (int c = 0; c < 4; c++) w[c] = w4[c];
     }
   }
-  // Two stencils over the same row at once (two independent chains).
-  template <typename T>
-  __device__ __forceinline__ static void eval2(const EStencil &a, const EStencil &b,
-                                               const T *Gu, T &qa, T &qb) {
+};
+
+template <typename T>
+__device__ __forceinline__ unsigned lds_address(const T *p) {
+  return (unsigned)(uintptr_t)(const __attribute__((address_space(3))) T *)p;
+}
+template <typename T>
+__device__ __forceinline__ T lds_load(unsigned address) {
+  return *(const __attribute__((address_space(3))) T *)(uintptr_t)address;
+}
+
+// The G values of one P state staged in LDS, interleaved: the value of E
+// state e under P command u at element e * na + u.  The values of one
+// vertex under all P commands are adjacent, so one vertex address serves
+// every command (the command becomes the immediate offset of the LDS read).
+template <typename T>
+struct LdsTile {
+  // LDS byte address of element 0 (int ids), or of the values of the
+  // pair's own E state (the packed records' offsets are relative to it)
+  unsigned base;
+  int na;
+  // byte address of vertex k of s under command 0, plus `add`
+  template <typename ID>
+  __device__ __forceinline__ unsigned vertex(const EStencil<ID> &s, int k, unsigned add) const {
+    if constexpr (std::is_same<ID, ERec>::value)
+      return add + (unsigned)s.id[k];
+    else
+      return add + __umul24((unsigned)s.id[k], (unsigned)na * (unsigned)sizeof(T));
+  }
+  // sum_k w_k G_u[id_k], vertex order (grid_game.py:406-408)
+  template <typename ID>
+  __device__ __forceinline__ T eval(const EStencil<ID> &s, int u) const {
+    const unsigned b = base + (unsigned)u * (unsigned)sizeof(T);
+    T q = 0;
+#pragma unroll
+    for (int k = 0; k < kStencil; k++) q = madd(s.weight(k), lds_load<T>(vertex(s, k, b)), q);
+    return q;
+  }
+  // Two stencils under the same command at once (two independent chains).
+  template <typename ID>
+  __device__ __forceinline__ void eval2(const EStencil<ID> &sa, const EStencil<ID> &sb, int u,
+                                        T &qa, T &qb) const {
+    const unsigned b = base + (unsigned)u * (unsigned)sizeof(T);
     qa = 0;
     qb = 0;
 #pragma unroll
-    for (int c = 0; c < kStencil / 4; c++) {
-      qa = madd(a.w[c].x, a.at(Gu, 4 * c), qa);
-      qb = madd(b.w[c].x, b.at(Gu, 4 * c), qb);
-      qa = madd(a.w[c].y, a.at(Gu, 4 * c + 1), qa);
-      qb = madd(b.w[c].y, b.at(Gu, 4 * c + 1), qb);
-      qa = madd(a.w[c].z, a.at(Gu, 4 * c + 2), qa);
-      qb = madd(b.w[c].z, b.at(Gu, 4 * c + 2), qb);
-      qa = madd(a.w[c].w, a.at(Gu, 4 * c + 3), qa);
-      qb = madd(b.w[c].w, b.at(Gu, 4 * c + 3), qb);
+    for (int k = 0; k < kStencil; k++) {
+      qa = madd(sa.weight(k), lds_load<T>(vertex(sa, k, b)), qa);
+      qb = madd(sb.weight(k), lds_load<T>(vertex(sb, k, b)), qb);
     }
   }
-  // sum_k w_k G[id_k], vertex order (grid_game.py:406-408)
-  template <typename T>
-  __device__ __forceinline__ T eval(const T *Gu) const {
+  // The vertex addresses of one stencil, for evaluating it under several
+  // commands.
+  struct Verts {
+    unsigned a[kStencil];
+  };
+  template <typename ID>
+  __device__ __forceinline__ Verts verts(const EStencil<ID> &s) const {
+    Verts v;
+#pragma unroll
+    for (int k = 0; k < kStencil; k++) v.a[k] = vertex(s, k, base);
+    return v;
+  }
+  // eval(s, u) from the vertex addresses; u a compile-time constant here.
+  template <typename ID>
+  __device__ __forceinline__ T eval_at(const Verts &v, const EStencil<ID> &s, int u) const {
     T q = 0;
 #pragma unroll
-    for (int c = 0; c < kStencil / 4; c++) {
-      q = madd(w[c].x, at(Gu, 4 * c), q);
-      q = madd(w[c].y, at(Gu, 4 * c + 1), q);
-      q = madd(w[c].z, at(Gu, 4 * c + 2), q);
-      q = madd(w[c].w, at(Gu, 4 * c + 3), q);
-    }
+    for (int k = 0; k < kStencil; k++)
+      q = madd(s.weight(k), lds_load<T>(v.a[k] + (unsigned)u * (unsigned)sizeof(T)), q);
     return q;
   }
 };
 
+// The G rows of one P state in global memory: row u at p + u * ustride
+// (G is [na][np][ne] per map), gathered with the 32-bit ids.
+template <typename T>
+struct GlobalRows {
+  const T *__restrict__ p;
+  long long ustride;
+  __device__ __forceinline__ T eval(const EStencil<int> &s, int u) const {
+    const T *__restrict__ Gu = p + u * ustride;
+    T q = 0;
+#pragma unroll
+    for (int k = 0; k < kStencil; k++) q = madd(s.weight(k), Gu[s.id[k]], q);
+    return q;
+  }
+  __device__ __forceinline__ void eval2(const EStencil<int> &sa, const EStencil<int> &sb, int u,
+                                        T &qa, T &qb) const {
+    const T *__restrict__ Gu = p + u * ustride;
+    qa = 0;
+    qb = 0;
+#pragma unroll
+    for (int k = 0; k < kStencil; k++) {
+      qa = madd(sa.weight(k), Gu[sa.id[k]], qa);
+      qb = madd(sb.weight(k), Gu[sb.id[k]], qb);
+    }
+  }
+  struct Verts {};
+  __device__ __forceinline__ Verts verts(const EStencil<int> &) const { return {}; }
+  __device__ __forceinline__ T eval_at(const Verts &, const EStencil<int> &s, int u) const {
+    return eval(s, u);
+  }
+};
+
 // Min over allowed P commands of the max over allowed E commands of one
-// inside pair (ip, ie), starting from its current value; row(u) is the
-// row G_u(ip, .).  Early exit: a P command whose running max over E
-// commands reaches the running min cannot lower it (grid_game.py:409-421).
-// Updates the stored command order of the pair.
+// inside pair (ip, ie), starting from its current value; `tile` holds the
+// values G_u(ip, .) of the P state.  Early exit: a P command whose running
+// max over E commands reaches the running min cannot lower it
+// (grid_game.py:409-421).  Updates the stored command order of the pair.
 //
 // ORDER selects the visiting order:
 //   kPlainOrder   commands in index order, u-major, as in the reference;
@@ -236,10 +304,10 @@ struct EStencil {
 //   kOnePass      u* first, u-major; then every other allowed P command is
 //                 carried along one pass over the E commands, so each E
 //                 stencil is read once for all of them instead of once per P
-//                 command (more registers: ~96 VGPRs against ~48);
+//                 command (more registers);
 //   kOnePass9     the same with room for 9 commands per state instead of
 //                 15 (the {-A, 0, A} x {-K, 0, K} set): the running maxima
-//                 take 6 registers fewer, ~73 VGPRs in all.
+//                 take 6 registers fewer.
 // The value and the stored commands are the same in all three: max and min
 // are exact, each (u, b) combination is summed in the same order, ties go
 // to the first command in visiting order, and a command cut against the
@@ -247,9 +315,9 @@ struct EStencil {
 // value at or above the final min).
 constexpr int kPlainOrder = 0, kUMajor = 1, kOnePass = 2, kOnePass9 = 3;
 
-template <bool EARLY, int ORDER, typename T, typename Row, typename ID>
+template <bool EARLY, int ORDER, typename T, typename Tile, typename ID>
 __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T start,
-                                         long long pair, Row row,
+                                         long long pair, const Tile &tile,
                                          const ID *__restrict__ eids,
                                          const float *__restrict__ ew,
                                          const uint16_t *__restrict__ emask,
@@ -280,7 +348,6 @@ __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T s
     // two independent sums; the second result is dropped when the first
     // already cuts, so the visited sequence is the sequential one.
     if ((pm >> ufirst) & 1u) {
-      const T *Gu = row(ufirst);
       unsigned ea = ea_visit;
       T worst = T(-1e30);
       int arg = -1;
@@ -295,7 +362,7 @@ __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T s
         s0.load(eids, ew, cmd0 + b0);
         s1.load(eids, ew, cmd0 + b1);
         T q0, q1;
-        EStencil<ID>::eval2(s0, s1, Gu, q0, q1);
+        tile.eval2(s0, s1, ufirst, q0, q1);
         if (q0 > worst) {
           worst = q0;
           arg = b0;
@@ -320,7 +387,6 @@ __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T s
   for (int i = 0; i < (ONE_PASS ? 0 : m.na); i++) {
     const int u = ORDERED ? nth_command(i, ufirst) : i;
     if (!((pm >> u) & 1u)) continue;
-    const T *Gu = row(u);
     T worst = T(-1e30);
     int arg = -1;
     bool cut = false;
@@ -329,7 +395,7 @@ __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T s
       if (!((em >> b) & 1u)) continue;
       EStencil<ID> st;
       st.load(eids, ew, cmd0 + b);
-      const T q = st.eval(Gu);
+      const T q = tile.eval(st, u);
       if (q > worst) {
         worst = q;
         arg = b;
@@ -355,10 +421,11 @@ __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T s
       const int b = nth_command(__builtin_ctz(todo), bfirst);
       EStencil<ID> st;
       st.load(eids, ew, cmd0 + b);
+      const auto vt = tile.verts(st);
 #pragma unroll
       for (int u = 0; u < MAXA; u++) {
         if (!((alive >> u) & 1u)) continue;
-        const T q = st.eval(row(u));
+        const T q = tile.eval_at(vt, st, u);
         if (q > worst[u]) {
           worst[u] = q;
           args = (args & ~(15ull << (4 * u))) | ((unsigned long long)b << (4 * u));
@@ -395,12 +462,13 @@ __device__ __forceinline__ void block_residual(T change, unsigned long long *slo
   }
 }
 
-// E side with G gathered from global memory: one thread per pair.
-template <typename T, bool EARLY, int ORDER, typename ID>
+// E side with G gathered from global memory: one thread per pair, with the
+// 32-bit id tables.
+template <typename T, bool EARLY, int ORDER>
 __global__ __launch_bounds__(256) void sl_e_side(
     const SlMap *__restrict__ maps, const T *__restrict__ V,
     T *__restrict__ Vn, const T *__restrict__ G,
-    const ID *__restrict__ eids, const float *__restrict__ ew,
+    const int *__restrict__ eids, const float *__restrict__ ew,
     const uint16_t *__restrict__ emask, const uint16_t *__restrict__ pmask,
     const int *__restrict__ ppos, const int *__restrict__ epos,
     const uint8_t *__restrict__ inside, uint8_t *__restrict__ order,
@@ -418,8 +486,8 @@ __global__ __launch_bounds__(256) void sl_e_side(
       T best = start;
       const int prow = m.p_row + ip;
       if (inside[m.tab_off + (long long)ppos[prow] * m.npos_e + ep]) {
-        auto row = [&](int u) { return G + m.g_off + ((long long)u * m.np + ip) * m.ne; };
-        best = pair_backup<EARLY, ORDER>(m, prow, erow, start, pair, row, eids, ew,
+        const GlobalRows<T> rows{G + m.g_off + (long long)ip * m.ne, (long long)m.np * m.ne};
+        best = pair_backup<EARLY, ORDER>(m, prow, erow, start, pair, rows, eids, ew,
                                            emask, pmask, order);
       }
       Vn[pair] = best;
@@ -433,8 +501,8 @@ __global__ __launch_bounds__(256) void sl_e_side(
 // E side with the G rows of one P state staged in LDS: one block per
 // (P state, map) walks the inside pairs of that P state, so the 16-point E
 // gathers of every pair read LDS instead of scattered cache lines.  Dynamic
-// LDS holds [na][ne] floats; only the rows of allowed P commands are filled,
-// and only those are read.  The pairs come from the list of inside E states
+// LDS holds the P state's G values as [ne][na] (see LdsTile); only the
+// values of allowed P commands are read.  The pairs come from the list of inside E states
 // of the P state's position, so every lane has a pair to update; the pairs
 // outside the exit radius are not written (both value buffers already hold
 // the same exit value there).  Same arithmetic as sl_e_side, value for value.
@@ -460,30 +528,40 @@ __global__ __launch_bounds__(kLdsMaxThreads) void sl_e_side_lds(
   const int prow = m.p_row + ip;
   if (active) {
     // All na rows (rows of disallowed commands are read but never used),
-    // flattened as tile[u * ne + e], eight loads in flight per thread.
+    // read row by row (t = u * ne + e, coalesced) with eight loads in
+    // flight per thread and written to tile[e * na + u].
     const int total = m.na * m.ne;
     const T *__restrict__ src = G + m.g_off + (long long)ip * m.ne;
     const long long ustride = (long long)m.np * m.ne;
     constexpr int kBatch = 8;
     const int nt = (int)blockDim.x;
-    int u = threadIdx.x / m.ne, e = threadIdx.x - u * m.ne;  // of t0
+    // Element t0 = u * ne + e is read from src[u * ustride + e] (index g)
+    // and written to tile[e * na + u] (index at); both advance by adds.
+    const int u0 = threadIdx.x / m.ne;
+    int e = threadIdx.x - u0 * m.ne;
+    long long g = u0 * ustride + e;
+    int at = e * m.na + u0;
+    const int at_step = nt * m.na, at_wrap = 1 - m.ne * m.na;
+    const long long g_wrap = ustride - m.ne;
     for (int t0 = threadIdx.x; t0 < total; t0 += kBatch * nt) {
       T v[kBatch];
-      int uu = u, ee = e;
+      int to[kBatch];
 #pragma unroll
       for (int b = 0; b < kBatch; b++) {
-        if (t0 + b * nt < total) v[b] = src[uu * ustride + ee];
-        ee += nt;
-        while (ee >= m.ne) {
-          ee -= m.ne;
-          uu++;
+        if (t0 + b * nt < total) v[b] = src[g];
+        to[b] = at;
+        e += nt;
+        g += nt;
+        at += at_step;
+        while (e >= m.ne) {
+          e -= m.ne;
+          g += g_wrap;
+          at += at_wrap;
         }
       }
 #pragma unroll
       for (int b = 0; b < kBatch; b++)
-        if (t0 + b * nt < total) tile[t0 + b * nt] = v[b];
-      u = uu;
-      e = ee;
+        if (t0 + b * nt < total) tile[to[b]] = v[b];
     }
   }
   __syncthreads();
@@ -491,13 +569,18 @@ __global__ __launch_bounds__(kLdsMaxThreads) void sl_e_side_lds(
   if (active) {
     const int k0 = ioff[m.ipos_off + ppos[prow]];
     const int count = ioff[m.ipos_off + ppos[prow] + 1] - k0;
-    auto row = [&](int u) { return tile + u * m.ne; };
+    const unsigned tile_at = lds_address(tile);
     for (int k = threadIdx.x; k < count; k += blockDim.x) {
       const int ie = ilist[k0 + k];
+      const LdsTile<T> lt{
+          std::is_same<ID, ERec>::value
+              ? tile_at + __umul24((unsigned)ie, (unsigned)m.na * (unsigned)sizeof(T))
+              : tile_at,
+          m.na};
       const int erow = m.e_row + ie;
       const long long pair = m.v_off + (long long)ip * m.ne + ie;
       const T start = probe ? T(3.0e38f) : V[pair];
-      const T best = pair_backup<EARLY, ORDER>(m, prow, erow, start, pair, row, eids, ew,
+      const T best = pair_backup<EARLY, ORDER>(m, prow, erow, start, pair, lt, eids, ew,
                                                  emask, pmask, order);
       Vn[pair] = best;
       const T d = best > start ? best - start : start - best;
@@ -548,22 +631,20 @@ void enqueue_sweep_t(pe_sl *h, const void *src_, void *dst_, int flags,
   const bool packed = h->erec && !(flags & PE_SL_WIDE_IDS);
   // One pass over the E commands unless u-major is asked for.
   const bool one_pass = !(flags & PE_SL_U_MAJOR);
-#define PE_E_SIDE_ID(E, O, ID, IDS)                                                    \
-  if (lds)                                                                             \
-    hipLaunchKernelGGL((sl_e_side_lds<T, E, O, ID>), lds_grid, lds_block, h->lds_bytes,  \
-                       h->stream, h->d_maps, src, dst, G, IDS, h->ew, h->emask,          \
-                       h->pmask, h->ppos, h->ilist, h->ioff, h->order, slot, probe);     \
-  else                                                                                   \
-    hipLaunchKernelGGL((sl_e_side<T, E, O, ID>), grid, block, 0, h->stream, h->d_maps,   \
-                       src, dst, G, IDS, h->ew, h->emask, h->pmask, h->ppos,             \
-                       h->epos, h->inside, h->order, slot, probe)
-#define PE_E_SIDE(E, O)                                            \
-  do {                                                             \
-    if (packed) {                                                  \
-      PE_E_SIDE_ID(E, O, ERec, (const ERec *)h->erec);             \
-    } else {                                                       \
-      PE_E_SIDE_ID(E, O, int, h->eids);                            \
-    }                                                              \
+#define PE_E_SIDE_LDS(E, O, ID, IDS)                                                   \
+  hipLaunchKernelGGL((sl_e_side_lds<T, E, O, ID>), lds_grid, lds_block, h->lds_bytes,    \
+                     h->stream, h->d_maps, src, dst, G, IDS, h->ew, h->emask, h->pmask,  \
+                     h->ppos, h->ilist, h->ioff, h->order, slot, probe)
+#define PE_E_SIDE(E, O)                                                                \
+  do {                                                                                 \
+    if (lds && packed)                                                                 \
+      PE_E_SIDE_LDS(E, O, ERec, (const ERec *)h->erec);                                \
+    else if (lds)                                                                      \
+      PE_E_SIDE_LDS(E, O, int, h->eids);                                               \
+    else                                                                               \
+      hipLaunchKernelGGL((sl_e_side<T, E, O>), grid, block, 0, h->stream, h->d_maps,   \
+                         src, dst, G, h->eids, h->ew, h->emask, h->pmask, h->ppos,     \
+                         h->epos, h->inside, h->order, slot, probe);                   \
   } while (0)
   const bool nine = h->na <= 9;
   if (early && ordered && one_pass && nine) PE_E_SIDE(true, kOnePass9);
@@ -575,31 +656,8 @@ void enqueue_sweep_t(pe_sl *h, const void *src_, void *dst_, int flags,
   else if (ordered) PE_E_SIDE(false, kUMajor);
   else PE_E_SIDE(false, kPlainOrder);
 #undef PE_E_SIDE
-#undef PE_E_SIDE_ID
+#undef PE_E_SIDE_LDS
   if (record) sl_record(h, 1, false);
-}
-
-// Interleaved records of the E stencils: 16-bit byte offsets (id times the
-// value size `scale`, every offset below 65,536) then the 16 weights, one
-// record per (state, command).
-__global__ __launch_bounds__(256) inline void sl_pack_erec(const int *__restrict__ ids,
-                                                           const float *__restrict__ w,
-                                                           ERec *__restrict__ out,
-                                                           long long n, int scale) {
-  const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (r >= n) return;
-  unsigned short id[16];
-  float wt[16];
-  for (int k = 0; k < 16; k++) {
-    id[k] = (unsigned short)(ids[r * 16 + k] * scale);
-    wt[k] = w[r * 16 + k];
-  }
-  ERec rec;
-  unsigned *u = reinterpret_cast<unsigned *>(&rec.ids[0]);
-  for (int k = 0; k < 8; k++) u[k] = (unsigned)id[2 * k] | ((unsigned)id[2 * k + 1] << 16);
-  float *f = reinterpret_cast<float *>(&rec.w[0]);
-  for (int k = 0; k < 16; k++) f[k] = wt[k];
-  out[r] = rec;
 }
 
 // Lets the LDS E side use `bytes` of dynamic LDS for every flag variant;
