@@ -343,47 +343,6 @@ __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T s
   // bfirst, then the others ascending)
   const unsigned ea_visit = ((em >> bfirst) & 1u) | ((em & ((1u << bfirst) - 1u)) << 1) |
                             (em & ~((2u << bfirst) - 1u));
-  if constexpr (ONE_PASS) {
-    // u* over its E commands two at a time: two stencil reads in flight and
-    // two independent sums; the second result is dropped when the first
-    // already cuts, so the visited sequence is the sequential one.
-    if ((pm >> ufirst) & 1u) {
-      unsigned ea = ea_visit;
-      T worst = T(-1e30);
-      int arg = -1;
-      bool cut = false;
-      while (ea && !cut) {
-        const int b0 = nth_command(__builtin_ctz(ea), bfirst);
-        ea &= ea - 1;
-        const bool two = ea != 0;
-        const int b1 = two ? nth_command(__builtin_ctz(ea), bfirst) : b0;
-        if (two) ea &= ea - 1;
-        EStencil<ID> s0, s1;
-        s0.load(eids, ew, cmd0 + b0);
-        s1.load(eids, ew, cmd0 + b1);
-        T q0, q1;
-        tile.eval2(s0, s1, ufirst, q0, q1);
-        if (q0 > worst) {
-          worst = q0;
-          arg = b0;
-        }
-        if (EARLY && worst >= best) {
-          cut = true;
-        } else if (two) {
-          if (q1 > worst) {
-            worst = q1;
-            arg = b1;
-          }
-          if (EARLY && worst >= best) cut = true;
-        }
-      }
-      if (!cut && worst < best) {
-        best = worst;
-        best_u = ufirst;
-        best_b = arg;
-      }
-    }
-  }
   for (int i = 0; i < (ONE_PASS ? 0 : m.na); i++) {
     const int u = ORDERED ? nth_command(i, ufirst) : i;
     if (!((pm >> u) & 1u)) continue;
@@ -412,12 +371,83 @@ __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T s
     }
   }
   if constexpr (ONE_PASS) {
-    unsigned alive = pm & ~(1u << ufirst);
-    T worst[MAXA];
-    unsigned long long args = 0;  // 4 bits per P command
+    T worst[MAXA];                // running max of each P command
+    unsigned long long args = 0;  // its argmax, 4 bits per P command
 #pragma unroll
     for (int u = 0; u < MAXA; u++) worst[u] = T(-1e30);
-    for (unsigned todo = ea_visit; todo && alive; todo &= todo - 1) {
+    unsigned todo = ea_visit;
+    T wu = T(-1e30);  // u*: running max and argmax
+    int arg = -1;
+    bool cut = false;
+    if (todo) {
+      // The first visited E command under every allowed P command, u*
+      // included: one stencil read and one set of vertex addresses for all.
+      // Only u*'s result is judged now; the others are judged against the
+      // best value once u* is done, as if visited after it.
+      const int bf = nth_command(__builtin_ctz(todo), bfirst);
+      todo &= todo - 1;
+      EStencil<ID> st;
+      st.load(eids, ew, cmd0 + bf);
+      const auto vt = tile.verts(st);
+#pragma unroll
+      for (int u = 0; u < MAXA; u++) {
+        if (!((pm >> u) & 1u)) continue;
+        const T q = tile.eval_at(vt, st, u);
+        if (q > worst[u]) {
+          worst[u] = q;
+          args |= (unsigned long long)bf << (4 * u);
+          if (u == ufirst) {
+            wu = q;
+            arg = bf;
+          }
+        }
+      }
+      cut = EARLY && wu >= best;
+    }
+    if ((pm >> ufirst) & 1u) {
+      // u* over its other E commands two at a time: two stencil reads in
+      // flight and two independent sums; the second result is dropped when
+      // the first already cuts, so the visited sequence is the sequential
+      // one.
+      unsigned ea = todo;
+      while (ea && !cut) {
+        const int b0 = nth_command(__builtin_ctz(ea), bfirst);
+        ea &= ea - 1;
+        const bool two = ea != 0;
+        const int b1 = two ? nth_command(__builtin_ctz(ea), bfirst) : b0;
+        if (two) ea &= ea - 1;
+        EStencil<ID> s0, s1;
+        s0.load(eids, ew, cmd0 + b0);
+        s1.load(eids, ew, cmd0 + b1);
+        T q0, q1;
+        tile.eval2(s0, s1, ufirst, q0, q1);
+        if (q0 > wu) {
+          wu = q0;
+          arg = b0;
+        }
+        if (EARLY && wu >= best) {
+          cut = true;
+        } else if (two) {
+          if (q1 > wu) {
+            wu = q1;
+            arg = b1;
+          }
+          if (EARLY && wu >= best) cut = true;
+        }
+      }
+      if (!cut && wu < best) {
+        best = wu;
+        best_u = ufirst;
+        best_b = arg;
+      }
+    }
+    unsigned alive = pm & ~(1u << ufirst);
+    if (EARLY) {
+#pragma unroll
+      for (int u = 0; u < MAXA; u++)
+        if (worst[u] >= best) alive &= ~(1u << u);
+    }
+    for (; todo && alive; todo &= todo - 1) {
       const int b = nth_command(__builtin_ctz(todo), bfirst);
       EStencil<ID> st;
       st.load(eids, ew, cmd0 + b);
