@@ -301,18 +301,18 @@ struct GlobalRows {
 //   kPlainOrder   commands in index order, u-major, as in the reference;
 //   kUMajor       the stored best commands (u*, b*) first, then the others
 //                 in index order, u-major;
-//   kOnePass      u* first, u-major; then every other allowed P command is
-//                 carried along one pass over the E commands, so each E
-//                 stencil is read once for all of them instead of once per P
-//                 command (more registers);
+//   kOnePass      one pass over the E commands (b* first) carrying every
+//                 allowed P command along, so each E stencil is read once
+//                 for all of them instead of once per P command (more
+//                 registers); u* is judged first at the end;
 //   kOnePass9     the same with room for 9 commands per state instead of
 //                 15 (the {-A, 0, A} x {-K, 0, K} set): the running maxima
 //                 take 6 registers fewer.
 // The value and the stored commands are the same in all three: max and min
 // are exact, each (u, b) combination is summed in the same order, ties go
-// to the first command in visiting order, and a command cut against the
-// bound of u* could not have been chosen later (its max already reaches a
-// value at or above the final min).
+// to the first command in visiting order, and a command cut against a
+// bound could not have been chosen later (its max already reaches a value
+// at or above the final min).
 constexpr int kPlainOrder = 0, kUMajor = 1, kOnePass = 2, kOnePass9 = 3;
 
 template <bool EARLY, int ORDER, typename T, typename Tile, typename ID>
@@ -371,83 +371,19 @@ __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T s
     }
   }
   if constexpr (ONE_PASS) {
+    // One pass over the E commands in visiting order carries every allowed
+    // P command, u* included: each E stencil record is read once per pair.
+    // A command is cut once its running max reaches the pair's start value
+    // (it cannot lower the value).  u*, if it survives, then lowers the
+    // bound, and the other survivors are judged against that: the same
+    // outcome as visiting u* first, since the start value is at or above
+    // every later bound and the running maxima only grow.
     T worst[MAXA];                // running max of each P command
     unsigned long long args = 0;  // its argmax, 4 bits per P command
 #pragma unroll
     for (int u = 0; u < MAXA; u++) worst[u] = T(-1e30);
-    unsigned todo = ea_visit;
-    T wu = T(-1e30);  // u*: running max and argmax
-    int arg = -1;
-    bool cut = false;
-    if (todo) {
-      // The first visited E command under every allowed P command, u*
-      // included: one stencil read and one set of vertex addresses for all.
-      // Only u*'s result is judged now; the others are judged against the
-      // best value once u* is done, as if visited after it.
-      const int bf = nth_command(__builtin_ctz(todo), bfirst);
-      todo &= todo - 1;
-      EStencil<ID> st;
-      st.load(eids, ew, cmd0 + bf);
-      const auto vt = tile.verts(st);
-#pragma unroll
-      for (int u = 0; u < MAXA; u++) {
-        if (!((pm >> u) & 1u)) continue;
-        const T q = tile.eval_at(vt, st, u);
-        if (q > worst[u]) {
-          worst[u] = q;
-          args |= (unsigned long long)bf << (4 * u);
-          if (u == ufirst) {
-            wu = q;
-            arg = bf;
-          }
-        }
-      }
-      cut = EARLY && wu >= best;
-    }
-    if ((pm >> ufirst) & 1u) {
-      // u* over its other E commands two at a time: two stencil reads in
-      // flight and two independent sums; the second result is dropped when
-      // the first already cuts, so the visited sequence is the sequential
-      // one.
-      unsigned ea = todo;
-      while (ea && !cut) {
-        const int b0 = nth_command(__builtin_ctz(ea), bfirst);
-        ea &= ea - 1;
-        const bool two = ea != 0;
-        const int b1 = two ? nth_command(__builtin_ctz(ea), bfirst) : b0;
-        if (two) ea &= ea - 1;
-        EStencil<ID> s0, s1;
-        s0.load(eids, ew, cmd0 + b0);
-        s1.load(eids, ew, cmd0 + b1);
-        T q0, q1;
-        tile.eval2(s0, s1, ufirst, q0, q1);
-        if (q0 > wu) {
-          wu = q0;
-          arg = b0;
-        }
-        if (EARLY && wu >= best) {
-          cut = true;
-        } else if (two) {
-          if (q1 > wu) {
-            wu = q1;
-            arg = b1;
-          }
-          if (EARLY && wu >= best) cut = true;
-        }
-      }
-      if (!cut && wu < best) {
-        best = wu;
-        best_u = ufirst;
-        best_b = arg;
-      }
-    }
-    unsigned alive = pm & ~(1u << ufirst);
-    if (EARLY) {
-#pragma unroll
-      for (int u = 0; u < MAXA; u++)
-        if (worst[u] >= best) alive &= ~(1u << u);
-    }
-    for (; todo && alive; todo &= todo - 1) {
+    unsigned alive = pm;
+    for (unsigned todo = ea_visit; todo && alive; todo &= todo - 1) {
       const int b = nth_command(__builtin_ctz(todo), bfirst);
       EStencil<ID> st;
       st.load(eids, ew, cmd0 + b);
@@ -463,7 +399,13 @@ __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T s
         if (EARLY && worst[u] >= best) alive &= ~(1u << u);
       }
     }
-    // Survivors in visiting order (ascending u after u*).
+    // Survivors in visiting order: u*, then ascending u.
+    if (((alive >> ufirst) & 1u) && worst[ufirst] < best) {
+      best = worst[ufirst];
+      best_u = ufirst;
+      best_b = ea_visit ? (int)((args >> (4 * ufirst)) & 15) : -1;
+    }
+    alive &= ~(1u << ufirst);
 #pragma unroll
     for (int u = 0; u < MAXA; u++) {
       if (((alive >> u) & 1u) && worst[u] < best) {
