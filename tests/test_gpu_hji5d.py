@@ -284,7 +284,9 @@ def test_profiling_counts_launches(gpu):
                                               (RelativeGrid(65, 36, 15, 15, 8.0), 3, 2),
                                               (RelativeGrid(33, 24, 9, 9, 4.0), 12, 1),
                                               (RelativeGrid(17, 7, 7, 7, 4.0), 18, 2),
-                                              (RelativeGrid(17, 4, 7, 7, 4.0), 18, 2)])
+                                              (RelativeGrid(17, 4, 7, 7, 4.0), 18, 2),
+                                              (RelativeGrid(17, 6, 5, 8, 4.0), 12, 2),
+                                              (RelativeGrid(17, 6, 5, 8, 4.0), 12, 1)])
 def test_fp32_kernel_variants_are_bit_identical(gpu, grid, steps, order):
     """The FP32 kernel's launch variants do the same arithmetic on the same
     neighbour values: two psi planes per thread (even heading counts; with
