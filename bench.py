@@ -330,6 +330,8 @@ def run_c2(args):
         s.set_profiling(True)
         s.sweep(min(args.steps, 20), flags & ~_native.PE_SL_GRAPH)
         prof = s.stats()
+        s.sweep(min(args.steps, 20), (flags & ~_native.PE_SL_GRAPH) | _native.PE_SL_STENCIL_P_SIDE)
+        prof_s = s.stats()
         s.sweep(min(args.steps, 20), (flags & ~_native.PE_SL_GRAPH) | _native.PE_SL_GLOBAL_GATHER)
         prof_g = s.stats()
         s.sweep(min(args.steps, 20), (flags & ~_native.PE_SL_GRAPH) | _native.PE_SL_WIDE_IDS)
@@ -348,8 +350,12 @@ def run_c2(args):
             "p_side_ms": prof["p_ms"] / max(1, prof["p_launches"]),
             "e_side_lds_threads": st0["lds_e_side"],
             "e_side_ms": prof["e_ms"] / max(1, prof["e_launches"]),
-            "e_side_global_gather_ms": (prof_g["e_ms"] - prof["e_ms"]) /
-                                       max(1, prof_g["e_launches"] - prof["e_launches"]),
+            # P side summing each command's stencil in vertex order instead
+            # of adding each union row to all commands' sums.
+            "p_side_stencil_ms": (prof_s["p_ms"] - prof["p_ms"]) /
+                                 max(1, prof_s["p_launches"] - prof["p_launches"]),
+            "e_side_global_gather_ms": (prof_g["e_ms"] - prof_s["e_ms"]) /
+                                       max(1, prof_g["e_launches"] - prof_s["e_launches"]),
             "e_side_unpacked_tables_ms": (prof_w["e_ms"] - prof_g["e_ms"]) /
                                    max(1, prof_w["e_launches"] - prof_g["e_launches"]),
             # The reference's early exit in index order, without the stored
@@ -435,6 +441,8 @@ def run_c4(args):
         s.set_profiling(True)
         s.sweep(20, flags & ~_native.PE_SL_GRAPH)
         prof = s.stats()
+        s.sweep(20, (flags & ~_native.PE_SL_GRAPH) | _native.PE_SL_STENCIL_P_SIDE)
+        prof_s = s.stats()
         s.sweep(20, _native.PE_SL_EARLY_EXIT)
         prof_r = s.stats()
         s.set_profiling(False)
@@ -450,10 +458,14 @@ def run_c4(args):
             "p_side_ms": prof["p_ms"] / max(1, prof["p_launches"]),
             "e_side_ms": prof["e_ms"] / max(1, prof["e_launches"]),
             "e_side_lds_threads": st0["lds_e_side"],
+            # P side summing each command's stencil in vertex order instead
+            # of adding each union row to all commands' sums.
+            "p_side_stencil_ms": (prof_s["p_ms"] - prof["p_ms"]) /
+                                 max(1, prof_s["p_launches"] - prof["p_launches"]),
             # The reference's early exit in index order, without the stored
             # command order.
-            "e_side_reference_order_ms": (prof_r["e_ms"] - prof["e_ms"]) /
-                                         max(1, prof_r["e_launches"] - prof["e_launches"]),
+            "e_side_reference_order_ms": (prof_r["e_ms"] - prof_s["e_ms"]) /
+                                         max(1, prof_r["e_launches"] - prof_s["e_launches"]),
         }
         cars = [g[2] for g in games] + [g[3] for g in games]
         out["roofline"] = sl_roofline(prof, st0, wall / args.steps, cars,

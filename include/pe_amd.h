@@ -283,6 +283,11 @@ typedef struct pe_sl_game {
                                  weight tables instead of the packed
                                  96-byte records of 16-bit ids + weights
                                  (comparison runs)                         */
+#define PE_SL_STENCIL_P_SIDE 256 /* FP32: P side sums each command's compacted
+                                 stencil in vertex order instead of adding
+                                 each of the P state's union rows to all
+                                 commands' sums (comparison runs; equal to
+                                 rounding)                                 */
 
 typedef struct pe_sl_stats {
   int64_t sweeps;

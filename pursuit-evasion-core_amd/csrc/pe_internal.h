@@ -68,6 +68,12 @@ struct pe_sl {
   // P stencils compacted (non-zero weights first, in vertex order) and the
   // count of non-zero weights per (state, command); pids / pw hold them.
   uint8_t *pcount = nullptr;
+  // FP32 P side from the union of each P state's stencil rows (na <= 9):
+  // P state row i reads urows[uoff[i] .. uoff[i + 1]) (ascending, within
+  // its map) with weights uw[row][9] per command (0 where the command does
+  // not use the row); null when not built
+  int *uoff = nullptr, *urows = nullptr;
+  float *uw = nullptr;
   float *pw = nullptr, *ew = nullptr;
   // allowed commands of each state as a bit mask (bit u), for the E side
   uint16_t *pmask = nullptr, *emask = nullptr;

@@ -487,6 +487,53 @@ extern "C" int pe_sl_create_ex(const pe_sl_game *games, int32_t n_maps, int32_t 
       PE_HIP(hipMemcpy(h->pw, wts.data(), sizeof(float) * wts.size(), hipMemcpyHostToDevice));
       PE_HIP(hipMalloc((void **)&h->pcount, std::max<size_t>(n, 1)));
       PE_HIP(hipMemcpy(h->pcount, cnt.data(), n, hipMemcpyHostToDevice));
+      if (!h->f64 && na <= kUnionCommands) {
+        // Union rows of each P state and their weight per command.
+        std::vector<int> uoff(1, 0), urows;
+        std::vector<float> uw;
+        std::vector<int> rows;
+        for (int b = 0; b < n_maps; b++) {
+          const pe_sl_car &c = games[b].p;
+          for (int i = 0; i < c.n; i++) {
+            rows.clear();
+            for (int u = 0; u < na; u++) {
+              const long long cmd = (long long)i * na + u;
+              if (!c.allowed[cmd]) continue;
+              for (int q = 0; q < kStencil; q++)
+                if (c.weights[cmd * kStencil + q] != 0.f)
+                  rows.push_back(c.successors[cmd * kStencil + q]);
+            }
+            std::sort(rows.begin(), rows.end());
+            rows.erase(std::unique(rows.begin(), rows.end()), rows.end());
+            const size_t real = rows.size();
+            while (rows.size() % kUnionBatch) rows.push_back(rows.back());  // zero weights
+            const size_t at = uw.size();
+            uw.resize(at + rows.size() * kUnionCommands, 0.f);
+            for (int u = 0; u < na; u++) {
+              const long long cmd = (long long)i * na + u;
+              if (!c.allowed[cmd]) continue;
+              for (int q = 0; q < kStencil; q++) {
+                const float w = c.weights[cmd * kStencil + q];
+                if (w == 0.f) continue;
+                const size_t t = std::lower_bound(rows.begin(), rows.begin() + real,
+                                                  c.successors[cmd * kStencil + q]) -
+                                 rows.begin();
+                uw[at + t * kUnionCommands + u] += w;
+              }
+            }
+            urows.insert(urows.end(), rows.begin(), rows.end());
+            uoff.push_back((int)urows.size());
+          }
+        }
+        PE_HIP(hipMalloc((void **)&h->uoff, sizeof(int) * uoff.size()));
+        PE_HIP(hipMemcpy(h->uoff, uoff.data(), sizeof(int) * uoff.size(),
+                         hipMemcpyHostToDevice));
+        PE_HIP(hipMalloc((void **)&h->urows, sizeof(int) * std::max<size_t>(urows.size(), 1)));
+        PE_HIP(hipMemcpy(h->urows, urows.data(), sizeof(int) * urows.size(),
+                         hipMemcpyHostToDevice));
+        PE_HIP(hipMalloc((void **)&h->uw, sizeof(float) * std::max<size_t>(uw.size(), 1)));
+        PE_HIP(hipMemcpy(h->uw, uw.data(), sizeof(float) * uw.size(), hipMemcpyHostToDevice));
+      }
     }
     if (h->lds_bytes && packed) {  // only the LDS E side reads them
       PE_HIP(hipMalloc(&h->erec, sizeof(ERec) * std::max<size_t>(erecs.size(), 1)));
@@ -605,7 +652,8 @@ static int freeze_converged(pe_sl *h, double tol, bool *frozen) {
 
 static bool valid_flags(int32_t flags) {
   const int32_t known = PE_SL_EARLY_EXIT | PE_SL_ORDERED | PE_SL_GRAPH | PE_SL_GLOBAL_GATHER |
-                        PE_SL_WIDE_IDS | PE_SL_ONE_PASS | PE_SL_U_MAJOR | PE_SL_FREEZE;
+                        PE_SL_WIDE_IDS | PE_SL_ONE_PASS | PE_SL_U_MAJOR | PE_SL_FREEZE |
+                        PE_SL_STENCIL_P_SIDE;
   return !(flags & ~known) && !((flags & PE_SL_ONE_PASS) && (flags & PE_SL_U_MAJOR));
 }
 
@@ -787,7 +835,7 @@ extern "C" void pe_sl_destroy(pe_sl *h) {
   for (hipEvent_t e : h->events) hipEventDestroy(e);
   for (void *p : {(void *)h->d_maps, (void *)h->pids, (void *)h->eids, (void *)h->erec,
                   (void *)h->ilist, (void *)h->ioff, (void *)h->nlist, (void *)h->noff,
-                  (void *)h->pcount,
+                  (void *)h->pcount, (void *)h->uoff, (void *)h->urows, (void *)h->uw,
                   (void *)h->ppos, (void *)h->epos, (void *)h->pw, (void *)h->ew,
                   (void *)h->pallowed, (void *)h->eallowed, (void *)h->pmask,
                   (void *)h->emask, (void *)h->inside,

@@ -419,6 +419,65 @@ __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T s
   return best;
 }
 
+// P side in FP32 from the union of a P state's stencil rows: the V rows that
+// any allowed command of the P state reads with non-zero weight (22 per P
+// state at C2 against 75 padded stencil terms over its 9 commands).  Each
+// thread loads a row's value at its E state once and adds it, with that
+// row's weight for every command, to all the commands' sums: G_u = sum_r
+// W[r][u] V[row_r], rows ascending (then zero-weight padding).  The order
+// of the terms differs from the stencils' vertex order and zero weights are
+// added, so the sums agree with sl_p_side to rounding (the FP64 mode keeps
+// sl_p_side, the oracle's order).  Blocks as in sl_p_side; rows of disallowed commands are not
+// written.  MAXA: room for the commands of a state.
+constexpr int kUnionCommands = 9;  // commands per state of sl_p_side_union
+constexpr int kUnionBatch = 8;     // its union rows per batch of loads
+
+template <int MAXA>
+__global__ __launch_bounds__(kPThreads) void sl_p_side_union(
+    const SlMap *__restrict__ maps, const float *__restrict__ V, float *__restrict__ G,
+    const int *__restrict__ uoff, const int *__restrict__ urows,
+    const float *__restrict__ uw, const uint16_t *__restrict__ pmask,
+    const int *__restrict__ ppos, const int *__restrict__ nlist,
+    const int *__restrict__ noff) {
+  const SlMap m = maps[blockIdx.z];
+  const int gx = (int)gridDim.x, q = gx >> 3, r = gx & 7;
+  const int xcd = (int)blockIdx.x & 7;
+  const int ip = xcd * q + min(xcd, r) + ((int)blockIdx.x >> 3);
+  if (ip >= m.np || !m.active) return;  // blockIdx.y: strided chunk of the list
+  const float *__restrict__ Vm = V + m.v_off;
+  const int row = m.p_row + ip;
+  const int k0 = noff[m.ipos_off + ppos[row]];
+  const int count = noff[m.ipos_off + ppos[row] + 1] - k0;
+  const int r0 = uoff[row], nr = uoff[row + 1] - r0;
+  const unsigned pm = pmask[row];
+  const long long ustride = (long long)m.np * m.ne;
+  float *__restrict__ Gp = G + m.g_off + (long long)ip * m.ne;
+  for (int k = blockIdx.y * kPThreads + threadIdx.x; k < count; k += gridDim.y * kPThreads) {
+    const int e = nlist[k0 + k];
+    float acc[MAXA];
+#pragma unroll
+    for (int u = 0; u < MAXA; u++) acc[u] = 0.f;
+    // kUnionBatch rows at a time (the lists are padded to a multiple of it
+    // with zero-weight copies of their last row): their loads in flight
+    // together.
+    for (int t0 = 0; t0 < nr; t0 += kUnionBatch) {
+      float v[kUnionBatch];
+#pragma unroll
+      for (int i = 0; i < kUnionBatch; i++)
+        v[i] = Vm[(long long)urows[r0 + t0 + i] * m.ne + e];
+#pragma unroll
+      for (int i = 0; i < kUnionBatch; i++) {
+        const float *__restrict__ w = uw + (long long)(r0 + t0 + i) * MAXA;
+#pragma unroll
+        for (int u = 0; u < MAXA; u++) acc[u] = fmaf(w[u], v[i], acc[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < MAXA; u++)
+      if ((pm >> u) & 1u) Gp[u * ustride + e] = acc[u];
+  }
+}
+
 // Block max of `change` into the map's residual slot (blockDim.x <= 1024).
 template <typename T>
 __device__ __forceinline__ void block_residual(T change, unsigned long long *slot) {
@@ -591,9 +650,19 @@ void enqueue_sweep_t(pe_sl *h, const void *src_, void *dst_, int flags,
   const int chunks = (int)std::max(
       1ll, std::min({(long long)(h->ne_max + kPThreads - 1) / kPThreads,
                      (kPSideBlocks + p_blocks - 1) / p_blocks, 65535ll}));
-  hipLaunchKernelGGL(sl_p_side<T>, dim3((unsigned)h->np_max, (unsigned)chunks, (unsigned)h->B),
-                     dim3(kPThreads), 0, h->stream, h->d_maps, src, G,
-                     h->pids, h->pw, h->pcount, h->pallowed, h->ppos, h->nlist, h->noff);
+  const dim3 p_grid((unsigned)h->np_max, (unsigned)chunks, (unsigned)h->B);
+  bool union_rows = false;
+  if constexpr (std::is_same<T, float>::value) {
+    if (h->uoff && !(flags & PE_SL_STENCIL_P_SIDE)) {
+      hipLaunchKernelGGL(sl_p_side_union<kUnionCommands>, p_grid, dim3(kPThreads), 0,
+                         h->stream, h->d_maps, src, G, h->uoff, h->urows, h->uw, h->pmask,
+                         h->ppos, h->nlist, h->noff);
+      union_rows = true;
+    }
+  }
+  if (!union_rows)
+    hipLaunchKernelGGL(sl_p_side<T>, p_grid, dim3(kPThreads), 0, h->stream, h->d_maps, src, G,
+                       h->pids, h->pw, h->pcount, h->pallowed, h->ppos, h->nlist, h->noff);
   if (record) sl_record(h, 0, false);
   if (record) sl_record(h, 1, true);
   const bool early = flags & PE_SL_EARLY_EXIT, ordered = flags & PE_SL_ORDERED;

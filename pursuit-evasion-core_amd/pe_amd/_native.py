@@ -190,6 +190,7 @@ PE_SL_WIDE_IDS = 16
 PE_SL_ONE_PASS = 32
 PE_SL_U_MAJOR = 64
 PE_SL_FREEZE = 128
+PE_SL_STENCIL_P_SIDE = 256
 
 class World8DConfig(ctypes.Structure):
     _fields_ = [

@@ -200,6 +200,31 @@ def test_small_map_visiting_orders_are_exact(gpu, small_game):
         np.testing.assert_array_equal(out[name][1], out["plain"][1], err_msg=name)
 
 
+@pytest.mark.parametrize("game", ["small_game", "c2_game"])
+def test_union_row_p_side_agrees_with_stencil_p_side(gpu, oracle_lib, game, request):
+    """The FP32 P side that adds each of a P state's union rows to every
+    command's sum (the default) against the per-command stencil sums in
+    vertex order (PE_SL_STENCIL_P_SIDE): the same values to rounding over
+    20 sweeps, and both within the FP32 bar of the FP64 oracle sweep."""
+    config, arena, p, e, tables = request.getfixturevalue(game)
+    inside = dense_inside(p, e, 3.0)
+    out = {}
+    for name, flags in [("union", EARLY | ORDERED | GRAPH),
+                        ("stencil", EARLY | ORDERED | GRAPH | _native.PE_SL_STENCIL_P_SIDE)]:
+        with GameSolver(tables) as s:
+            v0 = s.read()
+            s.sweep(1, flags & ~GRAPH)
+            out[name + "1"] = s.read()
+            s.sweep(19, flags)
+            out[name] = s.read()
+    d = np.abs(out["union"] - out["stencil"])
+    print(f"{game}: union vs stencil P side after 20 sweeps: max|dV| {d.max():.3e}")
+    assert d.max() <= 5e-6  # a few FP32 ulps of values up to 3 (measured 1.7e-6)
+    ref1, _ = oracle_lib.sl_sweep(v0.astype(np.float64), p, e, inside, threads=16)
+    for name in ("union1", "stencil1"):
+        assert np.abs(out[name] - ref1).max() <= 1e-5, name
+
+
 def test_lds_block_sizes_are_exact(gpu, small_game):
     """The LDS E side gives the same values at 256, 512 and 1024 threads per
     block, in both visiting orders, with graph replay; other sizes are
