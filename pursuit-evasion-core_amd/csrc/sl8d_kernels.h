@@ -452,29 +452,42 @@ __global__ __launch_bounds__(kPThreads) void sl_p_side_union(
   const unsigned pm = pmask[row];
   const long long ustride = (long long)m.np * m.ne;
   float *__restrict__ Gp = G + m.g_off + (long long)ip * m.ne;
-  for (int k = blockIdx.y * kPThreads + threadIdx.x; k < count; k += gridDim.y * kPThreads) {
-    const int e = nlist[k0 + k];
-    float acc[MAXA];
+  // Each thread takes two E states of the list (k and k + stride) with the
+  // same rows and weights: twice the loads in flight per weight fetch.
+  const int stride = gridDim.y * kPThreads;
+  for (int k = blockIdx.y * kPThreads + threadIdx.x; k < count; k += 2 * stride) {
+    const bool two = k + stride < count;
+    const int e0 = nlist[k0 + k], e1 = two ? nlist[k0 + k + stride] : e0;
+    float acc0[MAXA], acc1[MAXA];
 #pragma unroll
-    for (int u = 0; u < MAXA; u++) acc[u] = 0.f;
+    for (int u = 0; u < MAXA; u++) acc0[u] = acc1[u] = 0.f;
     // kUnionBatch rows at a time (the lists are padded to a multiple of it
     // with zero-weight copies of their last row): their loads in flight
     // together.
     for (int t0 = 0; t0 < nr; t0 += kUnionBatch) {
-      float v[kUnionBatch];
+      float v0[kUnionBatch], v1[kUnionBatch];
 #pragma unroll
-      for (int i = 0; i < kUnionBatch; i++)
-        v[i] = Vm[(long long)urows[r0 + t0 + i] * m.ne + e];
+      for (int i = 0; i < kUnionBatch; i++) {
+        const float *__restrict__ vrow = Vm + (long long)urows[r0 + t0 + i] * m.ne;
+        v0[i] = vrow[e0];
+        v1[i] = vrow[e1];
+      }
 #pragma unroll
       for (int i = 0; i < kUnionBatch; i++) {
         const float *__restrict__ w = uw + (long long)(r0 + t0 + i) * MAXA;
 #pragma unroll
-        for (int u = 0; u < MAXA; u++) acc[u] = fmaf(w[u], v[i], acc[u]);
+        for (int u = 0; u < MAXA; u++) {
+          acc0[u] = fmaf(w[u], v0[i], acc0[u]);
+          acc1[u] = fmaf(w[u], v1[i], acc1[u]);
+        }
       }
     }
 #pragma unroll
     for (int u = 0; u < MAXA; u++)
-      if ((pm >> u) & 1u) Gp[u * ustride + e] = acc[u];
+      if ((pm >> u) & 1u) {
+        Gp[u * ustride + e0] = acc0[u];
+        if (two) Gp[u * ustride + e1] = acc1[u];
+      }
   }
 }
 
@@ -654,7 +667,9 @@ void enqueue_sweep_t(pe_sl *h, const void *src_, void *dst_, int flags,
   bool union_rows = false;
   if constexpr (std::is_same<T, float>::value) {
     if (h->uoff && !(flags & PE_SL_STENCIL_P_SIDE)) {
-      hipLaunchKernelGGL(sl_p_side_union<kUnionCommands>, p_grid, dim3(kPThreads), 0,
+      // half the chunks: each thread takes two E states
+      const dim3 u_grid(p_grid.x, (unsigned)std::max(1, (chunks + 1) / 2), p_grid.z);
+      hipLaunchKernelGGL(sl_p_side_union<kUnionCommands>, u_grid, dim3(kPThreads), 0,
                          h->stream, h->d_maps, src, G, h->uoff, h->urows, h->uw, h->pmask,
                          h->ppos, h->nlist, h->noff);
       union_rows = true;
