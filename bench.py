@@ -266,10 +266,27 @@ def load_counters(name):
         return None
 
 
-def sl_roofline(prof_stats, st0, sweep_seconds, cars, counters):
+def sl_windows(s, sweeps, variants):
+    """Per-launch P-side and E-side kernel times (ms) of `sweeps` profiled
+    sweeps under each variant's flags, in order: {name: (p_ms, e_ms)}; a
+    variant with no P-side launch (the fused sweep) reports p_ms 0."""
+    s.set_profiling(True)
+    out = {}
+    for name, fl in variants.items():
+        b = s.stats()
+        s.sweep(sweeps, fl)
+        a = s.stats()
+        out[name] = ((a["p_ms"] - b["p_ms"]) / max(1, a["p_launches"] - b["p_launches"]),
+                     (a["e_ms"] - b["e_ms"]) / max(1, a["e_launches"] - b["e_launches"]))
+    s.set_profiling(False)
+    return out
+
+
+def sl_roofline(win, st0, sweep_seconds, cars, counters):
     """Roofline object of an 8D sweep line: HBM against the algorithmic
-    bytes of the whole sweep (live timing), the E-side kernel's time, and
-    its VALU / LDS utilisation from the committed counter summary."""
+    bytes of the whole sweep (live timing), the sweep kernel's time, and
+    the E side's VALU / LDS utilisation from the committed counter
+    summary."""
     nbytes = sl_algorithmic_bytes(st0["pairs"], cars)
     achieved = nbytes / sweep_seconds / 1e9
     out = {"bound": "valu",
@@ -278,8 +295,9 @@ def sl_roofline(prof_stats, st0, sweep_seconds, cars, counters):
            "hbm": {"achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                    "frac": round(achieved / HBM_PEAK_GBS, 4),
                    "algorithmic_bytes_per_sweep": nbytes},
-           "e_side_ms": prof_stats["e_ms"] / max(1, prof_stats["e_launches"]),
-           "p_side_ms": prof_stats["p_ms"] / max(1, prof_stats["p_launches"])}
+           "fused_sweep_kernel_ms": win["fused"][1],
+           "separate_p_side_ms": win["separate"][0],
+           "separate_e_side_ms": win["separate"][1]}
     if counters:
         out["counters"] = counters
     return out
@@ -327,18 +345,14 @@ def run_c2(args):
         t1 = time.perf_counter()
         s.sweep(args.steps, flags)
         wall = time.perf_counter() - t1
-        s.set_profiling(True)
-        s.sweep(min(args.steps, 20), flags & ~_native.PE_SL_GRAPH)
-        prof = s.stats()
-        s.sweep(min(args.steps, 20), (flags & ~_native.PE_SL_GRAPH) | _native.PE_SL_STENCIL_P_SIDE)
-        prof_s = s.stats()
-        s.sweep(min(args.steps, 20), (flags & ~_native.PE_SL_GRAPH) | _native.PE_SL_GLOBAL_GATHER)
-        prof_g = s.stats()
-        s.sweep(min(args.steps, 20), (flags & ~_native.PE_SL_GRAPH) | _native.PE_SL_WIDE_IDS)
-        prof_w = s.stats()
-        s.sweep(min(args.steps, 20), _native.PE_SL_EARLY_EXIT)
-        prof_r = s.stats()
-        s.set_profiling(False)
+        base = flags & ~_native.PE_SL_GRAPH
+        win = sl_windows(s, min(args.steps, 20), {
+            "fused": base,
+            "separate": base | _native.PE_SL_SEPARATE_P_SIDE,
+            "stencil": base | _native.PE_SL_STENCIL_P_SIDE,
+            "global": base | _native.PE_SL_GLOBAL_GATHER,
+            "wide": base | _native.PE_SL_WIDE_IDS,
+            "reference": _native.PE_SL_EARLY_EXIT})
         out = {
             "workload": "C2 8D contact game, 11x11 positions (+-3 m) x 12 headings x 5 speeds "
                         "per car, 30 random rectangles (seed %d)" % args.seed,
@@ -347,24 +361,23 @@ def run_c2(args):
             "table_build_s": build_s, "sweeps": args.steps, "ms_per_sweep": wall / args.steps * 1e3,
             "legal_pair_updates_per_s": st0["pairs"] * args.steps / wall,
             "active_pair_updates_per_s": st0["active_pairs"] * args.steps / wall,
-            "p_side_ms": prof["p_ms"] / max(1, prof["p_launches"]),
             "e_side_lds_threads": st0["lds_e_side"],
-            "e_side_ms": prof["e_ms"] / max(1, prof["e_launches"]),
-            # P side summing each command's stencil in vertex order instead
-            # of adding each union row to all commands' sums.
-            "p_side_stencil_ms": (prof_s["p_ms"] - prof["p_ms"]) /
-                                 max(1, prof_s["p_launches"] - prof["p_launches"]),
-            "e_side_global_gather_ms": (prof_g["e_ms"] - prof_s["e_ms"]) /
-                                       max(1, prof_g["e_launches"] - prof_s["e_launches"]),
-            "e_side_unpacked_tables_ms": (prof_w["e_ms"] - prof_g["e_ms"]) /
-                                   max(1, prof_w["e_launches"] - prof_g["e_launches"]),
+            # The default sweep is one kernel: each E-side block computes its
+            # P state's values into LDS from V (no G buffer).
+            "fused_sweep_kernel_ms": win["fused"][1],
+            # The same sweep with the P side as its own launch writing G.
+            "p_side_ms": win["separate"][0], "e_side_ms": win["separate"][1],
+            # That P side summing each command's stencil in vertex order
+            # instead of adding each union row to all commands' sums.
+            "p_side_stencil_ms": win["stencil"][0],
+            "e_side_global_gather_ms": win["global"][1],
+            "e_side_unpacked_tables_ms": win["wide"][1],
             # The reference's early exit in index order, without the stored
-            # command order.
-            "e_side_reference_order_ms": (prof_r["e_ms"] - prof_w["e_ms"]) /
-                                         max(1, prof_r["e_launches"] - prof_w["e_launches"]),
+            # command order (separate P side).
+            "e_side_reference_order_ms": win["reference"][1],
             "residual_after": s.stats()["last_residual"],
         }
-        out["roofline"] = sl_roofline(prof, st0, wall / args.steps, (p, e),
+        out["roofline"] = sl_roofline(win, st0, wall / args.steps, (p, e),
                                       load_counters("sl8d_e_side_c2_counters.json"))
         if args.to_residual:
             s.initialize()
@@ -438,14 +451,12 @@ def run_c4(args):
         s.sweep(args.steps, flags)
         wall = time.perf_counter() - t1
         res = s.trace()[-1]
-        s.set_profiling(True)
-        s.sweep(20, flags & ~_native.PE_SL_GRAPH)
-        prof = s.stats()
-        s.sweep(20, (flags & ~_native.PE_SL_GRAPH) | _native.PE_SL_STENCIL_P_SIDE)
-        prof_s = s.stats()
-        s.sweep(20, _native.PE_SL_EARLY_EXIT)
-        prof_r = s.stats()
-        s.set_profiling(False)
+        base = flags & ~_native.PE_SL_GRAPH
+        win = sl_windows(s, 20, {
+            "fused": base,
+            "separate": base | _native.PE_SL_SEPARATE_P_SIDE,
+            "stencil": base | _native.PE_SL_STENCIL_P_SIDE,
+            "reference": _native.PE_SL_EARLY_EXIT})
         out = {
             "workload": f"C4 {args.maps} seeded 30-rectangle maps, 9x9 positions x 8 headings x "
                         "4 speeds per car, batched, hipGraph-replayed sweeps",
@@ -455,20 +466,20 @@ def run_c4(args):
             "legal_pair_updates_per_s": st0["pairs"] * args.steps / wall,
             "active_pair_updates_per_s": st0["active_pairs"] * args.steps / wall,
             "max_residual_after": float(res.max()),
-            "p_side_ms": prof["p_ms"] / max(1, prof["p_launches"]),
-            "e_side_ms": prof["e_ms"] / max(1, prof["e_launches"]),
             "e_side_lds_threads": st0["lds_e_side"],
-            # P side summing each command's stencil in vertex order instead
-            # of adding each union row to all commands' sums.
-            "p_side_stencil_ms": (prof_s["p_ms"] - prof["p_ms"]) /
-                                 max(1, prof_s["p_launches"] - prof["p_launches"]),
+            # The default sweep is one kernel: each E-side block computes its
+            # P state's values into LDS from V (no G buffer).
+            "fused_sweep_kernel_ms": win["fused"][1],
+            # The same sweep with the P side as its own launch writing G.
+            "p_side_ms": win["separate"][0], "e_side_ms": win["separate"][1],
+            # That P side summing each command's stencil in vertex order.
+            "p_side_stencil_ms": win["stencil"][0],
             # The reference's early exit in index order, without the stored
-            # command order.
-            "e_side_reference_order_ms": (prof_r["e_ms"] - prof_s["e_ms"]) /
-                                         max(1, prof_r["e_launches"] - prof_s["e_launches"]),
+            # command order (separate P side).
+            "e_side_reference_order_ms": win["reference"][1],
         }
         cars = [g[2] for g in games] + [g[3] for g in games]
-        out["roofline"] = sl_roofline(prof, st0, wall / args.steps, cars,
+        out["roofline"] = sl_roofline(win, st0, wall / args.steps, cars,
                                       load_counters("sl8d_e_side_c4_counters.json"))
         if args.to_residual:
             import numpy as np

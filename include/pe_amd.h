@@ -288,6 +288,11 @@ typedef struct pe_sl_game {
                                  each of the P state's union rows to all
                                  commands' sums (comparison runs; equal to
                                  rounding)                                 */
+#define PE_SL_SEPARATE_P_SIDE 512 /* FP32: the P side as its own launch writing
+                                 the G buffer, which the E side stages,
+                                 instead of each E-side block computing its
+                                 P state's values into LDS (comparison
+                                 runs; the same bits)                      */
 
 typedef struct pe_sl_stats {
   int64_t sweeps;

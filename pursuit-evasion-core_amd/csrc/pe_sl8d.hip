@@ -653,7 +653,7 @@ static int freeze_converged(pe_sl *h, double tol, bool *frozen) {
 static bool valid_flags(int32_t flags) {
   const int32_t known = PE_SL_EARLY_EXIT | PE_SL_ORDERED | PE_SL_GRAPH | PE_SL_GLOBAL_GATHER |
                         PE_SL_WIDE_IDS | PE_SL_ONE_PASS | PE_SL_U_MAJOR | PE_SL_FREEZE |
-                        PE_SL_STENCIL_P_SIDE;
+                        PE_SL_STENCIL_P_SIDE | PE_SL_SEPARATE_P_SIDE;
   return !(flags & ~known) && !((flags & PE_SL_ONE_PASS) && (flags & PE_SL_U_MAJOR));
 }
 

@@ -432,6 +432,37 @@ __device__ __forceinline__ T pair_backup(const SlMap &m, int prow, int erow, T s
 constexpr int kUnionCommands = 9;  // commands per state of sl_p_side_union
 constexpr int kUnionBatch = 8;     // its union rows per batch of loads
 
+// The P side's sums of one P state at E states e0 and e1 from its union rows
+// r0 .. r0 + nr (padded to kUnionBatch): acc[u] = sum_r W[r][u] V[row_r],
+// each row's value loaded once for both E states and all commands.
+template <int MAXA>
+__device__ __forceinline__ void union_sums(const float *__restrict__ Vm, int ne,
+                                           const int *__restrict__ urows,
+                                           const float *__restrict__ uw, int r0, int nr,
+                                           int e0, int e1, float (&acc0)[MAXA],
+                                           float (&acc1)[MAXA]) {
+#pragma unroll
+  for (int u = 0; u < MAXA; u++) acc0[u] = acc1[u] = 0.f;
+  for (int t0 = 0; t0 < nr; t0 += kUnionBatch) {
+    float v0[kUnionBatch], v1[kUnionBatch];
+#pragma unroll
+    for (int i = 0; i < kUnionBatch; i++) {
+      const float *__restrict__ vrow = Vm + (long long)urows[r0 + t0 + i] * ne;
+      v0[i] = vrow[e0];
+      v1[i] = vrow[e1];
+    }
+#pragma unroll
+    for (int i = 0; i < kUnionBatch; i++) {
+      const float *__restrict__ w = uw + (long long)(r0 + t0 + i) * MAXA;
+#pragma unroll
+      for (int u = 0; u < MAXA; u++) {
+        acc0[u] = fmaf(w[u], v0[i], acc0[u]);
+        acc1[u] = fmaf(w[u], v1[i], acc1[u]);
+      }
+    }
+  }
+}
+
 template <int MAXA>
 __global__ __launch_bounds__(kPThreads) void sl_p_side_union(
     const SlMap *__restrict__ maps, const float *__restrict__ V, float *__restrict__ G,
@@ -459,29 +490,7 @@ __global__ __launch_bounds__(kPThreads) void sl_p_side_union(
     const bool two = k + stride < count;
     const int e0 = nlist[k0 + k], e1 = two ? nlist[k0 + k + stride] : e0;
     float acc0[MAXA], acc1[MAXA];
-#pragma unroll
-    for (int u = 0; u < MAXA; u++) acc0[u] = acc1[u] = 0.f;
-    // kUnionBatch rows at a time (the lists are padded to a multiple of it
-    // with zero-weight copies of their last row): their loads in flight
-    // together.
-    for (int t0 = 0; t0 < nr; t0 += kUnionBatch) {
-      float v0[kUnionBatch], v1[kUnionBatch];
-#pragma unroll
-      for (int i = 0; i < kUnionBatch; i++) {
-        const float *__restrict__ vrow = Vm + (long long)urows[r0 + t0 + i] * m.ne;
-        v0[i] = vrow[e0];
-        v1[i] = vrow[e1];
-      }
-#pragma unroll
-      for (int i = 0; i < kUnionBatch; i++) {
-        const float *__restrict__ w = uw + (long long)(r0 + t0 + i) * MAXA;
-#pragma unroll
-        for (int u = 0; u < MAXA; u++) {
-          acc0[u] = fmaf(w[u], v0[i], acc0[u]);
-          acc1[u] = fmaf(w[u], v1[i], acc1[u]);
-        }
-      }
-    }
+    union_sums<MAXA>(Vm, m.ne, urows, uw, r0, nr, e0, e1, acc0, acc1);
 #pragma unroll
     for (int u = 0; u < MAXA; u++)
       if ((pm >> u) & 1u) {
@@ -490,6 +499,16 @@ __global__ __launch_bounds__(kPThreads) void sl_p_side_union(
       }
   }
 }
+
+// Where the fused E side finds its P side: the union rows and the near
+// lists (FP32 handles with packed records).
+struct FusedP {
+  const int *__restrict__ uoff;
+  const int *__restrict__ urows;
+  const float *__restrict__ uw;
+  const int *__restrict__ nlist;
+  const int *__restrict__ noff;
+};
 
 // Block max of `change` into the map's residual slot (blockDim.x <= 1024).
 template <typename T>
@@ -554,7 +573,14 @@ __global__ __launch_bounds__(256) void sl_e_side(
 // handle from the occupancy and the pairs per P state).
 constexpr int kLdsMaxThreads = 1024;
 
-template <typename T, bool EARLY, int ORDER, typename ID>
+//
+// FUSED (FP32, packed records): no G buffer and no P-side launch; the block
+// computes its P state's values straight into the tile from V with the
+// union rows (the sums of sl_p_side_union, bit for bit), for the near E
+// states only: the non-zero-weight vertices of every inside pair's stencils
+// are near, and the records point every zero-weight vertex at the pair's
+// own (inside, so near) state.
+template <typename T, bool EARLY, int ORDER, typename ID, bool FUSED = false>
 __global__ __launch_bounds__(kLdsMaxThreads) void sl_e_side_lds(
     const SlMap *__restrict__ maps, const T *__restrict__ V,
     T *__restrict__ Vn, const T *__restrict__ G,
@@ -562,7 +588,9 @@ __global__ __launch_bounds__(kLdsMaxThreads) void sl_e_side_lds(
     const uint16_t *__restrict__ emask, const uint16_t *__restrict__ pmask,
     const int *__restrict__ ppos, const int *__restrict__ ilist,
     const int *__restrict__ ioff, uint8_t *__restrict__ order,
-    unsigned long long *__restrict__ slot, int probe) {
+    unsigned long long *__restrict__ slot, int probe, FusedP fp) {
+  static_assert(!FUSED || (std::is_same<T, float>::value && std::is_same<ID, ERec>::value),
+                "the fused P side is FP32 with packed records");
   extern __shared__ __align__(16) unsigned char tile_bytes[];
   T *tile = reinterpret_cast<T *>(tile_bytes);
   const SlMap m = maps[blockIdx.z];
@@ -570,7 +598,27 @@ __global__ __launch_bounds__(kLdsMaxThreads) void sl_e_side_lds(
   const int ip = blockIdx.x;  // uniform per block
   const bool active = ip < m.np;
   const int prow = m.p_row + ip;
-  if (active) {
+  if constexpr (FUSED) {
+    if (active) {
+      const int n0 = fp.noff[m.ipos_off + ppos[prow]];
+      const int count = fp.noff[m.ipos_off + ppos[prow] + 1] - n0;
+      const int r0 = fp.uoff[prow], nr = fp.uoff[prow + 1] - r0;
+      const int nt = (int)blockDim.x;
+      const float *__restrict__ Vm = V + m.v_off;
+      for (int k = threadIdx.x; k < count; k += 2 * nt) {
+        const bool two = k + nt < count;
+        const int e0 = fp.nlist[n0 + k], e1 = two ? fp.nlist[n0 + k + nt] : e0;
+        float acc0[kUnionCommands], acc1[kUnionCommands];
+        union_sums<kUnionCommands>(Vm, m.ne, fp.urows, fp.uw, r0, nr, e0, e1, acc0, acc1);
+#pragma unroll
+        for (int u = 0; u < kUnionCommands; u++)
+          if (u < m.na) {
+            tile[e0 * m.na + u] = acc0[u];
+            if (two) tile[e1 * m.na + u] = acc1[u];
+          }
+      }
+    }
+  } else if (active) {
     // All na rows (rows of disallowed commands are read but never used),
     // read row by row (t = u * ne + e, coalesced) with eight loads in
     // flight per thread and written to tile[e * na + u].
@@ -649,15 +697,11 @@ inline void sl_record(pe_sl *h, int kind, bool start) {
   if (start) h->kinds.push_back(kind);
 }
 
-// Enqueues one sweep src -> dst (T values) writing its residuals to
-// slot[0..B).
+// Enqueues the separate P side of one sweep (G from src): the union-row
+// kernel for FP32 handles that have union rows, the stencil kernel
+// otherwise or with PE_SL_STENCIL_P_SIDE.
 template <typename T>
-void enqueue_sweep_t(pe_sl *h, const void *src_, void *dst_, int flags,
-                     unsigned long long *slot, bool record) {
-  const T *src = static_cast<const T *>(src_);
-  T *dst = static_cast<T *>(dst_);
-  T *G = reinterpret_cast<T *>(h->G);
-  const dim3 grid = pair_grid(h), block(256);
+void enqueue_p_side(pe_sl *h, const T *src, T *G, int flags, bool record) {
   if (record) sl_record(h, 0, true);
   const long long p_blocks = (long long)h->np_max * h->B;
   const int chunks = (int)std::max(
@@ -679,24 +723,44 @@ void enqueue_sweep_t(pe_sl *h, const void *src_, void *dst_, int flags,
     hipLaunchKernelGGL(sl_p_side<T>, p_grid, dim3(kPThreads), 0, h->stream, h->d_maps, src, G,
                        h->pids, h->pw, h->pcount, h->pallowed, h->ppos, h->nlist, h->noff);
   if (record) sl_record(h, 0, false);
+}
+
+// Enqueues one sweep src -> dst (T values) writing its residuals to
+// slot[0..B).
+template <typename T>
+void enqueue_sweep_t(pe_sl *h, const void *src_, void *dst_, int flags,
+                     unsigned long long *slot, bool record) {
+  const T *src = static_cast<const T *>(src_);
+  T *dst = static_cast<T *>(dst_);
+  T *G = reinterpret_cast<T *>(h->G);
+  const dim3 grid = pair_grid(h), block(256);
+  const bool lds = h->lds_bytes > 0 && !(flags & PE_SL_GLOBAL_GATHER);
+  const bool packed = h->erec && !(flags & PE_SL_WIDE_IDS);
+  // The E side computes its own P side from the union rows (no G buffer)
+  // unless a separate P side is asked for.
+  const bool fused = std::is_same<T, float>::value && lds && packed && h->uoff &&
+                     !(flags & (PE_SL_STENCIL_P_SIDE | PE_SL_SEPARATE_P_SIDE));
+  const FusedP fp{h->uoff, h->urows, h->uw, h->nlist, h->noff};
+  if (!fused) enqueue_p_side<T>(h, src, G, flags, record);
   if (record) sl_record(h, 1, true);
   const bool early = flags & PE_SL_EARLY_EXIT, ordered = flags & PE_SL_ORDERED;
   const int probe = h->probe_orders ? 1 : 0;
-  const bool lds = h->lds_bytes > 0 && !(flags & PE_SL_GLOBAL_GATHER);
   const dim3 lds_grid((unsigned)h->np_max, 1, (unsigned)h->B), lds_block(h->lds_threads);
-  const bool packed = h->erec && !(flags & PE_SL_WIDE_IDS);
   // One pass over the E commands unless u-major is asked for.
   const bool one_pass = !(flags & PE_SL_U_MAJOR);
-#define PE_E_SIDE_LDS(E, O, ID, IDS)                                                   \
-  hipLaunchKernelGGL((sl_e_side_lds<T, E, O, ID>), lds_grid, lds_block, h->lds_bytes,    \
+#define PE_E_SIDE_LDS(E, O, ID, IDS, F)                                                \
+  hipLaunchKernelGGL((sl_e_side_lds<T, E, O, ID, F>), lds_grid, lds_block, h->lds_bytes, \
                      h->stream, h->d_maps, src, dst, G, IDS, h->ew, h->emask, h->pmask,  \
-                     h->ppos, h->ilist, h->ioff, h->order, slot, probe)
+                     h->ppos, h->ilist, h->ioff, h->order, slot, probe, fp)
 #define PE_E_SIDE(E, O)                                                                \
   do {                                                                                 \
-    if (lds && packed)                                                                 \
-      PE_E_SIDE_LDS(E, O, ERec, (const ERec *)h->erec);                                \
+    if (fused) {                                                                       \
+      if constexpr (std::is_same<T, float>::value)                                     \
+        PE_E_SIDE_LDS(E, O, ERec, (const ERec *)h->erec, true);                        \
+    } else if (lds && packed)                                                          \
+      PE_E_SIDE_LDS(E, O, ERec, (const ERec *)h->erec, false);                         \
     else if (lds)                                                                      \
-      PE_E_SIDE_LDS(E, O, int, h->eids);                                               \
+      PE_E_SIDE_LDS(E, O, int, h->eids, false);                                        \
     else                                                                               \
       hipLaunchKernelGGL((sl_e_side<T, E, O>), grid, block, 0, h->stream, h->d_maps,   \
                          src, dst, G, h->eids, h->ew, h->emask, h->pmask, h->ppos,     \
@@ -739,6 +803,18 @@ bool sl_allow_lds(size_t bytes) {
                         (const void *)sl_e_side_lds<T, false, kPlainOrder, ERec>})
     ok = ok && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) ==
                    hipSuccess;
+  if constexpr (std::is_same<T, float>::value) {
+    for (const void *f : {(const void *)sl_e_side_lds<T, true, kOnePass9, ERec, true>,
+                          (const void *)sl_e_side_lds<T, false, kOnePass9, ERec, true>,
+                          (const void *)sl_e_side_lds<T, true, kOnePass, ERec, true>,
+                          (const void *)sl_e_side_lds<T, false, kOnePass, ERec, true>,
+                          (const void *)sl_e_side_lds<T, true, kUMajor, ERec, true>,
+                          (const void *)sl_e_side_lds<T, false, kUMajor, ERec, true>,
+                          (const void *)sl_e_side_lds<T, true, kPlainOrder, ERec, true>,
+                          (const void *)sl_e_side_lds<T, false, kPlainOrder, ERec, true>})
+      ok = ok && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)bytes) == hipSuccess;
+  }
   if (!ok) (void)hipGetLastError();
   return ok;
 }

@@ -191,6 +191,7 @@ PE_SL_ONE_PASS = 32
 PE_SL_U_MAJOR = 64
 PE_SL_FREEZE = 128
 PE_SL_STENCIL_P_SIDE = 256
+PE_SL_SEPARATE_P_SIDE = 512
 
 class World8DConfig(ctypes.Structure):
     _fields_ = [

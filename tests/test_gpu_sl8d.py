@@ -131,7 +131,8 @@ def run_variants(tables, sweeps):
                         ("u_major", EARLY | ORDERED | U_MAJOR),
                         ("one_pass_full", ORDERED | ONE_PASS),
                         ("u_major_global", EARLY | ORDERED | U_MAJOR | GLOBAL),
-                        ("one_pass_global_wide", EARLY | ORDERED | ONE_PASS | GLOBAL | WIDE)]:
+                        ("one_pass_global_wide", EARLY | ORDERED | ONE_PASS | GLOBAL | WIDE),
+                        ("separate_p", EARLY | ORDERED | _native.PE_SL_SEPARATE_P_SIDE)]:
         with GameSolver(tables) as s:
             s.sweep(sweeps, flags)
             out[name] = (s.read(), s.trace()[:, 0])
@@ -150,7 +151,7 @@ def test_early_exit_ordering_and_graph_are_exact(gpu, na):
     base, trace = res["plain"]
     for name in ("early", "ordered", "graph", "global", "global_plain", "wide_ids",
                  "wide_ids_global", "one_pass", "u_major", "one_pass_full", "u_major_global",
-                 "one_pass_global_wide"):
+                 "one_pass_global_wide", "separate_p"):
         np.testing.assert_array_equal(res[name][0], base, err_msg=name)
         np.testing.assert_array_equal(res[name][1], trace, err_msg=name)
 
@@ -223,6 +224,28 @@ def test_union_row_p_side_agrees_with_stencil_p_side(gpu, oracle_lib, game, requ
     ref1, _ = oracle_lib.sl_sweep(v0.astype(np.float64), p, e, inside, threads=16)
     for name in ("union1", "stencil1"):
         assert np.abs(out[name] - ref1).max() <= 1e-5, name
+
+
+@pytest.mark.parametrize("game", ["small_game", "c2_game"])
+def test_fused_p_side_equals_separate_p_side(gpu, game, request):
+    """The default FP32 sweep computes each P state's values inside its
+    E-side block (no G buffer); with PE_SL_SEPARATE_P_SIDE the union-row P
+    side writes G and the E side stages it.  Same sums, so the same bits:
+    fields, residual traces and stored command orders over 30 sweeps."""
+    tables = request.getfixturevalue(game)[4]
+    out = {}
+    for name, flags in [("fused", EARLY | ORDERED | GRAPH),
+                        ("separate", EARLY | ORDERED | GRAPH | _native.PE_SL_SEPARATE_P_SIDE),
+                        ("fused_u_major", EARLY | ORDERED | U_MAJOR),
+                        ("separate_u_major", EARLY | ORDERED | U_MAJOR |
+                         _native.PE_SL_SEPARATE_P_SIDE)]:
+        with GameSolver(tables) as s:
+            s.sweep(30, flags)
+            out[name] = (s.read(), s.trace()[:, 0])
+    for a, b in (("fused", "separate"), ("fused_u_major", "separate_u_major"),
+                 ("fused", "fused_u_major")):
+        np.testing.assert_array_equal(out[a][0], out[b][0], err_msg=f"{a} {b}")
+        np.testing.assert_array_equal(out[a][1], out[b][1], err_msg=f"{a} {b}")
 
 
 def test_lds_block_sizes_are_exact(gpu, small_game):
