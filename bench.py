@@ -540,7 +540,7 @@ def run_learned(args):
     grid = dict(positions=9, headings=8, speeds=4)
     out = compare_batched(range(0, args.learn_train), range(100, 104),
                           range(1000, 1000 + args.learn_test), grid=grid,
-                          steps=args.learn_steps, log=log)
+                          steps=args.learn_steps, under_weight=args.learn_under_weight, log=log)
     out["workload"] = (f"learned warm start vs plain DP: network trained on maps "
                        f"0-{args.learn_train - 1}, delta from maps 100-103, held-out maps "
                        f"1000-{999 + args.learn_test} (the first maps of the C4 batch); C4 "
@@ -574,6 +574,8 @@ def main() -> int:
     ap.add_argument("--learn-train", type=int, default=16)
     ap.add_argument("--learn-test", type=int, default=8)
     ap.add_argument("--learn-steps", type=int, default=3000)
+    ap.add_argument("--learn-under-weight", type=float, default=4.0,
+                    help="learned line: loss weight of the network's underestimates")
     ap.add_argument("--vstar-sweeps", type=int, default=16384,
                     help="C2: cap of the run to V* for the time to 1e-3 of V* (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0,

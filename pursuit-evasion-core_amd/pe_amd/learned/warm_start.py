@@ -283,10 +283,32 @@ def _targets(first, over=None):
                 batch_gpu_seconds=max(f[1] for f in chosen) if done else None)
 
 
+def accuracy_bar(tables, vstar, masks, lifts=(1.0, 0.1, 0.01), *, tol_err=1e-3, tol_res=1e-4,
+                 cap=100_000, seed=0):
+    """What a warm start would have to achieve before it pays: on maps whose
+    V* is known, the plain sweeps from starts min(phi, V* + lift * u), u
+    uniform in [0, 1) per inside pair (so never below V*, and on average
+    lift / 2 above it), until every map is within tol_err of V* and at
+    residual tol_res.  Sweeps per map to both targets for each lift."""
+    rng = np.random.default_rng(seed)
+    with GameSolver(tables) as s:
+        phi = [s.read(k) for k in range(len(tables))]
+    rows = []
+    for lift in lifts:
+        v0 = [np.where(m, np.minimum(f, z + np.float32(lift) * rng.random(z.shape, np.float32)),
+                       f).astype(np.float32) for f, z, m in zip(phi, vstar, masks)]
+        with GameSolver(tables) as s:
+            s.initialize(v0)
+            e_first, r_first, _, _ = converge_batch(s, vstar, masks, tol_err=tol_err,
+                                                    tol_res=tol_res, cap=cap)
+        rows.append(dict(lift=lift, to_vstar=_targets(e_first), to_residual=_targets(r_first)))
+    return rows
+
+
 def compare_batched(train_seeds, validate_seeds, test_seeds, *, grid, label_tol=1e-6,
                     label_cap=20_000, tol_err=1e-3, tol_res=1e-4, vstar_tol=1e-7,
                     vstar_cap=100_000, steps=3000, batch=8192, samples=32768, seed=0,
-                    log=print):
+                    under_weight=4.0, log=print):
     """The learned warm start against the plain DP on batches of maps.
 
     Labels: the plain DP to residual `label_tol` (or `label_cap` sweeps) on
@@ -326,7 +348,8 @@ def compare_batched(train_seeds, validate_seeds, test_seeds, *, grid, label_tol=
     p0, e0, _ = games[0]
     t1 = time.perf_counter()
     net, info = train(np.concatenate(xs), np.concatenate(ys), p0, e0, steps=steps, batch=batch,
-                      seed=seed, under_weight=4.0)
+                      seed=seed, under_weight=under_weight)
+    info["under_weight"] = under_weight
     out.update(training=info, train_wall_seconds=time.perf_counter() - t1)
     worst, rmse = 0.0, []
     for (p, e, _), f, x in zip(games[n_train:], fields[n_train:], inputs[n_train:]):
@@ -426,6 +449,23 @@ def compare_batched(train_seeds, validate_seeds, test_seeds, *, grid, label_tol=
         f"sweeps, to residual {tol_res:g} {warm_t['to_residual']['batch_sweeps']} sweeps; "
         f"start below V* by up to {worst_below:.3f}; judge {out['judge_ok']}")
     net.close()
+    # The bar a network would have to clear: starts between V* and V* +
+    # lift on the maps that have a V*, against the cold runs above.
+    known = [k for k, ok in enumerate(has_vstar) if ok]
+    if known:
+        bar = accuracy_bar([tables[k] for k in known], [vstar[k] for k in known],
+                           [masks[k] for k in known], tol_err=tol_err, tol_res=tol_res,
+                           cap=vstar_cap, seed=seed)
+        out["start_accuracy_bar"] = dict(
+            maps=[int(list(test_seeds)[k]) for k in known],
+            cold_to_vstar=[cold_t["to_vstar"]["map_sweeps"][k] for k in known],
+            cold_to_residual=[cold_t["to_residual"]["map_sweeps"][k] for k in known],
+            starts=bar)
+        for row in bar:
+            log(f"[learned] start within {row['lift']:g} above V*: to {tol_err:g} of V* "
+                f"{row['to_vstar']['map_sweeps']} sweeps, to residual {tol_res:g} "
+                f"{row['to_residual']['map_sweeps']} (cold {out['start_accuracy_bar']['cold_to_vstar']}, "
+                f"{out['start_accuracy_bar']['cold_to_residual']})")
     return out
 
 
