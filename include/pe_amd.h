@@ -420,7 +420,7 @@ void pe_rtable_destroy(pe_rtable *h);
  * (local_game/training.py:351-352); optimiser AdamW (training.py:185-188).
  * ====================================================================== */
 typedef struct pe_mlp_config {
-  int32_t extra;        /* map features after the 8 state coordinates (<= 8) */
+  int32_t extra;        /* map features after the 8 state coordinates (<= 40) */
   int32_t rff;          /* random Fourier features (sin + cos), even        */
   int32_t hidden;       /* hidden width, multiple of 64                     */
   int32_t layers;       /* hidden SiLU layers                               */

@@ -43,6 +43,7 @@
 namespace pe {
 
 constexpr int kRelative = 5;  // relative-frame features appended by mlp_features
+constexpr int kMaxExtra = 40;  // map features per point after the 8 state coordinates
 
 __device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }
 __device__ __forceinline__ float silu(float z) { return z / (1.f + __expf(-z)); }
@@ -835,7 +836,7 @@ extern "C" int pe_mlp_create(const pe_mlp_config *cfg, pe_mlp **out) {
   if (!cfg || !out) return fail(PE_ERR_INVALID, "null argument");
   *out = nullptr;
   const pe_mlp_config &c = *cfg;
-  if (c.extra < 0 || c.extra > 8 || c.rff < 0 || c.rff % 2 || c.rff > 64 || c.hidden < 16 ||
+  if (c.extra < 0 || c.extra > kMaxExtra || c.rff < 0 || c.rff % 2 || c.rff > 64 || c.hidden < 16 ||
       c.hidden % 64 || c.layers < 1 || c.layers > 8 || !(c.learning_rate > 0) ||
       !(c.under_weight >= 1.f && c.under_weight <= 1e4f))
     return fail(PE_ERR_INVALID, "invalid MLP configuration");
