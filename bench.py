@@ -549,6 +549,77 @@ def run_learned(args):
     return out
 
 
+def run_secondary(args, cfg):
+    """The lines beside the headline: C1 to residual 1e-4, the 8D contact
+    game (C2, C4), the learned warm start, the dense 8D square game and the
+    5D reader.  A line that raises is recorded with its error, and no
+    further line runs, so the headline line is still printed."""
+    from pe_amd import _native
+
+    secondary = {}
+
+    def c1_to_residual():
+        # The plain DP's time to residual 1e-4 of the C1 solve from phi.
+        with _native.HJI5D(cfg) as h:
+            t0 = time.perf_counter()
+            ok = h.run_to_residual(1e-4, 30000, 16)
+            wall = time.perf_counter() - t0
+            st = h.stats()
+        return {"tolerance": 1e-4, "converged": ok, "steps": st["internal_steps"],
+                "solve_time_s": st["time"], "seconds": wall,
+                "last_residual": st["last_residual"]}
+
+    def c2_line():
+        c2 = argparse.Namespace(seed=0, warmup=10, steps=200, to_residual=1e-4,
+                                max_sweeps=20000, cpu_sweeps=0 if args.no_cpu_baseline else 48,
+                                vstar_sweeps=args.vstar_sweeps)
+        return run_c2(c2)
+
+    def c4_line():
+        c4 = argparse.Namespace(maps=128, warmup=10, steps=100, to_residual=1e-4,
+                                max_sweeps=20000, no_freeze=False,
+                                cpu_sweeps=0 if args.no_cpu_baseline else 100)
+        return run_c4(c4)
+
+    def learned_line():
+        learned = run_learned(args)
+        if "contact_game_c4" in secondary:
+            # the plain DP's time to 1e-3 of V* on the first maps of the C4
+            # batch, from the learned line's cold run
+            cold = learned["cold"]
+            secondary["contact_game_c4"]["to_1e-3_of_vstar"] = {
+                "maps": f"1000-{999 + args.learn_test} (batched, no freezing)",
+                "vstar": learned["vstar"],
+                "sweeps": cold["to_vstar"], "to_residual_1e-4": cold["to_residual"]}
+        return learned
+
+    lines = [("hji5d_c1_to_residual", "C1 from phi to per-step residual 1e-4", c1_to_residual),
+             ("contact_game_c2", "8D contact game (seed 0)", c2_line)]
+    if not args.no_c4:
+        lines.append(("contact_game_c4", "8D contact game, 128 maps (C4)", c4_line))
+    if not args.no_learned:
+        lines.append(("learned_warm_start_c4_grid", "learned warm start vs plain DP",
+                      learned_line))
+    lines += [("world8d_default", "dense 8D square game",
+               lambda: run_world8d((9, 8, 7, 7), 40, 2, 0 if args.no_cpu_baseline else 40)),
+              ("world8d_13x12x7x7", "dense 8D square game, 13x12x7x7",
+               lambda: run_world8d((13, 12, 7, 7), 20, 2)),
+              ("reader_c1", "5D table reader",
+               lambda: run_reader_c1(cfg, cpu_queries=0 if args.no_cpu_baseline
+                                     else 1_000_000))]
+    for key, label, fn in lines:
+        print(f"[bench] secondary: {label}", file=sys.stderr, flush=True)
+        try:
+            secondary[key] = fn()
+        except Exception as exc:  # recorded; the remaining lines are skipped
+            secondary[key] = {"error": f"{type(exc).__name__}: {exc}"}
+            secondary["skipped_after_error"] = [k for k, _, _ in lines[
+                [k for k, _, _ in lines].index(key) + 1:]]
+            print(f"[bench] secondary line {key} failed: {exc}", file=sys.stderr, flush=True)
+            break
+    return secondary
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -680,50 +751,7 @@ def main() -> int:
     if args.copy_probe:
         out["copy_probe_gbps"] = _native.copy_bandwidth(args.copy_probe, 1)
     if not args.no_secondary:
-        # The 8D contact game beside the headline: sweep throughput and the
-        # plain DP's time to residual 1e-4 on the seed-0 map.
-        print("[bench] secondary: C1 from phi to per-step residual 1e-4", file=sys.stderr,
-              flush=True)
-        with _native.HJI5D(cfg) as h:
-            t0 = time.perf_counter()
-            ok = h.run_to_residual(1e-4, 30000, 16)
-            wall = time.perf_counter() - t0
-            st = h.stats()
-        res5 = {"tolerance": 1e-4, "converged": ok, "steps": st["internal_steps"],
-                "solve_time_s": st["time"], "seconds": wall,
-                "last_residual": st["last_residual"]}
-        print("[bench] secondary: 8D contact game (seed 0)", file=sys.stderr, flush=True)
-        c2 = argparse.Namespace(seed=0, warmup=10, steps=200, to_residual=1e-4,
-                                max_sweeps=20000, cpu_sweeps=0 if args.no_cpu_baseline else 48,
-                                vstar_sweeps=args.vstar_sweeps)
-        secondary = {"hji5d_c1_to_residual": res5, "contact_game_c2": run_c2(c2)}
-        if not args.no_c4:
-            print("[bench] secondary: 8D contact game, 128 maps (C4)", file=sys.stderr,
-                  flush=True)
-            c4 = argparse.Namespace(maps=128, warmup=10, steps=100, to_residual=1e-4,
-                                    max_sweeps=20000, no_freeze=False,
-                                    cpu_sweeps=0 if args.no_cpu_baseline else 100)
-            secondary["contact_game_c4"] = run_c4(c4)
-        if not args.no_learned:
-            print("[bench] secondary: learned warm start vs plain DP", file=sys.stderr,
-                  flush=True)
-            learned = secondary["learned_warm_start_c4_grid"] = run_learned(args)
-            if "contact_game_c4" in secondary:
-                # the plain DP's time to 1e-3 of V* on the first maps of the
-                # C4 batch, from the learned line's cold run
-                cold = learned["cold"]
-                secondary["contact_game_c4"]["to_1e-3_of_vstar"] = {
-                    "maps": f"1000-{999 + args.learn_test} (batched, no freezing)",
-                    "vstar": learned["vstar"],
-                    "sweeps": cold["to_vstar"], "to_residual_1e-4": cold["to_residual"]}
-        print("[bench] secondary: dense 8D square game", file=sys.stderr, flush=True)
-        secondary["world8d_default"] = run_world8d(
-            (9, 8, 7, 7), 40, 2, 0 if args.no_cpu_baseline else 40)
-        secondary["world8d_13x12x7x7"] = run_world8d((13, 12, 7, 7), 20, 2)
-        print("[bench] secondary: 5D table reader", file=sys.stderr, flush=True)
-        secondary["reader_c1"] = run_reader_c1(cfg, cpu_queries=0 if args.no_cpu_baseline
-                                               else 1_000_000)
-        out["secondary"] = secondary
+        out["secondary"] = run_secondary(args, cfg)
     print(json.dumps(out), flush=True)
     return 0
 
