@@ -540,12 +540,15 @@ def run_learned(args):
     grid = dict(positions=9, headings=8, speeds=4)
     out = compare_batched(range(0, args.learn_train), range(100, 104),
                           range(1000, 1000 + args.learn_test), grid=grid,
-                          steps=args.learn_steps, under_weight=args.learn_under_weight, log=log)
+                          steps=args.learn_steps, under_weight=args.learn_under_weight,
+                          check_seeds=None if args.no_c4 else range(1000, 1000 + args.maps),
+                          log=log)
     out["workload"] = (f"learned warm start vs plain DP: network trained on maps "
                        f"0-{args.learn_train - 1}, delta from maps 100-103, held-out maps "
                        f"1000-{999 + args.learn_test} (the first maps of the C4 batch); C4 "
                        "grid (9x9 positions x 8 headings x 4 speeds per car); each run swept "
-                       "until every map is within 1e-3 of V* and at residual 1e-4")
+                       "until every map is within 1e-3 of V* and at residual 1e-4; then the "
+                       "whole C4 batch warm-started and judged against the plain DP")
     return out
 
 

@@ -283,6 +283,37 @@ def _targets(first, over=None):
                 batch_gpu_seconds=max(f[1] for f in chosen) if done else None)
 
 
+def check_batch(net, delta, seeds, grid, *, tol_res=1e-4, tol_err=1e-3, max_sweeps=20_000):
+    """The plain DP as the judge of a whole batch: the maps of `seeds` in one
+    handle, run to residual tol_res with converged maps frozen, once from phi
+    and once from the warm start min(phi, net + delta); per map the largest
+    difference of the two answers over its inside pairs (the judge wants at
+    most tol_err), and each run's sweeps and GPU seconds."""
+    held = _batch(list(seeds), grid)
+    tables = [g[2] for g in held]
+    masks = [dense_inside(p, e, EXIT_RADIUS) for p, e, _ in held]
+    runs = {}
+    for name in ("cold", "warm"):
+        with GameSolver(tables) as s:
+            if name == "warm":
+                s.warm_start(net, delta)
+            seconds, done, ok, stops = _run_batch(s, tol_res, max_sweeps)
+            runs[name] = dict(fields=[s.read(k) for k in range(len(tables))], seconds=seconds,
+                              sweeps=done, converged=ok, map_sweeps=stops)
+    diff = [float(np.abs(w - c)[m].max()) if m.any() else 0.0
+            for w, c, m in zip(runs["warm"]["fields"], runs["cold"]["fields"], masks)]
+    seeds = list(seeds)
+    out = dict(maps=len(tables), tolerance_residual=tol_res, tolerance_vs_plain=tol_err,
+               max_abs_diff_vs_plain=max(diff), maps_within_tolerance=int(sum(d <= tol_err
+                                                                            for d in diff)),
+               maps_over_tolerance={str(sd): d for sd, d in zip(seeds, diff) if d > tol_err})
+    for name, r in runs.items():
+        out[name] = dict(sweeps=r["sweeps"], gpu_seconds=r["seconds"], converged=r["converged"],
+                         map_sweeps_total=int(sum(k or r["sweeps"] for k in r["map_sweeps"])))
+    out["judge_ok"] = out["maps_within_tolerance"] == len(tables)
+    return out
+
+
 def accuracy_bar(tables, vstar, masks, lifts=(1.0, 0.1, 0.01), *, tol_err=1e-3, tol_res=1e-4,
                  cap=100_000, seed=0):
     """What a warm start would have to achieve before it pays: on maps whose
@@ -308,7 +339,7 @@ def accuracy_bar(tables, vstar, masks, lifts=(1.0, 0.1, 0.01), *, tol_err=1e-3, 
 def compare_batched(train_seeds, validate_seeds, test_seeds, *, grid, label_tol=1e-6,
                     label_cap=20_000, tol_err=1e-3, tol_res=1e-4, vstar_tol=1e-7,
                     vstar_cap=100_000, steps=3000, batch=8192, samples=32768, seed=0,
-                    under_weight=4.0, log=print):
+                    under_weight=4.0, check_seeds=None, log=print):
     """The learned warm start against the plain DP on batches of maps.
 
     Labels: the plain DP to residual `label_tol` (or `label_cap` sweeps) on
@@ -448,6 +479,15 @@ def compare_batched(train_seeds, validate_seeds, test_seeds, *, grid, label_tol=
     log(f"[learned] warm: batch to {tol_err:g} of V* {warm_t['to_vstar']['batch_sweeps']} "
         f"sweeps, to residual {tol_res:g} {warm_t['to_residual']['batch_sweeps']} sweeps; "
         f"start below V* by up to {worst_below:.3f}; judge {out['judge_ok']}")
+    if check_seeds is not None:
+        chk = out["batch_checked_by_plain_dp"] = check_batch(net, delta, check_seeds, grid,
+                                                              tol_res=tol_res, tol_err=tol_err)
+        chk["seeds"] = f"{min(check_seeds)}-{max(check_seeds)}"
+        log(f"[learned] batch of {chk['maps']} maps, warm and cold to residual {tol_res:g}: "
+            f"max |warm - plain| {chk['max_abs_diff_vs_plain']:.2e}, "
+            f"{chk['maps_within_tolerance']} maps within {tol_err:g}; map-sweeps "
+            f"{chk['warm']['map_sweeps_total']} warm, {chk['cold']['map_sweeps_total']} cold; "
+            f"GPU {chk['warm']['gpu_seconds']:.2f} s warm, {chk['cold']['gpu_seconds']:.2f} s cold")
     net.close()
     # The bar a network would have to clear: starts between V* and V* +
     # lift on the maps that have a V*, against the cold runs above.
