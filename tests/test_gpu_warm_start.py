@@ -142,3 +142,43 @@ def test_network_command_order_leaves_the_values(case):
         np.testing.assert_array_equal(plain.trace(), ordered.trace())
     print(f"[gpu] {case['name']}: first sweep's E side {first['index order']:.3f} ms in index "
           f"order, {first['network order']:.3f} ms in the network's order", flush=True)
+
+
+def test_constructed_starts_around_vstar(case):
+    """The judge's two facts on constructed starts (no network): a start
+    between V* and phi moves towards V* and never rises; a start lowered
+    below V* at one inside pair stays there, so the answer ends more than
+    1e-3 from the plain DP's and the judge rejects it.  The small-map case
+    only (the C2-grid case checks the same sweep)."""
+    if case["name"] != "small":
+        pytest.skip("the constructed starts run on the small maps")
+    vstar, inside = case["vstar"], case["inside"]
+    rng = np.random.default_rng(7)
+    with GameSolver(case["tables"]) as s:
+        phi = s.read()
+        lift = np.float32(0.05) * rng.random(vstar.shape, np.float32)
+        above = np.where(inside, np.minimum(phi, vstar + lift), phi).astype(np.float32)
+        lowered = above.copy()
+        cand = np.flatnonzero((inside & (vstar > -0.4)).ravel())
+        at = int(cand[len(cand) // 2])
+        lowered.ravel()[at] = vstar.ravel()[at] - np.float32(0.1)
+        for name, v0 in (("between V* and phi", above), ("one pair 0.1 below V*", lowered)):
+            s.initialize(v0)
+            gap0 = float(np.abs(v0 - vstar)[inside].max())
+            prev = v0
+            for _ in range(8):
+                s.sweep(case["chunk"], ws.FLAGS)
+                v = s.read()
+                assert (v <= prev).all(), f"{name}: a value rose"
+                prev = v
+            gap = float(np.abs(prev - vstar)[inside].max())
+            print(f"[gpu] constructed start {name}: max |V - V*| {gap0:.3e} -> {gap:.3e} "
+                  f"after {8 * case['chunk']} sweeps", flush=True)
+            if name == "between V* and phi":
+                # V* is the plain field at residual 1e-6: later sweeps may
+                # still lower it by the remaining tail, far below 2e-3 here
+                assert (prev >= vstar - 2e-3)[inside].all()
+                assert gap <= gap0
+            else:
+                assert prev.ravel()[at] <= lowered.ravel()[at]
+                assert gap >= 0.1 - 1e-6
