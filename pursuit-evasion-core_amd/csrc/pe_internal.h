@@ -86,7 +86,7 @@ struct pe_sl {
   // so each inside pair records its argmin / argmax commands (an order
   // probe into scratch buffers; the handle's fields are not touched)
   bool probe_orders = false;
-  // residual ring [slots][B] and graph slots [graph_sweeps][B]
+  // residual ring [slots][B] and graph slots [graph_sweeps[0]][B]
   unsigned long long *ring = nullptr, *gslots = nullptr;
   int ring_slots = 256, ring_used = 0;
   std::vector<double> trace;  // [sweeps][B]
@@ -98,10 +98,13 @@ struct pe_sl {
   double ms[2] = {0, 0};
   long long launches[2] = {0, 0};
   hipEvent_t t0 = nullptr, t1 = nullptr;
-  // captured sweeps
-  hipGraphExec_t graph = nullptr;
-  int graph_sweeps = 0, graph_flags = -1;
-  const float *graph_src = nullptr;  // field the captured sweeps start from
+  // captured sweeps: a long graph replayed while enough sweeps remain, then
+  // a short one (both even, so the value buffers end where they started)
+  static constexpr int kGraphs = 2;
+  hipGraphExec_t graph[kGraphs] = {nullptr, nullptr};
+  int graph_sweeps[kGraphs] = {16, 2};
+  int graph_flags[kGraphs] = {-1, -1};
+  const float *graph_src[kGraphs] = {nullptr, nullptr};  // field the sweeps start from
 };
 
 struct pe_mlp {

@@ -154,19 +154,21 @@ static int sl_flush(pe_sl *h) {
   return PE_OK;
 }
 
-static int build_graph(pe_sl *h, int flags) {
-  if (h->graph && h->graph_flags == flags && h->graph_src == h->va) return PE_OK;
-  if (h->graph) {
-    hipGraphExecDestroy(h->graph);
-    h->graph = nullptr;
+static int build_graph(pe_sl *h, int flags, int gi) {
+  hipGraphExec_t &exec = h->graph[gi];
+  const int sweeps = h->graph_sweeps[gi];
+  if (exec && h->graph_flags[gi] == flags && h->graph_src[gi] == h->va) return PE_OK;
+  if (exec) {
+    hipGraphExecDestroy(exec);
+    exec = nullptr;
   }
   hipGraph_t g;
   PE_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
   // Zero the graph's residual slots with a kernel (no memset node).
-  const long long n = (long long)h->graph_sweeps * h->B;
+  const long long n = (long long)sweeps * h->B;
   hipLaunchKernelGGL(sl_fill_u64, dim3((unsigned)((n + 255) / 256)), dim3(256),
                      0, h->stream, h->gslots, n);
-  for (int s = 0; s < h->graph_sweeps; s++) {
+  for (int s = 0; s < sweeps; s++) {
     const bool even = (s % 2) == 0;
     enqueue_sweep(h, even ? h->va : h->vb, even ? h->vb : h->va, flags,
                   h->gslots + (long long)s * h->B, false);
@@ -176,35 +178,46 @@ static int build_graph(pe_sl *h, int flags) {
     (void)hipGetLastError();
     return fail(PE_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(end));
   }
-  hipError_t inst = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
+  hipError_t inst = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
   hipGraphDestroy(g);
   if (inst != hipSuccess) {
     (void)hipGetLastError();
-    h->graph = nullptr;
+    exec = nullptr;
     return fail(PE_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(inst));
   }
-  h->graph_flags = flags;
-  h->graph_src = h->va;
+  h->graph_flags[gi] = flags;
+  h->graph_src[gi] = h->va;
   return PE_OK;
+}
+
+static void drop_graphs(pe_sl *h) {
+  for (hipGraphExec_t &g : h->graph)
+    if (g) {
+      hipGraphExecDestroy(g);
+      g = nullptr;
+    }
 }
 
 static int do_sweeps(pe_sl *h, int count, int flags) {
   PE_HIP(hipEventRecord(h->t0, h->stream));
   int done = 0;
   if ((flags & PE_SL_GRAPH) && !h->profiling) {
-    int rc = build_graph(h, flags & ~PE_SL_GRAPH);
-    if (rc) return rc;
-    while (count - done >= h->graph_sweeps) {
-      if (h->ring_used + h->graph_sweeps > h->ring_slots) {
-        rc = sl_flush(h);
-        if (rc) return rc;
+    for (int gi = 0; gi < pe_sl::kGraphs; gi++) {
+      const int sweeps = h->graph_sweeps[gi];
+      if (count - done < sweeps) continue;
+      int rc = build_graph(h, flags & ~PE_SL_GRAPH, gi);
+      if (rc) return rc;
+      while (count - done >= sweeps) {
+        if (h->ring_used + sweeps > h->ring_slots) {
+          rc = sl_flush(h);
+          if (rc) return rc;
+        }
+        PE_HIP(hipGraphLaunch(h->graph[gi], h->stream));
+        PE_HIP(hipMemcpyAsync(h->ring + (long long)h->ring_used * h->B, h->gslots,
+                              (size_t)sweeps * h->B * 8, hipMemcpyDeviceToDevice, h->stream));
+        h->ring_used += sweeps;
+        done += sweeps;
       }
-      PE_HIP(hipGraphLaunch(h->graph, h->stream));
-      PE_HIP(hipMemcpyAsync(h->ring + (long long)h->ring_used * h->B, h->gslots,
-                            (size_t)h->graph_sweeps * h->B * 8,
-                            hipMemcpyDeviceToDevice, h->stream));
-      h->ring_used += h->graph_sweeps;
-      done += h->graph_sweeps;
     }
   }
   for (; done < count; done++) {
@@ -374,7 +387,6 @@ extern "C" int pe_sl_create_ex(const pe_sl_game *games, int32_t n_maps, int32_t 
   }
   h->pairs = voff;
   h->g_size = goff;
-  h->graph_sweeps = 2;
   auto cleanup = [&](int code) {
     pe_sl_destroy(h);
     return code;
@@ -562,7 +574,7 @@ extern "C" int pe_sl_create_ex(const pe_sl_game *games, int32_t n_maps, int32_t 
     PE_HIP(hipMalloc((void **)&h->order, h->pairs));
     PE_HIP(hipMalloc((void **)&h->G, h->esize * h->g_size));
     PE_HIP(hipMalloc((void **)&h->ring, 8ll * h->ring_slots * n_maps));
-    PE_HIP(hipMalloc((void **)&h->gslots, 8ll * h->graph_sweeps * n_maps));
+    PE_HIP(hipMalloc((void **)&h->gslots, 8ll * h->graph_sweeps[0] * n_maps));
     hipLaunchKernelGGL(sl_fill_f32, dim3(4096), dim3(256), 0, h->stream, h->G,
                        h->g_size * (long long)(h->esize / sizeof(float)));
     PE_LAUNCHED(sl_fill_f32);
@@ -818,10 +830,9 @@ extern "C" int pe_sl_set_lds_threads(pe_sl *h, int32_t threads) {
   if (!h) return fail(PE_ERR_INVALID, "null handle");
   if (threads != 256 && threads != 512 && threads != 1024)
     return fail(PE_ERR_INVALID, "LDS E-side blocks have 256, 512 or 1024 threads");
-  if (h->graph) {  // captured launches carry the old block size
+  if (h->graph[0] || h->graph[1]) {  // captured launches carry the old block size
     PE_HIP(hipStreamSynchronize(h->stream));
-    PE_HIP(hipGraphExecDestroy(h->graph));
-    h->graph = nullptr;
+    drop_graphs(h);
   }
   h->lds_threads = threads;
   return PE_OK;
@@ -831,7 +842,7 @@ extern "C" void pe_sl_destroy(pe_sl *h) {
   if (!h) return;
   hipSetDevice(h->device);
   if (h->stream) hipStreamSynchronize(h->stream);
-  if (h->graph) hipGraphExecDestroy(h->graph);
+  drop_graphs(h);
   for (hipEvent_t e : h->events) hipEventDestroy(e);
   for (void *p : {(void *)h->d_maps, (void *)h->pids, (void *)h->eids, (void *)h->erec,
                   (void *)h->ilist, (void *)h->ioff, (void *)h->nlist, (void *)h->noff,
