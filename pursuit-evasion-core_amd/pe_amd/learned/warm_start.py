@@ -355,7 +355,10 @@ def compare_batched(train_seeds, validate_seeds, test_seeds, *, grid, label_tol=
     tol_res: sweeps and GPU seconds of each map and of the batch to both
     targets.  The judge: the warm start is nowhere more than tol_err below
     V* (a value below V* stays below it), and the warm run reaches tol_err
-    of V* on every map that has one."""
+    of V* on every map that has one.  With `check_seeds`, the maps of those
+    seeds are then warm-started as one batch and judged against the plain
+    DP run beside them (check_batch); last, the starts between V* and V* +
+    lift on the held-out maps with a V* (accuracy_bar)."""
     out = dict(train_maps=len(train_seeds), validate_maps=len(validate_seeds),
                test_maps=len(test_seeds), tolerance_vs_vstar=tol_err,
                tolerance_residual=tol_res)
