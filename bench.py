@@ -572,6 +572,47 @@ def run_secondary(args, cfg):
                 "solve_time_s": st["time"], "seconds": wall,
                 "last_residual": st["last_residual"]}
 
+    def c1_plane_ab():
+        # The two-plane blend kernel (default for an even heading count)
+        # against the one-plane kernel (PE_HJI5D_PSI_SINGLE) in one process:
+        # both handles start from phi, then take the same steps in
+        # alternating chunks, so the clock state is shared.
+        import numpy as np
+
+        single_cfg = type(cfg).from_buffer_copy(cfg)
+        single_cfg.flags |= _native.PE_HJI5D_PSI_SINGLE
+        chunk, rounds = 40, 4
+        with _native.HJI5D(cfg) as pair, _native.HJI5D(single_cfg) as single:
+            dt = pair.stats()["cfl_dt_limit"]
+            handles = {"two_planes": pair, "one_plane": single}
+            for h in handles.values():
+                h.step(dt, 5)
+            times = {k: [] for k in handles}
+            for _ in range(rounds):
+                for k, h in handles.items():
+                    t0 = time.perf_counter()
+                    h.step(dt, chunk)
+                    times[k].append((time.perf_counter() - t0) / chunk * 1e3)
+            out = {"chunk_steps": chunk, "rounds": rounds}
+            for k, h in handles.items():
+                h.set_profiling(True)
+                h.step(dt, 20)
+                st = h.stats()
+                avg = [st["stage_ms"][j] / max(1, st["stage_launches"][j]) for j in range(2)]
+                out[k] = {"ms_per_step_chunks": [round(t, 4) for t in times[k]],
+                          "ms_per_step_median": round(float(np.median(times[k])), 4),
+                          "stage1_launch_ms": round(avg[0], 5),
+                          "blend_launch_ms": round(avg[1], 5)}
+            f_pair, a_pair = pair.read()
+            f_single, a_single = single.read()
+            out["bit_identical"] = bool(np.array_equal(f_pair.view(np.uint32),
+                                                       f_single.view(np.uint32))
+                                        and np.array_equal(a_pair.view(np.uint32),
+                                                           a_single.view(np.uint32)))
+        out["speedup_two_planes"] = round(out["one_plane"]["ms_per_step_median"]
+                                          / out["two_planes"]["ms_per_step_median"], 4)
+        return out
+
     def c2_line():
         c2 = argparse.Namespace(seed=0, warmup=10, steps=200, to_residual=1e-4,
                                 max_sweeps=20000, cpu_sweeps=0 if args.no_cpu_baseline else 48,
@@ -597,6 +638,7 @@ def run_secondary(args, cfg):
         return learned
 
     lines = [("hji5d_c1_to_residual", "C1 from phi to per-step residual 1e-4", c1_to_residual),
+             ("hji5d_c1_plane_ab", "C1 two-plane against one-plane kernel", c1_plane_ab),
              ("contact_game_c2", "8D contact game (seed 0)", c2_line)]
     if not args.no_c4:
         lines.append(("contact_game_c4", "8D contact game, 128 maps (C4)", c4_line))
