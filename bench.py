@@ -42,8 +42,8 @@ _HOST_CORES = None
 def host_cores():
     """(threads for the CPU baselines, physical cores this process may run
     on, logical CPUs).  The threads are the physical cores, capped by
-    OMP_NUM_THREADS when the environment sets it (a GPU box grants each GPU
-    a share of its host cores that way).  Read once, at the first call:
+    OMP_NUM_THREADS when the environment sets it (a shared host may grant
+    each GPU a share of its cores that way).  Read once, at the first call:
     with OMP_PROC_BIND=close the first OpenMP region binds the calling
     thread to one core, and the affinity mask read after it has one CPU."""
     global _HOST_CORES
